@@ -1,0 +1,906 @@
+// C ABI: interaction constants, the host pair search and the single-system GPU
+// task object that mirrors the reference's Nbnxm::gpu_* API
+// (src/gromacs/nbnxm/nbnxm_gpu.h:66-302, src/gromacs/nbnxm/gpu_data_mgmt.h).
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "capi_common.h"
+#include "nbnxm_kernels.h"
+#include "pairsearch.h"
+
+namespace nbnxm_amd
+{
+
+static thread_local std::string t_lastError;
+
+void setError(const std::string& msg)
+{
+    t_lastError = msg;
+}
+
+int fail(int code, const std::string& msg)
+{
+    t_lastError = msg;
+    return code;
+}
+
+int hipFail(hipError_t e, const char* what)
+{
+    t_lastError = std::string(what) + ": " + hipGetErrorString(e);
+    return NBNXM_ERR_HIP;
+}
+
+int validateAtomdata(const nbnxm_atomdata_t* nbat)
+{
+    if (!nbat || nbat->numAtoms < 0 || nbat->numAtoms % NBNXM_CLUSTER_SIZE != 0)
+    {
+        return fail(NBNXM_ERR_INVALID, "atomdata: numAtoms must be a non-negative multiple of 8");
+    }
+    if (nbat->numAtoms > 0 && !nbat->xq)
+    {
+        return fail(NBNXM_ERR_INVALID, "atomdata: xq is null");
+    }
+    if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        if (nbat->numTypes < 1 || !nbat->nbfp || (nbat->numAtoms > 0 && !nbat->type))
+        {
+            return fail(NBNXM_ERR_INVALID, "atomdata: type table kernel needs numTypes, type and nbfp");
+        }
+        for (int i = 0; i < nbat->numAtoms; i++)
+        {
+            if (nbat->type[i] < 0 || nbat->type[i] >= nbat->numTypes)
+            {
+                return fail(NBNXM_ERR_INVALID, "atomdata: type of slot " + std::to_string(i) + " out of range");
+            }
+        }
+    }
+    else if (nbat->vdwType == NBNXM_VDW_CUT_COMB_GEOM || nbat->vdwType == NBNXM_VDW_CUT_COMB_LB)
+    {
+        if (nbat->numAtoms > 0 && !nbat->ljComb)
+        {
+            return fail(NBNXM_ERR_INVALID, "atomdata: combination-rule kernel needs ljComb");
+        }
+    }
+    else
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "atomdata: unknown vdwType");
+    }
+    return NBNXM_OK;
+}
+
+int validatePairlist(const nbnxm_pairlist_gpu_t* list, int numAtoms)
+{
+    if (!list || list->na_ci != NBNXM_CLUSTER_SIZE || list->nsci < 0 || list->ncjPacked < 0 || list->nexcl < 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "pairlist: bad sizes (na_ci must be 8, nexcl >= 1)");
+    }
+    if ((list->nsci > 0 && !list->sci) || (list->ncjPacked > 0 && !list->cjPacked) || !list->excl)
+    {
+        return fail(NBNXM_ERR_INVALID, "pairlist: null array");
+    }
+    const int numClusters = numAtoms / NBNXM_CLUSTER_SIZE;
+    for (int i = 0; i < list->nsci; i++)
+    {
+        const nbnxm_sci_t& s = list->sci[i];
+        if (s.shift < 0 || s.shift >= NBNXM_NUM_SHIFTS || s.sci < 0 || (s.sci + 1) * NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER > numClusters
+            || s.cjPackedBegin < 0 || s.cjPackedEnd < s.cjPackedBegin || s.cjPackedEnd > list->ncjPacked)
+        {
+            return fail(NBNXM_ERR_INVALID, "pairlist: sci entry " + std::to_string(i) + " out of range");
+        }
+    }
+    for (int g = 0; g < list->ncjPacked; g++)
+    {
+        const nbnxm_cj_packed_t& p = list->cjPacked[g];
+        if (p.imei[0].excl_ind < 0 || p.imei[0].excl_ind >= list->nexcl)
+        {
+            return fail(NBNXM_ERR_INVALID, "pairlist: excl_ind out of range in group " + std::to_string(g));
+        }
+        for (int k = 0; k < NBNXM_JGROUP_SIZE; k++)
+        {
+            const bool used = (p.imei[0].imask >> (k * NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER)) & 0xFFu;
+            if ((used || p.cj[k] != 0) && (p.cj[k] < 0 || p.cj[k] >= numClusters))
+            {
+                return fail(NBNXM_ERR_INVALID, "pairlist: cj out of range in group " + std::to_string(g));
+            }
+        }
+    }
+    return NBNXM_OK;
+}
+
+} // namespace nbnxm_amd
+
+using namespace nbnxm_amd;
+
+// ---------------------------------------------------------------------------
+// The single-system GPU task (NbnxmGpu, src/gromacs/nbnxm/cuda/nbnxm_cuda_types.h:67-138)
+// ---------------------------------------------------------------------------
+struct nbnxm_gpu
+{
+    hipStream_t stream = nullptr;
+    nbnxm_interaction_const_t ic{};
+    nbnxm_pairlist_params_t listParams{};
+    int accumulation = NBNXM_ACCUM_FIXED64;
+    int vdwType = NBNXM_VDW_CUT_TYPE_TABLE;
+    int numAtoms = 0;
+    int numTypes = 0;
+    NbLaunchConfig launchConfig;
+
+    DeviceBuffer<float4> xq;
+    DeviceBuffer<int> types;
+    DeviceBuffer<float2> ljComb;
+    DeviceBuffer<float2> nbfp;
+    DeviceBuffer<float4> shiftVec;
+    DeviceBuffer<float4> paramsA;
+    DeviceBuffer<float2> paramsB;
+    DeviceBuffer<double> paramsD;
+
+    int nsci = 0;
+    int ncjPacked = 0;
+    DeviceBuffer<nbnxm_sci_t> sci;
+    DeviceBuffer<int> sciOrder;
+    DeviceBuffer<nbnxm_cj_packed_t> cjPacked;
+    DeviceBuffer<nbnxm_excl_t> excl;
+    DeviceBuffer<unsigned int> imaskOuter;
+    DeviceBuffer<int> rollingPart;
+    DeviceBuffer<int> sciCount;
+    DeviceBuffer<int> sciCountSorted;
+    DeviceBuffer<int> sciHistogram;
+    DeviceBuffer<int> iota;
+    DeviceBuffer<char> sortTemp;
+    bool haveFreshList = false;
+    bool haveSortedOrder = false;
+
+    DeviceBuffer<float> fFloat;
+    DeviceBuffer<unsigned long long> fFixed;
+    DeviceBuffer<float> fOut;
+    DeviceBuffer<double> fOutDouble;
+    DeviceBuffer<double> partials;
+    DeviceBuffer<double> energies;
+    DeviceBuffer<double> fshift;
+    DeviceBuffer<int> entryRanges;
+    DeviceBuffer<int> entryList;
+    DeviceBuffer<int> shiftRanges;
+    DeviceBuffer<int> shiftEntryList;
+
+    double* hEnergies = nullptr; // pinned, 2
+    double* hFshift = nullptr;   // pinned, 3*45
+    bool lastComputeEnergy = false;
+    bool lastComputeVirial = false;
+
+    ~nbnxm_gpu()
+    {
+        if (hEnergies)
+        {
+            (void)hipHostFree(hEnergies);
+        }
+        if (hFshift)
+        {
+            (void)hipHostFree(hFshift);
+        }
+        if (stream)
+        {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace
+{
+
+int uploadParams(nbnxm_gpu* nb)
+{
+    const float rc = static_cast<float>(nb->ic.rcoulomb);
+    const float4 pa = make_float4(rc * rc, static_cast<float>(nb->ic.epsfac), static_cast<float>(nb->ic.ewaldcoeff_q), static_cast<float>(nb->ic.sh_ewald));
+    const float2 pb = make_float2(static_cast<float>(nb->ic.repulsion_shift_cpot), static_cast<float>(nb->ic.dispersion_shift_cpot));
+    NBNXM_HIP_CHECK(nb->paramsA.resize(1));
+    NBNXM_HIP_CHECK(nb->paramsB.resize(1));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsA.ptr, &pa, sizeof(pa), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsB.ptr, &pb, sizeof(pb), hipMemcpyHostToDevice, nb->stream));
+    const double pd[8] = { nb->ic.rcoulomb * nb->ic.rcoulomb, nb->ic.epsfac, nb->ic.ewaldcoeff_q, nb->ic.sh_ewald,
+                           nb->ic.repulsion_shift_cpot, nb->ic.dispersion_shift_cpot, 0.0, 0.0 };
+    NBNXM_HIP_CHECK(nb->paramsD.resize(8));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsD.ptr, pd, sizeof(pd), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    return NBNXM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* nbnxm_last_error(void)
+{
+    return t_lastError.c_str();
+}
+
+const char* nbnxm_version(void)
+{
+    return "nbnxm_amd 0.1 gfx950";
+}
+
+double nbnxm_calc_ewaldcoeff_q(double rc, double rtol)
+{
+    // Same doubling-then-bisection as calc_ewaldcoeff_q (src/gromacs/ewald/ewald_utils.cpp:43-71).
+    double beta = 5.0;
+    int i = 0;
+    do
+    {
+        i++;
+        beta *= 2.0;
+    } while (std::erfc(beta * rc) > rtol);
+    const int n = i + 60;
+    double low = 0.0, high = beta;
+    for (int k = 0; k < n; k++)
+    {
+        beta = 0.5 * (low + high);
+        if (std::erfc(beta * rc) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
+int nbnxm_init_interaction_const(double rc, double ewaldRtol, double epsilonR, nbnxm_interaction_const_t* ic)
+{
+    if (!ic || !(rc > 0) || !(ewaldRtol > 0) || !(ewaldRtol < 1))
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const: need rc > 0 and 0 < rtol < 1");
+    }
+    ic->rcoulomb = rc;
+    ic->rvdw = rc;
+    ic->ewaldcoeff_q = nbnxm_calc_ewaldcoeff_q(rc, ewaldRtol);
+    ic->sh_ewald = std::erfc(ic->ewaldcoeff_q * rc) / rc;
+    ic->epsfac = (epsilonR != 0) ? one4PiEps0() / epsilonR : 0.0;
+    ic->dispersion_shift_cpot = -1.0 / std::pow(rc, 6);
+    ic->repulsion_shift_cpot = -1.0 / std::pow(rc, 12);
+    return NBNXM_OK;
+}
+
+int nbnxm_calc_shifts(const double box[3], float* shiftVec)
+{
+    if (!box || !shiftVec)
+    {
+        return fail(NBNXM_ERR_INVALID, "calc_shifts: null pointer");
+    }
+    for (int tz = -1, k = 0; tz <= 1; tz++)
+    {
+        for (int ty = -1; ty <= 1; ty++)
+        {
+            for (int tx = -2; tx <= 2; tx++, k++)
+            {
+                shiftVec[3 * k + 0] = static_cast<float>(tx) * static_cast<float>(box[0]);
+                shiftVec[3 * k + 1] = static_cast<float>(ty) * static_cast<float>(box[1]);
+                shiftVec[3 * k + 2] = static_cast<float>(tz) * static_cast<float>(box[2]);
+            }
+        }
+    }
+    return NBNXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pair search (producer of the list and of the grid-order atom data)
+// ---------------------------------------------------------------------------
+struct nbnxm_search
+{
+    GridSearch grid;
+};
+
+nbnxm_search_t* nbnxm_search_create(void)
+{
+    return new (std::nothrow) nbnxm_search();
+}
+
+void nbnxm_search_free(nbnxm_search_t* s)
+{
+    delete s;
+}
+
+int nbnxm_search_put_atoms_on_grid(nbnxm_search_t* s, const double box[3], int numAtoms, const double* x)
+{
+    if (!s)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null handle");
+    }
+    const int rc = s->grid.putAtomsOnGrid(box, numAtoms, x);
+    return rc == NBNXM_OK ? rc : fail(rc, "search: invalid box or coordinates");
+}
+
+int nbnxm_search_construct_pairlist(nbnxm_search_t* s, double rlist, const int* exclRanges, const int* exclElements, int maxPairsPerEntry)
+{
+    if (!s)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null handle");
+    }
+    const int rc = s->grid.constructPairlist(rlist, exclRanges, exclElements, maxPairsPerEntry);
+    return rc == NBNXM_OK ? rc : fail(rc, "search: every box edge must exceed rlist > 0, exclusions need ranges and elements");
+}
+
+int nbnxm_search_get_pairlist(const nbnxm_search_t* s, nbnxm_pairlist_gpu_t* list)
+{
+    if (!s || !list)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    *list = s->grid.listView();
+    return NBNXM_OK;
+}
+
+int nbnxm_search_num_grid_atoms(const nbnxm_search_t* s)
+{
+    return s ? s->grid.numGridAtoms() : NBNXM_ERR_INVALID;
+}
+
+int nbnxm_search_atom_indices(const nbnxm_search_t* s, int* atomIndices)
+{
+    if (!s || !atomIndices)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    std::copy(s->grid.atomIndices().begin(), s->grid.atomIndices().end(), atomIndices);
+    return NBNXM_OK;
+}
+
+int nbnxm_search_shift_vec(const nbnxm_search_t* s, float* shiftVec)
+{
+    if (!s || !shiftVec)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    std::memcpy(shiftVec, s->grid.shiftVec(), sizeof(float) * 3 * NBNXM_NUM_SHIFTS);
+    return NBNXM_OK;
+}
+
+int nbnxm_search_fill_xq(const nbnxm_search_t* s, const double* x, const double* q, float* xq)
+{
+    if (!s || !xq)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    s->grid.fillXq(x, q, xq);
+    return NBNXM_OK;
+}
+
+int nbnxm_search_fill_types(const nbnxm_search_t* s, const int* type, int dummyType, int* slotType)
+{
+    if (!s || !type || !slotType)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    s->grid.fillTypes(type, dummyType, slotType);
+    return NBNXM_OK;
+}
+
+int nbnxm_search_fill_ljcomb(const nbnxm_search_t* s, const int* type, const float* perTypeComb, float* slotComb)
+{
+    if (!s || !type || !perTypeComb || !slotComb)
+    {
+        return fail(NBNXM_ERR_INVALID, "search: null pointer");
+    }
+    s->grid.fillLjComb(type, perTypeComb, slotComb);
+    return NBNXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GPU task
+// ---------------------------------------------------------------------------
+int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
+                   const nbnxm_pairlist_params_t* listParams,
+                   const nbnxm_atomdata_t* nbat,
+                   int accumulation,
+                   nbnxm_gpu_t** out)
+{
+    if (!ic || !listParams || !nbat || !out)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init: null pointer");
+    }
+    *out = nullptr;
+    if (accumulation < NBNXM_ACCUM_FLOAT || accumulation > NBNXM_ACCUM_FP64)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init: unknown accumulation mode");
+    }
+    if (std::fabs(ic->rvdw - ic->rcoulomb) > 1e-12 * ic->rcoulomb)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "gpu_init: rvdw must equal rcoulomb (no twin cut-off kernel)");
+    }
+    if (!(listParams->rlistOuter >= ic->rcoulomb) || !(listParams->rlistInner >= ic->rcoulomb)
+        || listParams->rlistInner > listParams->rlistOuter || listParams->numRollingPruningParts < 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init: need rc <= rlistInner <= rlistOuter and numRollingPruningParts >= 1");
+    }
+    int rc = validateAtomdata(nbat);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    int device = 0;
+    NBNXM_HIP_CHECK(hipGetDevice(&device));
+    std::unique_ptr<nbnxm_gpu> nb(new (std::nothrow) nbnxm_gpu());
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init: out of host memory");
+    }
+    NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&nb->stream, hipStreamNonBlocking));
+    NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hEnergies), 2 * sizeof(double), hipHostMallocDefault));
+    NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hFshift), 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipHostMallocDefault));
+    nb->ic = *ic;
+    nb->listParams = *listParams;
+    nb->accumulation = accumulation;
+    rc = uploadParams(nb.get());
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    NBNXM_HIP_CHECK(nb->energies.resize(2));
+    NBNXM_HIP_CHECK(nb->fshift.resize(3 * NBNXM_NUM_SHIFTS));
+    rc = nbnxm_gpu_init_atomdata(nb.get(), nbat);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    if (nbat->shiftVec)
+    {
+        rc = nbnxm_gpu_upload_shiftvec(nb.get(), nbat->shiftVec);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    *out = nb.release();
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init_atomdata: null handle");
+    }
+    int rc = validateAtomdata(nbat);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    const int n = nbat->numAtoms;
+    nb->numAtoms = n;
+    nb->vdwType = nbat->vdwType;
+    nb->numTypes = nbat->numTypes;
+    NBNXM_HIP_CHECK(nb->xq.resize(n));
+    NBNXM_HIP_CHECK(nb->fFixed.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fFloat.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fOut.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fOutDouble.resize(3 * static_cast<size_t>(n)));
+    if (n > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->xq.ptr, nbat->xq, sizeof(float4) * n, hipMemcpyHostToDevice, nb->stream));
+    }
+    if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        NBNXM_HIP_CHECK(nb->types.resize(n));
+        NBNXM_HIP_CHECK(nb->nbfp.resize(static_cast<size_t>(nbat->numTypes) * nbat->numTypes));
+        if (n > 0)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->types.ptr, nbat->type, sizeof(int) * n, hipMemcpyHostToDevice, nb->stream));
+        }
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->nbfp.ptr, nbat->nbfp, sizeof(float2) * nbat->numTypes * nbat->numTypes, hipMemcpyHostToDevice, nb->stream));
+    }
+    else
+    {
+        NBNXM_HIP_CHECK(nb->ljComb.resize(n));
+        if (n > 0)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ljComb.ptr, nbat->ljComb, sizeof(float2) * n, hipMemcpyHostToDevice, nb->stream));
+        }
+    }
+    NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
+    // Host buffers of the caller may be reused after return.
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_upload_shiftvec(nbnxm_gpu_t* nb, const float* shiftVec)
+{
+    if (!nb || !shiftVec)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_upload_shiftvec: null pointer");
+    }
+    float4 sv[NBNXM_NUM_SHIFTS];
+    for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
+    {
+        sv[k] = make_float4(shiftVec[3 * k], shiftVec[3 * k + 1], shiftVec[3 * k + 2], 0.0f);
+    }
+    NBNXM_HIP_CHECK(nb->shiftVec.resize(NBNXM_NUM_SHIFTS));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftVec.ptr, sv, sizeof(sv), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init_pairlist: null handle");
+    }
+    int rc = validatePairlist(list, nb->numAtoms);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    if (!(list->excl[0].pair[0] == 0xFFFFFFFFu))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init_pairlist: excl[0] must be the all-interaction mask");
+    }
+    const int nsci = list->nsci;
+    const int ncj = list->ncjPacked;
+    nb->nsci = nsci;
+    nb->ncjPacked = ncj;
+    NBNXM_HIP_CHECK(nb->sci.resize(nsci));
+    NBNXM_HIP_CHECK(nb->cjPacked.resize(ncj));
+    NBNXM_HIP_CHECK(nb->excl.resize(list->nexcl));
+    NBNXM_HIP_CHECK(nb->imaskOuter.resize(ncj));
+    NBNXM_HIP_CHECK(nb->sciOrder.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciCount.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciCountSorted.resize(nsci));
+    NBNXM_HIP_CHECK(nb->iota.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciHistogram.resize(NBNXM_SCI_HISTOGRAM_SIZE));
+    NBNXM_HIP_CHECK(nb->partials.resize(8 * static_cast<size_t>(std::max(nsci, 1))));
+    const int wavesPerBlockPrune = 4;
+    const int rollingSize = ((nsci + wavesPerBlockPrune - 1) / wavesPerBlockPrune + 1) * wavesPerBlockPrune;
+    NBNXM_HIP_CHECK(nb->rollingPart.resize(rollingSize));
+
+    if (nsci > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sci.ptr, list->sci, sizeof(nbnxm_sci_t) * nsci, hipMemcpyHostToDevice, nb->stream));
+    }
+    if (ncj > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->cjPacked.ptr, list->cjPacked, sizeof(nbnxm_cj_packed_t) * ncj, hipMemcpyHostToDevice, nb->stream));
+    }
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->excl.ptr, list->excl, sizeof(nbnxm_excl_t) * list->nexcl, hipMemcpyHostToDevice, nb->stream));
+
+    // Identity work order and the per-shift entry lists for the FP64 reductions.
+    std::vector<int> iota(nsci);
+    std::iota(iota.begin(), iota.end(), 0);
+    std::vector<int> shiftRanges(NBNXM_NUM_SHIFTS + 1, 0);
+    for (int i = 0; i < nsci; i++)
+    {
+        shiftRanges[list->sci[i].shift + 1]++;
+    }
+    for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
+    {
+        shiftRanges[k + 1] += shiftRanges[k];
+    }
+    std::vector<int> shiftList(nsci);
+    {
+        std::vector<int> fillPos(shiftRanges.begin(), shiftRanges.end() - 1);
+        for (int i = 0; i < nsci; i++)
+        {
+            shiftList[fillPos[list->sci[i].shift]++] = i;
+        }
+    }
+    const int entryRanges[2] = { 0, nsci };
+    NBNXM_HIP_CHECK(nb->entryRanges.resize(2));
+    NBNXM_HIP_CHECK(nb->entryList.resize(std::max(nsci, 1)));
+    NBNXM_HIP_CHECK(nb->shiftRanges.resize(NBNXM_NUM_SHIFTS + 1));
+    NBNXM_HIP_CHECK(nb->shiftEntryList.resize(std::max(nsci, 1)));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryRanges.ptr, entryRanges, sizeof(entryRanges), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftRanges.ptr, shiftRanges.data(), sizeof(int) * shiftRanges.size(), hipMemcpyHostToDevice, nb->stream));
+    if (nsci > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryList.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sciOrder.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->iota.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+    }
+    NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, rollingSize, 0, nb->stream));
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->partials.ptr), 2 * 8 * static_cast<size_t>(std::max(nsci, 1)), 0, nb->stream));
+
+    // Radix-sort scratch for the sci ordering.
+    size_t tempBytes = 0;
+    NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, std::max(nsci, 1), 0, 14, nb->stream));
+    NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(tempBytes, 1)));
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    nb->haveFreshList = true;
+    nb->haveSortedOrder = false;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_copy_xq_to_gpu(nbnxm_gpu_t* nb, const float* xq, int numAtoms)
+{
+    if (!nb || (!xq && numAtoms > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_copy_xq_to_gpu: null pointer");
+    }
+    if (numAtoms != nb->numAtoms)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_copy_xq_to_gpu: atom count differs from gpu_init_atomdata");
+    }
+    if (numAtoms > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->xq.ptr, xq, sizeof(float4) * numAtoms, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_clear_outputs(nbnxm_gpu_t* nb, int computeVirial)
+{
+    (void)computeVirial;
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_clear_outputs: null handle");
+    }
+    NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(nb->numAtoms), nb->stream));
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(nb->numAtoms), 0, nb->stream));
+    return NBNXM_OK;
+}
+
+static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
+{
+    PruneArgs pa;
+    pa.xq = nb->xq.ptr;
+    pa.shiftVec = nb->shiftVec.ptr;
+    pa.sci = nb->sci.ptr;
+    pa.sciOrder = (nb->haveFreshList || !nb->haveSortedOrder) ? nullptr : nb->sciOrder.ptr;
+    pa.nsci = nb->nsci;
+    pa.cjPacked = nb->cjPacked.ptr;
+    pa.imaskOuter = nb->imaskOuter.ptr;
+    pa.rollingPart = nb->rollingPart.ptr;
+    pa.sciCount = nb->sciCount.ptr;
+    pa.sciHistogram = nb->sciHistogram.ptr;
+    const float ro = static_cast<float>(nb->listParams.rlistOuter);
+    const float ri = static_cast<float>(nb->listParams.rlistInner);
+    pa.rlistOuter_sq = ro * ro;
+    pa.rlistInner_sq = ri * ri;
+    pa.numParts = numParts;
+    const bool fresh = nb->haveFreshList;
+    if (fresh)
+    {
+        if (numParts != 1)
+        {
+            return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel_pruneonly: a fresh list is pruned in one part");
+        }
+        NBNXM_HIP_CHECK(launchFillI32(nb->sciHistogram.ptr, NBNXM_SCI_HISTOGRAM_SIZE, 0, nb->stream));
+    }
+    NBNXM_HIP_CHECK(launchPrune(pa, fresh, nb->stream));
+    if (fresh)
+    {
+        // Order entries by descending pruned pair count (the role of
+        // nbnxnKernelBucketSciSort, src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_sci_sort.cuh:66-87);
+        // a stable radix sort on the histogram bin keeps the order deterministic.
+        if (nb->nsci > 0)
+        {
+            size_t tempBytes = nb->sortTemp.size;
+            NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nb->sortTemp.ptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, nb->nsci, 0, 14, nb->stream));
+        }
+        nb->haveSortedOrder = true;
+        nb->haveFreshList = false;
+        // Rolling pruning restarts from part 0 after a fresh list.
+        NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, nb->rollingPart.size, 0, nb->stream));
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_launch_kernel_pruneonly(nbnxm_gpu_t* nb, int numParts)
+{
+    if (!nb || numParts < 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel_pruneonly: null handle or numParts < 1");
+    }
+    if (nb->nsci == 0 && nb->ncjPacked == 0 && !nb->sci.ptr)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_launch_kernel_pruneonly: no pair list uploaded");
+    }
+    return launchPruneImpl(nb, numParts);
+}
+
+int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeVirial)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel: null handle");
+    }
+    if (!nb->shiftVec.ptr || !nb->sci.ptr)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_launch_kernel: shift vectors and pair list must be uploaded first");
+    }
+    // A fresh list is pruned first (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:505-513).
+    if (nb->haveFreshList)
+    {
+        const int rc = launchPruneImpl(nb, 1);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+    }
+    NbKernelArgs a;
+    a.xq = nb->xq.ptr;
+    a.atomTypes = nb->types.ptr;
+    a.ljComb = nb->ljComb.ptr;
+    a.nbfp = nb->nbfp.ptr;
+    a.numTypes = nb->numTypes;
+    a.shiftVec = nb->shiftVec.ptr;
+    a.sci = nb->sci.ptr;
+    a.sciOrder = nb->haveSortedOrder ? nb->sciOrder.ptr : nullptr;
+    a.nsci = nb->nsci;
+    a.cjPacked = nb->cjPacked.ptr;
+    a.excl = nb->excl.ptr;
+    a.sysParamsA = nb->paramsA.ptr;
+    a.sysParamsB = nb->paramsB.ptr;
+    a.sysParamsD = nb->paramsD.ptr;
+    a.fFloat = nb->fFloat.ptr;
+    a.fFixed = nb->fFixed.ptr;
+    a.entryPartials = nb->partials.ptr;
+    a.computeFshift = computeVirial ? 1 : 0;
+    NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
+    nb->lastComputeEnergy = computeEnergy != 0;
+    nb->lastComputeVirial = computeVirial != 0;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_launch_cpyback: null handle");
+    }
+    const size_t n3 = 3 * static_cast<size_t>(nb->numAtoms);
+    if (nb->accumulation == NBNXM_ACCUM_FLOAT)
+    {
+        if (f && n3)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fFloat.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+        }
+        if (fDouble)
+        {
+            return fail(NBNXM_ERR_UNSUPPORTED, "gpu_launch_cpyback: double forces need fixed-point accumulation");
+        }
+    }
+    else
+    {
+        NBNXM_HIP_CHECK(launchConvertForces(nb->fFixed.ptr, nb->numAtoms, nb->fOut.ptr, nb->fOutDouble.ptr, nb->stream));
+        if (f && n3)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fOut.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+        }
+        if (fDouble && n3)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(fDouble, nb->fOutDouble.ptr, sizeof(double) * n3, hipMemcpyDeviceToHost, nb->stream));
+        }
+    }
+    if (nb->lastComputeEnergy || nb->lastComputeVirial)
+    {
+        NBNXM_HIP_CHECK(launchReduceEnergies(nb->partials.ptr, nb->entryRanges.ptr, nb->entryList.ptr, nb->shiftRanges.ptr, nb->shiftEntryList.ptr, 1, nb->energies.ptr, nb->fshift.ptr, nb->lastComputeVirial ? 1 : 0, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hEnergies, nb->energies.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hFshift, nb->fshift.ptr, 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_wait_finish_task: null handle");
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    // Accumulate into the caller's outputs like gpu_reduce_staged_outputs
+    // (src/gromacs/nbnxm/gpu_common.h:138-160).
+    if (nb->lastComputeEnergy)
+    {
+        if (e_lj)
+        {
+            *e_lj += nb->hEnergies[0];
+        }
+        if (e_el)
+        {
+            *e_el += nb->hEnergies[1];
+        }
+    }
+    if (nb->lastComputeVirial && fshift)
+    {
+        for (int k = 0; k < 3 * NBNXM_NUM_SHIFTS; k++)
+        {
+            fshift[k] += nb->hFshift[k];
+        }
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_min_ci_balanced: null handle");
+    }
+    int device = 0, cus = 0;
+    NBNXM_HIP_CHECK(hipGetDevice(&device));
+    NBNXM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    // Enough entries for several waves on every SIMD (4 per CU).
+    return 4 * 4 * cus;
+}
+
+int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb)
+{
+    return nb ? 1 : NBNXM_ERR_INVALID;
+}
+
+int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int* imaskOuter)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_masks: null handle");
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    std::vector<nbnxm_cj_packed_t> tmp(nb->ncjPacked);
+    if (nb->ncjPacked > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpy(tmp.data(), nb->cjPacked.ptr, sizeof(nbnxm_cj_packed_t) * nb->ncjPacked, hipMemcpyDeviceToHost));
+        if (imaskOuter)
+        {
+            NBNXM_HIP_CHECK(hipMemcpy(imaskOuter, nb->imaskOuter.ptr, sizeof(unsigned int) * nb->ncjPacked, hipMemcpyDeviceToHost));
+        }
+    }
+    if (imaskInner)
+    {
+        for (int g = 0; g < nb->ncjPacked; g++)
+        {
+            imaskInner[g] = tmp[g].imei[0].imask;
+        }
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_get_sci_order(nbnxm_gpu_t* nb, int* order, int* sciCount)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_sci_order: null handle");
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    if (nb->nsci > 0)
+    {
+        if (order)
+        {
+            NBNXM_HIP_CHECK(hipMemcpy(order, nb->sciOrder.ptr, sizeof(int) * nb->nsci, hipMemcpyDeviceToHost));
+        }
+        if (sciCount)
+        {
+            NBNXM_HIP_CHECK(hipMemcpy(sciCount, nb->sciCount.ptr, sizeof(int) * nb->nsci, hipMemcpyDeviceToHost));
+        }
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock)
+{
+    if (!nb || wavesPerBlock < 1 || wavesPerBlock > 4)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_set_waves_per_block: need 1..4");
+    }
+    nb->launchConfig.wavesPerBlock = wavesPerBlock;
+    return NBNXM_OK;
+}
+
+void nbnxm_gpu_free(nbnxm_gpu_t* nb)
+{
+    delete nb;
+}
+
+} // extern "C"

@@ -1,0 +1,101 @@
+// Internal launch interface of the gfx950 nbnxm kernels (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nbnxm_amd.h"
+
+namespace nbnxm_amd
+{
+
+// Everything one nonbonded launch reads and writes.  Device pointers only.
+// Systems are concatenated: cluster indices are global, sci.shift is
+// 45*system + shift index and shiftVec holds 45 float4 per system.
+struct NbKernelArgs
+{
+    const float4* xq = nullptr;
+    const int* atomTypes = nullptr;
+    const float2* ljComb = nullptr;
+    const float2* nbfp = nullptr;
+    int numTypes = 0;
+    const float4* shiftVec = nullptr;
+
+    const nbnxm_sci_t* sci = nullptr;
+    const int* sciOrder = nullptr; // optional permutation of entries (work order)
+    int nsci = 0;
+    const nbnxm_cj_packed_t* cjPacked = nullptr;
+    const nbnxm_excl_t* excl = nullptr;
+
+    // Per-system parameters (indexed by sci.shift / 45).
+    const float4* sysParamsA = nullptr; // rcoulomb_sq, epsfac, beta, sh_ewald
+    const float2* sysParamsB = nullptr; // repulsion cpot, dispersion cpot
+    const double* sysParamsD = nullptr; // FP64 mode: 8 per system, rc^2, epsfac, beta, sh_ewald, cpot rep, cpot disp
+
+    // Outputs.
+    float* fFloat = nullptr;                  // NBNXM_ACCUM_FLOAT: 3 per atom
+    unsigned long long* fFixed = nullptr;     // NBNXM_ACCUM_FIXED64 / FP64: 3 per atom
+    double* entryPartials = nullptr;          // 8 doubles per entry: Elj, Eel, fsx, fsy, fsz
+    int computeFshift = 0;
+};
+
+enum class KernelFlavor : int
+{
+    F = 0,
+    VF = 1
+};
+
+// Launch tuning chosen per size bucket by the autotuner.
+struct NbLaunchConfig
+{
+    int wavesPerBlock = 4;
+};
+
+hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream);
+
+struct PruneArgs
+{
+    const float4* xq = nullptr;
+    const float4* shiftVec = nullptr;
+    const nbnxm_sci_t* sci = nullptr;
+    const int* sciOrder = nullptr;
+    int nsci = 0;
+    nbnxm_cj_packed_t* cjPacked = nullptr; // imei[0].imask updated (inner mask)
+    unsigned int* imaskOuter = nullptr;    // per packed group
+    int* rollingPart = nullptr;            // per block part index (device), as the reference
+    int* sciCount = nullptr;               // histogram bin per entry (fresh list)
+    int* sciHistogram = nullptr;           // NBNXM_SCI_HISTOGRAM_SIZE bins (fresh list)
+    float rlistOuter_sq = 0;
+    float rlistInner_sq = 0;
+    int numParts = 1;
+};
+
+hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t stream);
+
+// Converts the fixed-point forces to float (optionally double) and clears the
+// accumulator for the next evaluation.
+hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream);
+
+// Per-system FP64 reduction of entry partials.  entryRanges: numSystems+1
+// offsets into entryList (entries of each system), shiftRanges: 45*numSystems+1
+// offsets into shiftEntryList.  out: per system Elj, Eel; fshift 45*3 per system.
+hipError_t launchReduceEnergies(const double* entryPartials,
+                                const int* entryRanges,
+                                const int* entryList,
+                                const int* shiftRanges,
+                                const int* shiftEntryList,
+                                int numSystems,
+                                double* energiesOut,
+                                double* fshiftOut,
+                                int computeFshift,
+                                hipStream_t stream);
+
+// x_k = x_0 + N(0, sigma) for real atoms (q and fillers kept), Philox4x32-10 counter RNG.
+hipError_t launchPerturb(const float4* x0, float4* xq, const int* atomIsReal, int numAtoms, float sigma, uint64_t seed, uint64_t step, hipStream_t stream);
+
+// Fill kernels (graph-safe clears).
+hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream);
+hipError_t launchFillI32(int* p, size_t n, int value, hipStream_t stream);
+
+} // namespace nbnxm_amd

@@ -1,0 +1,959 @@
+// gfx950 (CDNA4, wave64) kernels of the nbnxm cluster-pair path.
+//
+// One wavefront owns one pair-list entry (an i-super-cluster of 8 clusters x 8
+// atoms and a run of packed j-groups).  Lane l handles i-atom ti = l & 7 and
+// j-atom tj = l >> 3 of every 8x8 cluster pair, the thread layout of the
+// reference kernel (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:216-218) with
+// the cluster-pair split removed so the whole 8x8 pair is one wave.
+//  - i-cluster coordinates, charges and LJ parameters live in registers for the
+//    whole entry (the reference stages them in shared memory, :330-351);
+//  - j-forces are reduced over the 8 i-lanes with three DPP adds per component
+//    (row_half_mirror, quad_perm xor1, quad_perm xor2) instead of warp shuffles
+//    (:677, nbnxm_cuda_kernel_utils.cuh:467-496);
+//  - forces are accumulated as int64 fixed point (2^-32 units), so every
+//    evaluation is bit-reproducible whatever the atomic order;
+//  - energies and shift forces are reduced in FP64 per entry and written to a
+//    per-entry slot, then summed per system in a fixed order by a second kernel.
+#include "nbnxm_kernels.h"
+
+#include <hipcub/hipcub.hpp>
+
+namespace nbnxm_amd
+{
+
+namespace
+{
+
+constexpr int c_cl = NBNXM_CLUSTER_SIZE;
+constexpr int c_ncps = NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER;
+constexpr int c_jg = NBNXM_JGROUP_SIZE;
+constexpr float c_fixedScale = 4294967296.0f;        // 2^NBNXM_FIXED_FRAC_BITS
+constexpr double c_fixedScaleD = 4294967296.0;
+constexpr float c_fixedClamp = 9.0e18f;              // keeps the int64 conversion defined
+constexpr double c_oneSqrtPi = 0.56418958354775628695;
+
+template<int ctrl>
+__device__ __forceinline__ float dppF(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+// Sum over the 8 consecutive lanes of a group; every lane of the group gets the
+// same bits (the adds are pairwise commutative images of each other).
+__device__ __forceinline__ float reduceOver8Lanes(float v)
+{
+    v += dppF<0x141>(v); // row_half_mirror: lane k <-> 7-k within 8
+    v += dppF<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dppF<0x4E>(v);  // quad_perm [2,3,0,1]
+    return v;
+}
+
+// Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), 16, 32.
+__device__ __forceinline__ float reduceOverStride8(float v)
+{
+    v += dppF<0x128>(v);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+__device__ __forceinline__ double reduceWaveD(double v)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1)
+    {
+        v += __shfl_xor(v, m);
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long toFixed(float v)
+{
+    float s = v * c_fixedScale;
+    s = fminf(fmaxf(s, -c_fixedClamp), c_fixedClamp);
+    return static_cast<unsigned long long>(__float2ll_rn(s));
+}
+
+__device__ __forceinline__ unsigned long long toFixedD(double v)
+{
+    double s = v * c_fixedScaleD;
+    s = fmin(fmax(s, -9.0e18), 9.0e18);
+    return static_cast<unsigned long long>(__double2ll_rn(s));
+}
+
+// Analytical Ewald force correction: the FP32 rational of pmecorrF
+// (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_utils.cuh:407-443), same operation
+// order, hardware reciprocal as the reference's fast-math build.
+__device__ __forceinline__ float pmecorrF(float z2)
+{
+    const float FN6 = -1.7357322914161492954e-8F;
+    const float FN5 = 1.4703624142580877519e-6F;
+    const float FN4 = -0.000053401640219807709149F;
+    const float FN3 = 0.0010054721316683106153F;
+    const float FN2 = -0.019278317264888380590F;
+    const float FN1 = 0.069670166153766424023F;
+    const float FN0 = -0.75225204789749321333F;
+    const float FD4 = 0.0011193462567257629232F;
+    const float FD3 = 0.014866955030185295499F;
+    const float FD2 = 0.11583842382862377919F;
+    const float FD1 = 0.50736591960530292870F;
+    const float FD0 = 1.0F;
+
+    const float z4 = z2 * z2;
+    float polyFD0 = __builtin_fmaf(FD4, z4, FD2);
+    float polyFD1 = __builtin_fmaf(FD3, z4, FD1);
+    polyFD0 = __builtin_fmaf(polyFD0, z4, FD0);
+    polyFD0 = __builtin_fmaf(polyFD1, z2, polyFD0);
+    polyFD0 = __builtin_amdgcn_rcpf(polyFD0);
+
+    float polyFN0 = __builtin_fmaf(FN6, z4, FN4);
+    float polyFN1 = __builtin_fmaf(FN5, z4, FN3);
+    polyFN0 = __builtin_fmaf(polyFN0, z4, FN2);
+    polyFN1 = __builtin_fmaf(polyFN1, z4, FN1);
+    polyFN0 = __builtin_fmaf(polyFN0, z4, FN0);
+    polyFN0 = __builtin_fmaf(polyFN1, z2, polyFN0);
+    return polyFN0 * polyFD0;
+}
+
+// r^2 in the fixed evaluation order shared with the CPU prune restatement.
+__device__ __forceinline__ float norm2Fixed(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+template<bool computeEnergy, int vdwType, int accumulation>
+__global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a, int wavesPerBlock)
+{
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7;
+    const int tj = lane >> 3;
+    const int work = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    if (work >= a.nsci)
+    {
+        return;
+    }
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
+    const nbnxm_sci_t s = a.sci[entry];
+    const int sys = s.shift / NBNXM_NUM_SHIFTS;
+    const int shiftInSys = s.shift - sys * NBNXM_NUM_SHIFTS;
+    const bool centralShift = (shiftInSys == NBNXM_CENTRAL_SHIFT);
+
+    const float4 pA = a.sysParamsA[sys];
+    const float2 pB = a.sysParamsB[sys];
+    const float rcoulomb_sq = pA.x;
+    const float epsfac = pA.y;
+    const float beta = pA.z;
+    const float beta2 = beta * beta;
+    const float beta3 = beta * beta * beta;
+    const float ewaldShift = pA.w;
+    const float cpotRep = pB.x;
+    const float cpotDisp = pB.y;
+
+    const float4 sh = a.shiftVec[s.shift];
+
+    // i-atom data for all 8 clusters of the super-cluster, in registers.
+    float xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
+    int typeiOff[c_ncps];
+    float ljci0[c_ncps], ljci1[c_ncps];
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        const float4 v = a.xq[ai];
+        xi[c] = v.x + sh.x;
+        yi[c] = v.y + sh.y;
+        zi[c] = v.z + sh.z;
+        qi[c] = v.w * epsfac;
+        if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+        {
+            typeiOff[c] = a.numTypes * a.atomTypes[ai];
+        }
+        else
+        {
+            const float2 lc = a.ljComb[ai];
+            ljci0[c] = lc.x;
+            ljci1[c] = lc.y;
+        }
+    }
+
+    float fciX[c_ncps], fciY[c_ncps], fciZ[c_ncps];
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        fciX[c] = 0.0f;
+        fciY[c] = 0.0f;
+        fciZ[c] = 0.0f;
+    }
+
+    using EAcc = typename std::conditional<accumulation == NBNXM_ACCUM_FLOAT, float, double>::type;
+    EAcc eLJ = 0;
+    EAcc eEl = 0;
+    if constexpr (computeEnergy)
+    {
+        // Self energy on the diagonal entry (:381-418): -epsfac*beta/sqrt(pi)*sum q^2,
+        // added once per i-atom by the tj == 0 lanes.
+        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && a.cjPacked[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
+        {
+            EAcc q2 = 0;
+#pragma unroll
+            for (int c = 0; c < c_ncps; c++)
+            {
+                q2 += static_cast<EAcc>(qi[c]) * static_cast<EAcc>(qi[c]);
+            }
+            eEl -= q2 / static_cast<EAcc>(epsfac) * static_cast<EAcc>(beta) * static_cast<EAcc>(c_oneSqrtPi);
+        }
+    }
+
+    // Newton / self masking on the diagonal of the central shift (:423).
+    const bool nonSelfLane = !(centralShift && tj <= ti);
+
+    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+    {
+        const unsigned int imask = a.cjPacked[jp].imei[0].imask;
+        if (imask == 0)
+        {
+            continue;
+        }
+        const unsigned int wexcl = a.excl[a.cjPacked[jp].imei[0].excl_ind].pair[lane];
+#pragma unroll
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if (!(imask & (0xFFu << (jm * c_ncps))))
+            {
+                continue;
+            }
+            const int cj = a.cjPacked[jp].cj[jm];
+            const int aj = cj * c_cl + tj;
+            const float4 xqj = a.xq[aj];
+            int typej = 0;
+            float2 ljcj = make_float2(0.0f, 0.0f);
+            if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+            {
+                typej = a.atomTypes[aj];
+            }
+            else
+            {
+                ljcj = a.ljComb[aj];
+            }
+            float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
+
+#pragma unroll
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const int bit = jm * c_ncps + c;
+                if (!(imask & (1u << bit)))
+                {
+                    continue;
+                }
+                const float dx = xi[c] - xqj.x;
+                const float dy = yi[c] - xqj.y;
+                const float dz = zi[c] - xqj.z;
+                float r2 = norm2Fixed(dx, dy, dz);
+                const bool notDiag = (s.sci * c_ncps + c) != cj;
+                if (r2 < rcoulomb_sq && (nonSelfLane || notDiag))
+                {
+                    const float int_bit = ((wexcl >> bit) & 1u) ? 1.0f : 0.0f;
+                    float c6, c12;
+                    float sigma = 0.0f, epsilon = 0.0f;
+                    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                    {
+                        const float2 p = a.nbfp[typeiOff[c] + typej];
+                        c6 = p.x;
+                        c12 = p.y;
+                    }
+                    else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                    {
+                        c6 = ljci0[c] * ljcj.x;
+                        c12 = ljci1[c] * ljcj.y;
+                    }
+                    else
+                    {
+                        sigma = ljci0[c] + ljcj.x;
+                        epsilon = ljci1[c] * ljcj.y;
+                        const float sigma2 = sigma * sigma;
+                        const float sigma6 = sigma2 * sigma2 * sigma2;
+                        c6 = epsilon * sigma6;
+                        c12 = c6 * sigma6;
+                    }
+                    r2 = fmaxf(r2, NBNXM_MIN_DISTANCE_SQUARED);
+                    const float inv_r = __builtin_amdgcn_rsqf(r2);
+                    const float inv_r2 = inv_r * inv_r;
+                    float F_invr;
+                    float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                    inv_r6 *= int_bit;
+                    if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                    {
+                        const float sig_r = sigma * inv_r;
+                        const float sig_r2 = sig_r * sig_r;
+                        float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+                        sig_r6 *= int_bit;
+                        F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
+                    }
+                    else
+                    {
+                        F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                    }
+                    const float qq = qi[c] * xqj.w;
+                    F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
+                    if constexpr (computeEnergy)
+                    {
+                        const float elj = int_bit
+                                          * (c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f)
+                                             - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f));
+                        const float eel = qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
+                        eLJ += static_cast<EAcc>(elj);
+                        eEl += static_cast<EAcc>(eel);
+                    }
+                    const float tx = dx * F_invr;
+                    const float ty = dy * F_invr;
+                    const float tz = dz * F_invr;
+                    fcjX -= tx;
+                    fcjY -= ty;
+                    fcjZ -= tz;
+                    fciX[c] += tx;
+                    fciY[c] += ty;
+                    fciZ[c] += tz;
+                }
+            }
+
+            // j-force: sum over the 8 i-lanes, lanes ti = 0,1,2 store x,y,z.
+            fcjX = reduceOver8Lanes(fcjX);
+            fcjY = reduceOver8Lanes(fcjY);
+            fcjZ = reduceOver8Lanes(fcjZ);
+            if (ti < 3)
+            {
+                const float v = (ti == 0) ? fcjX : ((ti == 1) ? fcjY : fcjZ);
+                if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                {
+                    atomicAdd(a.fFloat + 3 * aj + ti, v);
+                }
+                else
+                {
+                    atomicAdd(a.fFixed + 3 * aj + ti, toFixed(v));
+                }
+            }
+        }
+    }
+
+    // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
+    double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const float fx = reduceOverStride8(fciX[c]);
+        const float fy = reduceOverStride8(fciY[c]);
+        const float fz = reduceOverStride8(fciZ[c]);
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        if (tj < 3)
+        {
+            const float v = (tj == 0) ? fx : ((tj == 1) ? fy : fz);
+            if (v != 0.0f)
+            {
+                if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                {
+                    atomicAdd(a.fFloat + 3 * ai + tj, v);
+                }
+                else
+                {
+                    atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v));
+                }
+            }
+        }
+        fsX += static_cast<double>(fx);
+        fsY += static_cast<double>(fy);
+        fsZ += static_cast<double>(fz);
+    }
+
+    if (computeEnergy || a.computeFshift)
+    {
+        // Every lane of an i-column holds the column's sums: take tj == 0 lanes.
+        const double keep = (tj == 0) ? 1.0 : 0.0;
+        double v0 = 0.0, v1 = 0.0;
+        if constexpr (computeEnergy)
+        {
+            v0 = reduceWaveD(static_cast<double>(eLJ));
+            v1 = reduceWaveD(static_cast<double>(eEl));
+        }
+        const bool doShift = a.computeFshift && !centralShift;
+        const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
+        const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
+        const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
+        if (lane < 8)
+        {
+            const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
+            a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+        }
+    }
+}
+
+// FP64 pair math with exact erf/exp (the GPU oracle mode), fixed-point forces.
+template<bool computeEnergy, int vdwType>
+__global__ __launch_bounds__(256) void nbnxmNonbondedKernelFp64(const NbKernelArgs a, int wavesPerBlock)
+{
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7;
+    const int tj = lane >> 3;
+    const int work = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    if (work >= a.nsci)
+    {
+        return;
+    }
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
+    const nbnxm_sci_t s = a.sci[entry];
+    const int sys = s.shift / NBNXM_NUM_SHIFTS;
+    const int shiftInSys = s.shift - sys * NBNXM_NUM_SHIFTS;
+    const bool centralShift = (shiftInSys == NBNXM_CENTRAL_SHIFT);
+    const double* pD = a.sysParamsD + 8 * sys;
+    const double rcoulomb_sq = pD[0];
+    const double epsfac = pD[1];
+    const double beta = pD[2];
+    const double ewaldShift = pD[3];
+    const double cpotRep = pD[4];
+    const double cpotDisp = pD[5];
+    const float4 sh = a.shiftVec[s.shift];
+
+    double eLJ = 0.0, eEl = 0.0;
+    if constexpr (computeEnergy)
+    {
+        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && a.cjPacked[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
+        {
+            double q2 = 0.0;
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const double q = a.xq[(s.sci * c_ncps + c) * c_cl + ti].w;
+                q2 += q * q;
+            }
+            eEl -= epsfac * q2 * beta * c_oneSqrtPi;
+        }
+    }
+    const bool nonSelfLane = !(centralShift && tj <= ti);
+    double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
+    double fci[c_ncps][3];
+    for (int c = 0; c < c_ncps; c++)
+    {
+        fci[c][0] = fci[c][1] = fci[c][2] = 0.0;
+    }
+
+    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+    {
+        const unsigned int imask = a.cjPacked[jp].imei[0].imask;
+        if (imask == 0)
+        {
+            continue;
+        }
+        const unsigned int wexcl = a.excl[a.cjPacked[jp].imei[0].excl_ind].pair[lane];
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if (!(imask & (0xFFu << (jm * c_ncps))))
+            {
+                continue;
+            }
+            const int cj = a.cjPacked[jp].cj[jm];
+            const int aj = cj * c_cl + tj;
+            const float4 xqj = a.xq[aj];
+            double fcj[3] = { 0.0, 0.0, 0.0 };
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const int bit = jm * c_ncps + c;
+                if (!(imask & (1u << bit)))
+                {
+                    continue;
+                }
+                const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+                const float4 xqi = a.xq[ai];
+                const double dx = (static_cast<double>(xqi.x) + sh.x) - xqj.x;
+                const double dy = (static_cast<double>(xqi.y) + sh.y) - xqj.y;
+                const double dz = (static_cast<double>(xqi.z) + sh.z) - xqj.z;
+                double r2 = dx * dx + dy * dy + dz * dz;
+                const bool notDiag = (s.sci * c_ncps + c) != cj;
+                if (r2 < rcoulomb_sq && (nonSelfLane || notDiag))
+                {
+                    const double int_bit = ((wexcl >> bit) & 1u) ? 1.0 : 0.0;
+                    double c6, c12;
+                    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                    {
+                        const float2 p = a.nbfp[a.numTypes * a.atomTypes[ai] + a.atomTypes[aj]];
+                        c6 = p.x;
+                        c12 = p.y;
+                    }
+                    else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                    {
+                        const float2 li = a.ljComb[ai];
+                        const float2 lj = a.ljComb[aj];
+                        c6 = static_cast<double>(li.x) * lj.x;
+                        c12 = static_cast<double>(li.y) * lj.y;
+                    }
+                    else
+                    {
+                        const float2 li = a.ljComb[ai];
+                        const float2 lj = a.ljComb[aj];
+                        const double sigma = static_cast<double>(li.x) + lj.x;
+                        const double epsilon = static_cast<double>(li.y) * lj.y;
+                        const double s2 = sigma * sigma;
+                        const double s6 = s2 * s2 * s2;
+                        c6 = epsilon * s6;
+                        c12 = c6 * s6;
+                    }
+                    r2 = fmax(r2, static_cast<double>(NBNXM_MIN_DISTANCE_SQUARED));
+                    const double r = sqrt(r2);
+                    const double inv_r = 1.0 / r;
+                    const double inv_r2 = inv_r * inv_r;
+                    const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
+                    double F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                    const double qq = static_cast<double>(xqi.w) * epsfac * xqj.w;
+                    const double br = beta * r;
+                    const double erfbr = erf(br);
+                    F_invr += qq * (int_bit * inv_r2 * inv_r + 2.0 * beta * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
+                    if constexpr (computeEnergy)
+                    {
+                        eLJ += int_bit * (c12 * (inv_r6 * inv_r6 + cpotRep) / 12.0 - c6 * (inv_r6 + cpotDisp) / 6.0);
+                        eEl += qq * (inv_r * (int_bit - erfbr) - int_bit * ewaldShift);
+                    }
+                    fcj[0] -= dx * F_invr;
+                    fcj[1] -= dy * F_invr;
+                    fcj[2] -= dz * F_invr;
+                    fci[c][0] += dx * F_invr;
+                    fci[c][1] += dy * F_invr;
+                    fci[c][2] += dz * F_invr;
+                }
+            }
+            for (int d = 0; d < 3; d++)
+            {
+                double v = fcj[d];
+                v += __shfl_xor(v, 1);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 4);
+                fcj[d] = v;
+            }
+            if (ti < 3)
+            {
+                atomicAdd(a.fFixed + 3 * aj + ti, toFixedD(fcj[ti == 0 ? 0 : (ti == 1 ? 1 : 2)]));
+            }
+        }
+    }
+    for (int c = 0; c < c_ncps; c++)
+    {
+        double f3[3];
+        for (int d = 0; d < 3; d++)
+        {
+            double v = fci[c][d];
+            v += __shfl_xor(v, 8);
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            f3[d] = v;
+        }
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        if (tj < 3)
+        {
+            atomicAdd(a.fFixed + 3 * ai + tj, toFixedD(f3[tj == 0 ? 0 : (tj == 1 ? 1 : 2)]));
+        }
+        fsX += f3[0];
+        fsY += f3[1];
+        fsZ += f3[2];
+    }
+    if (computeEnergy || a.computeFshift)
+    {
+        const double keep = (tj == 0) ? 1.0 : 0.0;
+        const double v0 = reduceWaveD(eLJ);
+        const double v1 = reduceWaveD(eEl);
+        const bool doShift = a.computeFshift && !centralShift;
+        const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
+        const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
+        const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
+        if (lane < 8)
+        {
+            const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
+            a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+        }
+    }
+}
+
+// Prune (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343), one
+// wave per entry of the current rolling part, split 1.
+template<bool haveFreshList>
+__global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7;
+    const int tj = lane >> 3;
+    const int wavesPerBlock = blockDim.x >> 6;
+    const int w = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+
+    // Each wave keeps its own copy of the part index, as each block does in the reference (:119-127).
+    const int part = a.rollingPart[w];
+    if (lane == 0)
+    {
+        a.rollingPart[w] = (part + 1) % a.numParts;
+    }
+    const int numInPart = (a.nsci - part + a.numParts - 1) / a.numParts;
+    if (w >= numInPart)
+    {
+        return;
+    }
+    const int work = w * a.numParts + part;
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
+    const nbnxm_sci_t s = a.sci[entry];
+    const float4 sh = a.shiftVec[s.shift];
+    float xi[c_ncps], yi[c_ncps], zi[c_ncps];
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const float4 v = a.xq[(s.sci * c_ncps + c) * c_cl + ti];
+        xi[c] = v.x + sh.x;
+        yi[c] = v.y + sh.y;
+        zi[c] = v.z + sh.z;
+    }
+    int prunedPairCount = 0;
+    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+    {
+        unsigned int imaskFull, imaskCheck, imaskNew;
+        if (haveFreshList)
+        {
+            imaskFull = a.cjPacked[jp].imei[0].imask;
+            imaskCheck = imaskFull;
+            imaskNew = 0;
+        }
+        else
+        {
+            imaskFull = a.imaskOuter[jp];
+            imaskNew = a.cjPacked[jp].imei[0].imask;
+            imaskCheck = imaskNew ^ imaskFull;
+        }
+        if (imaskCheck == 0)
+        {
+            continue;
+        }
+#pragma unroll
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if (!(imaskCheck & (0xFFu << (jm * c_ncps))))
+            {
+                continue;
+            }
+            const int cj = a.cjPacked[jp].cj[jm];
+            const float4 xj = a.xq[cj * c_cl + tj];
+#pragma unroll
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const unsigned int mask_ji = 1u << (jm * c_ncps + c);
+                if (!(imaskCheck & mask_ji))
+                {
+                    continue;
+                }
+                const float r2 = norm2Fixed(xi[c] - xj.x, yi[c] - xj.y, zi[c] - xj.z);
+                if (haveFreshList && __ballot(r2 < a.rlistOuter_sq) == 0)
+                {
+                    imaskFull &= ~mask_ji;
+                }
+                if (__ballot(r2 < a.rlistInner_sq) != 0)
+                {
+                    imaskNew |= mask_ji;
+                }
+            }
+        }
+        if (lane == 0)
+        {
+            if (haveFreshList)
+            {
+                a.imaskOuter[jp] = imaskFull;
+            }
+            a.cjPacked[jp].imei[0].imask = imaskNew;
+        }
+        if (haveFreshList)
+        {
+            prunedPairCount += __popc(imaskNew);
+        }
+    }
+    if (haveFreshList && lane == 0)
+    {
+        const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
+        if (a.sciHistogram)
+        {
+            atomicAdd(a.sciHistogram + index, 1);
+        }
+        if (a.sciCount)
+        {
+            a.sciCount[work] = index;
+        }
+    }
+}
+
+__global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* fOut, double* fOutDouble)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3)
+    {
+        return;
+    }
+    const long long v = static_cast<long long>(fFixed[i]);
+    const double d = static_cast<double>(v) * (1.0 / c_fixedScaleD);
+    if (fOut)
+    {
+        fOut[i] = static_cast<float>(d);
+    }
+    if (fOutDouble)
+    {
+        fOutDouble[i] = d;
+    }
+    fFixed[i] = 0ull;
+}
+
+// One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
+__global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partials,
+                                                           const int* entryRanges,
+                                                           const int* entryList,
+                                                           const int* shiftRanges,
+                                                           const int* shiftEntryList,
+                                                           int computeFshift,
+                                                           double* energiesOut,
+                                                           double* fshiftOut)
+{
+    const int sys = blockIdx.x / (1 + NBNXM_NUM_SHIFTS);
+    const int slot = blockIdx.x % (1 + NBNXM_NUM_SHIFTS);
+    const int lane = threadIdx.x;
+    if (slot == 0)
+    {
+        double e0 = 0.0, e1 = 0.0;
+        for (int k = entryRanges[sys] + lane; k < entryRanges[sys + 1]; k += 64)
+        {
+            const int e = entryList[k];
+            e0 += partials[8 * static_cast<size_t>(e)];
+            e1 += partials[8 * static_cast<size_t>(e) + 1];
+        }
+        e0 = reduceWaveD(e0);
+        e1 = reduceWaveD(e1);
+        if (lane == 0)
+        {
+            energiesOut[2 * sys] = e0;
+            energiesOut[2 * sys + 1] = e1;
+        }
+        return;
+    }
+    if (!computeFshift)
+    {
+        return;
+    }
+    const int sidx = sys * NBNXM_NUM_SHIFTS + slot - 1;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int k = shiftRanges[sidx] + lane; k < shiftRanges[sidx + 1]; k += 64)
+    {
+        const int e = shiftEntryList[k];
+        f0 += partials[8 * static_cast<size_t>(e) + 2];
+        f1 += partials[8 * static_cast<size_t>(e) + 3];
+        f2 += partials[8 * static_cast<size_t>(e) + 4];
+    }
+    f0 = reduceWaveD(f0);
+    f1 = reduceWaveD(f1);
+    f2 = reduceWaveD(f2);
+    if (lane == 0)
+    {
+        fshiftOut[3 * sidx + 0] = f0;
+        fshiftOut[3 * sidx + 1] = f1;
+        fshiftOut[3 * sidx + 2] = f2;
+    }
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used as a counter-based
+// generator: key = seed, counter = (atom, step).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 ctr, uint2 key)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++)
+    {
+        const uint32_t hi0 = __umulhi(M0, ctr.x);
+        const uint32_t lo0 = M0 * ctr.x;
+        const uint32_t hi1 = __umulhi(M1, ctr.z);
+        const uint32_t lo1 = M1 * ctr.z;
+        ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+        key.x += W0;
+        key.y += W1;
+    }
+    return ctr;
+}
+
+__device__ __forceinline__ float uniformOpen(uint32_t u)
+{
+    return (static_cast<float>(u >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+__global__ void perturbKernel(const float4* x0, float4* xq, const int* atomIsReal, int n, float sigma, uint2 key, uint32_t stepLo, uint32_t stepHi)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+    {
+        return;
+    }
+    float4 v = x0[i];
+    if (atomIsReal[i])
+    {
+        const uint4 r = philox4x32_10(make_uint4(static_cast<uint32_t>(i), 0u, stepLo, stepHi), key);
+        const float u1 = uniformOpen(r.x), u2 = uniformOpen(r.y), u3 = uniformOpen(r.z), u4 = uniformOpen(r.w);
+        const float m1 = sqrtf(-2.0f * logf(u1));
+        const float m2 = sqrtf(-2.0f * logf(u3));
+        const float a1 = 6.283185307179586f * u2;
+        const float a2 = 6.283185307179586f * u4;
+        v.x += sigma * m1 * cosf(a1);
+        v.y += sigma * m1 * sinf(a1);
+        v.z += sigma * m2 * cosf(a2);
+    }
+    xq[i] = v;
+}
+
+__global__ void fillU64Kernel(unsigned long long* p, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        p[i] = 0ull;
+    }
+}
+
+__global__ void fillI32Kernel(int* p, size_t n, int value)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        p[i] = value;
+    }
+}
+
+template<bool E, int V, int A>
+hipError_t launchNb(const NbKernelArgs& a, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    const int wpb = cfg.wavesPerBlock;
+    const int blocks = (a.nsci + wpb - 1) / wpb;
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    if constexpr (A == NBNXM_ACCUM_FP64)
+    {
+        hipLaunchKernelGGL((nbnxmNonbondedKernelFp64<E, V>), dim3(blocks), dim3(64 * wpb), 0, stream, a, wpb);
+    }
+    else
+    {
+        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A>), dim3(blocks), dim3(64 * wpb), 0, stream, a, wpb);
+    }
+    return hipGetLastError();
+}
+
+template<bool E, int V>
+hipError_t launchNbAcc(const NbKernelArgs& a, int accumulation, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    switch (accumulation)
+    {
+        case NBNXM_ACCUM_FLOAT: return launchNb<E, V, NBNXM_ACCUM_FLOAT>(a, cfg, stream);
+        case NBNXM_ACCUM_FIXED64: return launchNb<E, V, NBNXM_ACCUM_FIXED64>(a, cfg, stream);
+        case NBNXM_ACCUM_FP64: return launchNb<E, V, NBNXM_ACCUM_FP64>(a, cfg, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template<bool E>
+hipError_t launchNbVdw(const NbKernelArgs& a, int accumulation, int vdwType, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    switch (vdwType)
+    {
+        case NBNXM_VDW_CUT_TYPE_TABLE: return launchNbAcc<E, NBNXM_VDW_CUT_TYPE_TABLE>(a, accumulation, cfg, stream);
+        case NBNXM_VDW_CUT_COMB_GEOM: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_GEOM>(a, accumulation, cfg, stream);
+        case NBNXM_VDW_CUT_COMB_LB: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_LB>(a, accumulation, cfg, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    if (cfg.wavesPerBlock < 1 || cfg.wavesPerBlock > 4)
+    {
+        return hipErrorInvalidValue;
+    }
+    return computeEnergy ? launchNbVdw<true>(a, accumulation, vdwType, cfg, stream)
+                         : launchNbVdw<false>(a, accumulation, vdwType, cfg, stream);
+}
+
+hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t stream)
+{
+    const int wpb = 4;
+    const int maxInPart = (a.nsci + a.numParts - 1) / a.numParts;
+    const int blocks = (maxInPart + wpb - 1) / wpb;
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    if (haveFreshList)
+    {
+        hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * wpb), 0, stream, a);
+    }
+    else
+    {
+        hipLaunchKernelGGL(nbnxmPruneKernel<false>, dim3(blocks), dim3(64 * wpb), 0, stream, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream)
+{
+    const int n3 = 3 * numAtoms;
+    if (n3 == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(convertForcesKernel, dim3((n3 + 255) / 256), dim3(256), 0, stream, fFixed, n3, fOut, fOutDouble);
+    return hipGetLastError();
+}
+
+hipError_t launchReduceEnergies(const double* entryPartials,
+                                const int* entryRanges,
+                                const int* entryList,
+                                const int* shiftRanges,
+                                const int* shiftEntryList,
+                                int numSystems,
+                                double* energiesOut,
+                                double* fshiftOut,
+                                int computeFshift,
+                                hipStream_t stream)
+{
+    if (numSystems == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(reduceEnergiesKernel, dim3(numSystems * (1 + NBNXM_NUM_SHIFTS)), dim3(64), 0, stream, entryPartials, entryRanges, entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    return hipGetLastError();
+}
+
+hipError_t launchPerturb(const float4* x0, float4* xq, const int* atomIsReal, int numAtoms, float sigma, uint64_t seed, uint64_t step, hipStream_t stream)
+{
+    if (numAtoms == 0)
+    {
+        return hipSuccess;
+    }
+    const uint2 key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+    hipLaunchKernelGGL(perturbKernel, dim3((numAtoms + 255) / 256), dim3(256), 0, stream, x0, xq, atomIsReal, numAtoms, sigma, key, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32));
+    return hipGetLastError();
+}
+
+hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream)
+{
+    if (n == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(fillU64Kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launchFillI32(int* p, size_t n, int value, hipStream_t stream)
+{
+    if (n == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(fillI32Kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+
+} // namespace nbnxm_amd

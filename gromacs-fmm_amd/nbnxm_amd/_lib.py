@@ -1,0 +1,159 @@
+"""ctypes binding of libnbnxm_amd.so (the C ABI declared in include/nbnxm_amd.h).
+
+The library is built in-tree (``make -C gromacs-fmm_amd``).  Loading fails
+loudly when it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libnbnxm_amd.so")
+
+CLUSTER_SIZE = 8
+NUM_CLUSTER_PER_SUPERCLUSTER = 8
+JGROUP_SIZE = 4
+EXCL_SIZE = 64
+NUM_SHIFTS = 45
+CENTRAL_SHIFT = 22
+SCI_HISTOGRAM_SIZE = 8192
+FIXED_FRAC_BITS = 32
+
+VDW_CUT_TYPE_TABLE = 0
+VDW_CUT_COMB_GEOM = 1
+VDW_CUT_COMB_LB = 2
+
+ACCUM_FLOAT = 0
+ACCUM_FIXED64 = 1
+ACCUM_FP64 = 2
+
+
+class Sci(ctypes.Structure):
+    _fields_ = [("sci", ctypes.c_int), ("shift", ctypes.c_int),
+                ("cjPackedBegin", ctypes.c_int), ("cjPackedEnd", ctypes.c_int)]
+
+
+class CjPacked(ctypes.Structure):
+    _fields_ = [("cj", ctypes.c_int * 4), ("imask", ctypes.c_uint), ("excl_ind", ctypes.c_int)]
+
+
+class Excl(ctypes.Structure):
+    _fields_ = [("pair", ctypes.c_uint * EXCL_SIZE)]
+
+
+class PairlistGpu(ctypes.Structure):
+    _fields_ = [("na_ci", ctypes.c_int), ("rlist", ctypes.c_float),
+                ("nsci", ctypes.c_int), ("sci", ctypes.c_void_p),
+                ("ncjPacked", ctypes.c_int), ("cjPacked", ctypes.c_void_p),
+                ("nexcl", ctypes.c_int), ("excl", ctypes.c_void_p)]
+
+
+class InteractionConst(ctypes.Structure):
+    _fields_ = [("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double),
+                ("ewaldcoeff_q", ctypes.c_double), ("sh_ewald", ctypes.c_double),
+                ("epsfac", ctypes.c_double), ("dispersion_shift_cpot", ctypes.c_double),
+                ("repulsion_shift_cpot", ctypes.c_double)]
+
+
+class PairlistParams(ctypes.Structure):
+    _fields_ = [("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
+                ("useDynamicPruning", ctypes.c_int), ("numRollingPruningParts", ctypes.c_int)]
+
+
+class Atomdata(ctypes.Structure):
+    _fields_ = [("numAtoms", ctypes.c_int), ("xq", ctypes.c_void_p),
+                ("numTypes", ctypes.c_int), ("type", ctypes.c_void_p),
+                ("nbfp", ctypes.c_void_p), ("vdwType", ctypes.c_int),
+                ("ljComb", ctypes.c_void_p), ("shiftVec", ctypes.c_void_p)]
+
+
+# numpy dtypes with the same layout as the C structs
+SCI_DTYPE = np.dtype([("sci", np.int32), ("shift", np.int32),
+                      ("cjPackedBegin", np.int32), ("cjPackedEnd", np.int32)])
+CJ_DTYPE = np.dtype([("cj", np.int32, (4,)), ("imask", np.uint32), ("excl_ind", np.int32)])
+EXCL_DTYPE = np.dtype([("pair", np.uint32, (EXCL_SIZE,))])
+
+assert SCI_DTYPE.itemsize == ctypes.sizeof(Sci) == 16
+assert CJ_DTYPE.itemsize == ctypes.sizeof(CjPacked) == 24
+assert EXCL_DTYPE.itemsize == ctypes.sizeof(Excl) == 256
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_vp = ctypes.c_void_p
+_PTR = ctypes.POINTER
+
+# (name, restype, argtypes) of every exported function
+SIGNATURES = [
+    ("nbnxm_last_error", ctypes.c_char_p, []),
+    ("nbnxm_version", ctypes.c_char_p, []),
+    ("nbnxm_calc_ewaldcoeff_q", _c_double, [_c_double, _c_double]),
+    ("nbnxm_init_interaction_const", _c_int, [_c_double, _c_double, _c_double, _PTR(InteractionConst)]),
+    ("nbnxm_calc_shifts", _c_int, [_vp, _vp]),
+    ("nbnxm_search_create", _vp, []),
+    ("nbnxm_search_free", None, [_vp]),
+    ("nbnxm_search_put_atoms_on_grid", _c_int, [_vp, _vp, _c_int, _vp]),
+    ("nbnxm_search_construct_pairlist", _c_int, [_vp, _c_double, _vp, _vp, _c_int]),
+    ("nbnxm_search_get_pairlist", _c_int, [_vp, _PTR(PairlistGpu)]),
+    ("nbnxm_search_num_grid_atoms", _c_int, [_vp]),
+    ("nbnxm_search_atom_indices", _c_int, [_vp, _vp]),
+    ("nbnxm_search_shift_vec", _c_int, [_vp, _vp]),
+    ("nbnxm_search_fill_xq", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_search_fill_types", _c_int, [_vp, _vp, _c_int, _vp]),
+    ("nbnxm_search_fill_ljcomb", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_init", _c_int, [_PTR(InteractionConst), _PTR(PairlistParams), _PTR(Atomdata), _c_int, _PTR(_vp)]),
+    ("nbnxm_gpu_init_atomdata", _c_int, [_vp, _PTR(Atomdata)]),
+    ("nbnxm_gpu_upload_shiftvec", _c_int, [_vp, _vp]),
+    ("nbnxm_gpu_init_pairlist", _c_int, [_vp, _PTR(PairlistGpu)]),
+    ("nbnxm_gpu_copy_xq_to_gpu", _c_int, [_vp, _vp, _c_int]),
+    ("nbnxm_gpu_clear_outputs", _c_int, [_vp, _c_int]),
+    ("nbnxm_gpu_launch_kernel", _c_int, [_vp, _c_int, _c_int]),
+    ("nbnxm_gpu_launch_kernel_pruneonly", _c_int, [_vp, _c_int]),
+    ("nbnxm_gpu_launch_cpyback", _c_int, [_vp, _vp, _vp]),
+    ("nbnxm_gpu_wait_finish_task", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_min_ci_balanced", _c_int, [_vp]),
+    ("nbnxm_gpu_is_kernel_ewald_analytical", _c_int, [_vp]),
+    ("nbnxm_gpu_get_masks", _c_int, [_vp, _vp, _vp]),
+    ("nbnxm_gpu_get_sci_order", _c_int, [_vp, _vp, _vp]),
+    ("nbnxm_gpu_set_waves_per_block", _c_int, [_vp, _c_int]),
+    ("nbnxm_gpu_free", None, [_vp]),
+]
+
+_lib = None
+
+
+def load():
+    """Loads the library (once) and declares every signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C gromacs-fmm_amd` "
+            "(there is no fallback implementation)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+class NbnxmError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().nbnxm_last_error().decode()
+        raise NbnxmError(f"{what}: error {rc}: {msg}")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data

@@ -1,0 +1,248 @@
+/*
+ * nbnxm_amd.h — C ABI of the MI355X (gfx950) nbnxm short-range cluster-pair library.
+ *
+ * Everything here is plain C: opaque handles, plain pointers and sizes.  The
+ * data layouts are the ones GROMACS uses on its GPU path, so a GROMACS build
+ * (or any other host) can hand its own buffers over without conversion:
+ *
+ *   - pairlist: sci / cj_packed / excl exactly as NbnxnPairlistGpu
+ *     (src/gromacs/nbnxm/pairlist.h:201-357) with the cluster-pair split
+ *     disabled, i.e. the layout GROMACS itself selects for 64-wide AMD
+ *     devices (GMX_GPU_NB_DISABLE_CLUSTER_PAIR_SPLIT,
+ *     src/gromacs/nbnxm/pairlistparams.h:88-98): one imask/excl_ind per
+ *     packed j-group and 64 interaction words per exclusion entry.
+ *   - atom data: xq float4 (x, y, z, q) per atom in grid order, padded to
+ *     whole clusters (src/gromacs/nbnxm/atomdata.cpp:629-638,736-790),
+ *     per-atom type index with numTypes-1 the dummy type, nbfp holding
+ *     (6*C6, 12*C12) per type pair (src/gromacs/nbnxm/atomdata.h:190).
+ *   - shift vectors: 45 x 3 floats, index 5*(3*(z+1)+y+1)+x+2
+ *     (src/gromacs/pbcutil/include/gromacs/pbcutil/ishift.h:43-62).
+ *
+ * All functions return 0 on success and a negative NBNXM_ERR_* code on
+ * failure; nbnxm_last_error() gives the message.  The reference aborts via
+ * gmx_fatal / CU_RET_ERR instead (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:140-148).
+ */
+#ifndef NBNXM_AMD_H
+#define NBNXM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- geometry constants (src/gromacs/nbnxm/pairlistparams.h:59-99, pairlist.h:141-183) ---- */
+#define NBNXM_CLUSTER_SIZE 8              /* atoms per i- and j-cluster */
+#define NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER 8 /* 2x2x2 clusters per super-cluster */
+#define NBNXM_JGROUP_SIZE 4               /* j-clusters per packed group (32 mask bits / 8) */
+#define NBNXM_EXCL_SIZE 64                /* interaction words per exclusion entry, split 1 */
+#define NBNXM_NUM_SHIFTS 45
+#define NBNXM_CENTRAL_SHIFT 22
+#define NBNXM_MIN_DISTANCE_SQUARED 3.82e-07f /* src/gromacs/nbnxm/pairlist.h:169 */
+#define NBNXM_SCI_HISTOGRAM_SIZE 8192     /* src/gromacs/nbnxm/gpu_types_common.h:72 */
+
+/* ---- error codes ---- */
+#define NBNXM_OK 0
+#define NBNXM_ERR_INVALID -1   /* bad argument (null pointer, size, layout) */
+#define NBNXM_ERR_HIP -2       /* HIP runtime failure */
+#define NBNXM_ERR_STATE -3     /* call out of order (e.g. launch before list upload) */
+#define NBNXM_ERR_UNSUPPORTED -4
+
+/* ---- list types: NbnxnPairlistGpu, split = 1 (src/gromacs/nbnxm/pairlist.h:201-283) ---- */
+typedef struct nbnxm_sci
+{
+    int sci;           /* i-super-cluster index */
+    int shift;         /* shift vector index (0..44); the ensemble adds 45*system */
+    int cjPackedBegin; /* first packed j-group */
+    int cjPackedEnd;   /* one past the last packed j-group */
+} nbnxm_sci_t;
+
+typedef struct nbnxm_im_ei
+{
+    unsigned int imask; /* bit jm*8+im: j-cluster jm of the group x i-cluster im of the sci */
+    int excl_ind;       /* index into the exclusion array, 0 = no exclusions */
+} nbnxm_im_ei_t;
+
+typedef struct nbnxm_cj_packed
+{
+    int cj[NBNXM_JGROUP_SIZE];
+    nbnxm_im_ei_t imei[1];
+} nbnxm_cj_packed_t;
+
+typedef struct nbnxm_excl
+{
+    unsigned int pair[NBNXM_EXCL_SIZE]; /* word jInner*8+iInner, bit jm*8+im */
+} nbnxm_excl_t;
+
+/* Host view of a GPU pair list (NbnxnPairlistGpu, src/gromacs/nbnxm/pairlist.h:324-357). */
+typedef struct nbnxm_pairlist_gpu
+{
+    int na_ci; /* must be NBNXM_CLUSTER_SIZE */
+    float rlist;
+    int nsci;
+    const nbnxm_sci_t* sci;
+    int ncjPacked;
+    const nbnxm_cj_packed_t* cjPacked;
+    int nexcl;
+    const nbnxm_excl_t* excl; /* excl[0] must be all ones */
+} nbnxm_pairlist_gpu_t;
+
+/* ---- interaction parameters ---- */
+enum nbnxm_vdw_type
+{
+    NBNXM_VDW_CUT_TYPE_TABLE = 0, /* LJ cut-off, per-type-pair nbfp table (VdwType::Cut) */
+    NBNXM_VDW_CUT_COMB_GEOM = 1,  /* LJ cut-off, geometric combination (VdwType::CutCombGeom) */
+    NBNXM_VDW_CUT_COMB_LB = 2     /* LJ cut-off, Lorentz-Berthelot (VdwType::CutCombLB) */
+};
+
+/* Fixed-point force accumulation: forces are summed as int64 multiples of 2^-NBNXM_FIXED_FRAC_BITS. */
+#define NBNXM_FIXED_FRAC_BITS 32
+
+enum nbnxm_accumulation
+{
+    NBNXM_ACCUM_FLOAT = 0,   /* FP32 atomics, order-dependent (the reference behaviour) */
+    NBNXM_ACCUM_FIXED64 = 1, /* FP32 pair math, int64 fixed-point forces, FP64 energy/fshift reduction */
+    NBNXM_ACCUM_FP64 = 2     /* FP64 pair math with exact erfc/exp (GPU oracle mode) */
+};
+
+/* Subset of interaction_const_t used by the kernel
+ * (src/gromacs/mdtypes/interaction_const.cpp:106-115,277-278,309). */
+typedef struct nbnxm_interaction_const
+{
+    double rcoulomb;
+    double rvdw;              /* must equal rcoulomb: no twin cut-off in this kernel */
+    double ewaldcoeff_q;      /* beta, from nbnxm_calc_ewaldcoeff_q */
+    double sh_ewald;          /* erfc(beta*rc)/rc with PotShift, else 0 */
+    double epsfac;            /* c_one4PiEps0 / epsilon_r */
+    double dispersion_shift_cpot; /* -rvdw^-6 with PotShift */
+    double repulsion_shift_cpot;  /* -rvdw^-12 with PotShift */
+} nbnxm_interaction_const_t;
+
+/* Pair-list parameters (src/gromacs/nbnxm/pairlistparams.h, PairlistParams). */
+typedef struct nbnxm_pairlist_params
+{
+    double rlistOuter;
+    double rlistInner;
+    int useDynamicPruning;
+    int numRollingPruningParts;
+} nbnxm_pairlist_params_t;
+
+/* Host view of the kernel inputs of nbnxn_atomdata_t (src/gromacs/nbnxm/atomdata.h:176-310). */
+typedef struct nbnxm_atomdata
+{
+    int numAtoms;         /* grid atoms including fillers, multiple of 8 */
+    const float* xq;      /* 4*numAtoms */
+    int numTypes;         /* including the trailing dummy type */
+    const int* type;      /* numAtoms; may be NULL with a combination rule */
+    const float* nbfp;    /* 2*numTypes*numTypes: 6*C6, 12*C12 */
+    int vdwType;          /* enum nbnxm_vdw_type */
+    const float* ljComb;  /* 2*numAtoms, required for COMB_GEOM / COMB_LB */
+    const float* shiftVec;/* 3*NBNXM_NUM_SHIFTS */
+} nbnxm_atomdata_t;
+
+/* ---------------------------------------------------------------------------
+ * Library-wide
+ * ------------------------------------------------------------------------- */
+/* Message for the last failing call on this thread. */
+const char* nbnxm_last_error(void);
+/* Library version string. */
+const char* nbnxm_version(void);
+/* Beta from the Ewald tolerance: restates calc_ewaldcoeff_q
+ * (src/gromacs/ewald/ewald_utils.cpp:43-71). */
+double nbnxm_calc_ewaldcoeff_q(double rc, double rtol);
+/* Fills ic for Ewald (analytical) + LJ cut-off with PotShift on both
+ * (src/gromacs/mdtypes/interaction_const.cpp:106-115,266-278,306-314). */
+int nbnxm_init_interaction_const(double rc, double ewaldRtol, double epsilonR, nbnxm_interaction_const_t* ic);
+/* Shift vectors of a rectangular box (calc_shifts, src/gromacs/pbcutil/pbc.cpp:1218-1233). */
+int nbnxm_calc_shifts(const double box[3], float* shiftVec /* 3*45 */);
+
+/* ---------------------------------------------------------------------------
+ * Pair search: grid of 8-atom clusters and the GPU pair list.
+ * Replaces nonbonded_verlet_t::putAtomsOnGrid (src/gromacs/nbnxm/nbnxm.cpp) and
+ * constructPairlist (src/gromacs/nbnxm/pairlist.cpp:4353) for the GPU list
+ * type; the list semantics are those of make_cluster_list_supersub
+ * (pairlist.cpp:1122-1270), setExclusionsForIEntry (pairlist.cpp:1948-2046)
+ * and split_sci_entry (pairlist.cpp:2150-2262).
+ * ------------------------------------------------------------------------- */
+typedef struct nbnxm_search nbnxm_search_t;
+
+nbnxm_search_t* nbnxm_search_create(void);
+void nbnxm_search_free(nbnxm_search_t* s);
+/* Wraps x (3*numAtoms) into the rectangular box and sorts atoms into clusters. */
+int nbnxm_search_put_atoms_on_grid(nbnxm_search_t* s, const double box[3], int numAtoms, const double* x);
+/* Builds the list for rlist; exclusions as ListOfLists<int> (ranges numAtoms+1,
+ * elements), both may be NULL.  maxPairsPerEntry > 0 splits long entries.
+ * Every box edge must exceed rlist. */
+int nbnxm_search_construct_pairlist(nbnxm_search_t* s, double rlist, const int* exclRanges, const int* exclElements, int maxPairsPerEntry);
+/* View of the list; valid until the next construct or free. */
+int nbnxm_search_get_pairlist(const nbnxm_search_t* s, nbnxm_pairlist_gpu_t* list);
+/* Grid atoms including fillers (multiple of 64). */
+int nbnxm_search_num_grid_atoms(const nbnxm_search_t* s);
+/* Grid slot -> original atom index, -1 for fillers (gridSet.atomIndices()). */
+int nbnxm_search_atom_indices(const nbnxm_search_t* s, int* atomIndices);
+int nbnxm_search_shift_vec(const nbnxm_search_t* s, float* shiftVec /* 3*45 */);
+/* Grid-order xq (4 per slot) from original-order x (3 per atom, NULL = the
+ * wrapped grid coordinates) and q (NULL = 0); fillers at -1e6 with q = 0
+ * (copy_rvec_to_nbat_real, src/gromacs/nbnxm/atomdata.cpp:136-259). */
+int nbnxm_search_fill_xq(const nbnxm_search_t* s, const double* x, const double* q, float* xq);
+int nbnxm_search_fill_types(const nbnxm_search_t* s, const int* type, int dummyType, int* slotType);
+int nbnxm_search_fill_ljcomb(const nbnxm_search_t* s, const int* type, const float* perTypeComb, float* slotComb);
+
+/* ---------------------------------------------------------------------------
+ * Single-system GPU task: the Nbnxm::gpu_* API
+ * (src/gromacs/nbnxm/nbnxm_gpu.h:66-302, src/gromacs/nbnxm/gpu_data_mgmt.h).
+ * One HIP stream per task; calls are asynchronous except wait_finish_task and
+ * the getters.
+ * ------------------------------------------------------------------------- */
+typedef struct nbnxm_gpu nbnxm_gpu_t;
+
+/* gpu_init (src/gromacs/nbnxm/nbnxm_gpu_data_mgmt.cpp:442): parameters, atom
+ * data (as gpu_init_atomdata) and shift vectors when nbat->shiftVec is set. */
+int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
+                   const nbnxm_pairlist_params_t* listParams,
+                   const nbnxm_atomdata_t* nbat,
+                   int accumulation,
+                   nbnxm_gpu_t** nb);
+/* gpu_init_atomdata (nbnxm_gpu_data_mgmt.cpp:689). */
+int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat);
+/* gpu_upload_shiftvec (nbnxm_gpu_data_mgmt.cpp:528). */
+int nbnxm_gpu_upload_shiftvec(nbnxm_gpu_t* nb, const float* shiftVec);
+/* gpu_init_pairlist (nbnxm_gpu_data_mgmt.cpp:548); marks the list fresh. */
+int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list);
+/* gpu_copy_xq_to_gpu (nbnxm_gpu_data_mgmt.cpp:1006). */
+int nbnxm_gpu_copy_xq_to_gpu(nbnxm_gpu_t* nb, const float* xq, int numAtoms);
+/* gpu_clear_outputs (nbnxm_gpu_data_mgmt.cpp:792). */
+int nbnxm_gpu_clear_outputs(nbnxm_gpu_t* nb, int computeVirial);
+/* gpu_launch_kernel (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:478): prunes a fresh
+ * list first, then the nonbonded kernel (F or VF). */
+int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeVirial);
+/* gpu_launch_kernel_pruneonly (nbnxm_cuda.cu:606): fresh list -> outer and inner
+ * masks plus the sci ordering; otherwise one rolling part of numParts. */
+int nbnxm_gpu_launch_kernel_pruneonly(nbnxm_gpu_t* nb, int numParts);
+/* gpu_launch_cpyback (nbnxm_gpu_data_mgmt.cpp:853): forces in grid order to f
+ * (3*numAtoms floats) and/or fDouble; either may be NULL. */
+int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble);
+/* gpu_wait_finish_task (src/gromacs/nbnxm/gpu_common.h:345): waits and adds the
+ * energies and shift forces (3*45) of the last launch to the outputs. */
+int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift);
+/* gpu_min_ci_balanced (gpu_data_mgmt.h:119). */
+int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb);
+/* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): always 1 here. */
+int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb);
+/* Downloads the per-group masks (cjPacked imask and the rlistOuter mask,
+ * gpu_plist::imask); no counterpart in the reference, used by tests. */
+int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int* imaskOuter);
+/* Downloads the entry work order and histogram bins after a fresh prune
+ * (gpuPlistSorting::sciSorted / sciCount). */
+int nbnxm_gpu_get_sci_order(nbnxm_gpu_t* nb, int* order, int* sciCount);
+/* Launch shape of the nonbonded kernel: waves (entries) per workgroup, 1..4. */
+int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock);
+/* gpu_free. */
+void nbnxm_gpu_free(nbnxm_gpu_t* nb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBNXM_AMD_H */

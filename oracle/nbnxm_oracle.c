@@ -1,0 +1,803 @@
+/*
+ * nbnxm_oracle.c — CPU oracle for the nbnxm cluster-pair kernel.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for tests, the smoke check and the
+ * benchmark's CPU-baseline leg.  The product library never links this file.
+ * Built with -ffp-contract=off so that every fused multiply-add below is the
+ * explicit fmaf()/fma() written in the source and nothing else.
+ *
+ * Citations are relative to the GROMACS-FMM tree.
+ */
+#include "nbnxm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define CL NBNXM_CLUSTER_SIZE
+#define NCPS NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER
+#define JG NBNXM_JGROUP_SIZE
+
+/* c_one4PiEps0 is not needed here: epsfac arrives in the interaction constants. */
+static const double c_oneSqrtPi = 0.56418958354775628695; /* 1/sqrt(pi) */
+
+/* Restates calc_ewaldcoeff_q (src/gromacs/ewald/ewald_utils.cpp:43-71):
+ * double beta until erfc(beta*rc) <= rtol, then bisect for i+60 steps. */
+double oracle_calc_ewaldcoeff_q(double rc, double rtol)
+{
+    double beta = 5.0;
+    int i = 0;
+    do
+    {
+        i++;
+        beta *= 2.0;
+    } while (erfc(beta * rc) > rtol);
+
+    const int n = i + 60;
+    double low = 0.0;
+    double high = beta;
+    for (int k = 0; k < n; k++)
+    {
+        beta = 0.5 * (low + high);
+        if (erfc(beta * rc) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
+/* Restates pmecorrF (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_utils.cuh:407-443),
+ * the single-precision rational of pmeForceCorrection
+ * (src/gromacs/simd/include/gromacs/simd/simd_math.h:1611-1648), in the same
+ * evaluation order with every a*b+c written as an explicit fused multiply-add.
+ * The division is IEEE 1/x here; the HIP kernel uses the hardware reciprocal. */
+float oracle_pmecorrF(float z2)
+{
+    const float FN6 = -1.7357322914161492954e-8F;
+    const float FN5 = 1.4703624142580877519e-6F;
+    const float FN4 = -0.000053401640219807709149F;
+    const float FN3 = 0.0010054721316683106153F;
+    const float FN2 = -0.019278317264888380590F;
+    const float FN1 = 0.069670166153766424023F;
+    const float FN0 = -0.75225204789749321333F;
+
+    const float FD4 = 0.0011193462567257629232F;
+    const float FD3 = 0.014866955030185295499F;
+    const float FD2 = 0.11583842382862377919F;
+    const float FD1 = 0.50736591960530292870F;
+    const float FD0 = 1.0F;
+
+    const float z4 = z2 * z2;
+
+    float polyFD0 = fmaf(FD4, z4, FD2);
+    float polyFD1 = fmaf(FD3, z4, FD1);
+    polyFD0 = fmaf(polyFD0, z4, FD0);
+    polyFD0 = fmaf(polyFD1, z2, polyFD0);
+    polyFD0 = 1.0F / polyFD0;
+
+    float polyFN0 = fmaf(FN6, z4, FN4);
+    float polyFN1 = fmaf(FN5, z4, FN3);
+    polyFN0 = fmaf(polyFN0, z4, FN2);
+    polyFN1 = fmaf(polyFN1, z4, FN1);
+    polyFN0 = fmaf(polyFN0, z4, FN0);
+    polyFN0 = fmaf(polyFN1, z2, polyFN0);
+
+    return polyFN0 * polyFD0;
+}
+
+static int check_inputs(const nbnxm_pairlist_gpu_t* list, const nbnxm_atomdata_t* nbat)
+{
+    if (!list || !nbat || !nbat->xq || !nbat->shiftVec)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    if (list->na_ci != CL || (list->nsci > 0 && (!list->sci || !list->cjPacked || !list->excl)))
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE && (!nbat->type || !nbat->nbfp))
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    if (nbat->vdwType != NBNXM_VDW_CUT_TYPE_TABLE && !nbat->ljComb)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    return NBNXM_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * FP32 restatement of the CUDA kernel body
+ * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:325-718, EL_EWALD_ANA with
+ * LJ cut-off; LJ parameters as :514-533).
+ * ---------------------------------------------------------------------- */
+int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
+                      const nbnxm_atomdata_t* nbat,
+                      const nbnxm_interaction_const_t* ic,
+                      int computeEnergy,
+                      float* f,
+                      float* fshift,
+                      double* Vvdw,
+                      double* Vc)
+{
+    int rc = check_inputs(list, nbat);
+    if (rc != NBNXM_OK || !ic || !f || !fshift)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const float* xq = nbat->xq;
+    const float epsfac = (float)ic->epsfac;
+    const float rcoulomb_sq = (float)(ic->rcoulomb * ic->rcoulomb);
+    const float beta = (float)ic->ewaldcoeff_q;
+    const float beta2 = beta * beta;
+    const float beta3 = beta * beta * beta;
+    const float ewald_shift = (float)ic->sh_ewald;
+    const float cpot_rep = (float)ic->repulsion_shift_cpot;
+    const float cpot_disp = (float)ic->dispersion_shift_cpot;
+    const float oneSixth = 1.0F / 6.0F;
+    const float oneTwelveth = 1.0F / 12.0F;
+    const int ntypes = nbat->numTypes;
+    double vvdwTot = 0.0;
+    double vcTot = 0.0;
+
+    for (int e = 0; e < list->nsci; e++)
+    {
+        const nbnxm_sci_t s = list->sci[e];
+        const float* sh = nbat->shiftVec + 3 * s.shift;
+        float E_lj = 0.0F;
+        float E_el = 0.0F;
+
+        if (computeEnergy && s.shift == NBNXM_CENTRAL_SHIFT && s.cjPackedEnd > s.cjPackedBegin
+            && list->cjPacked[s.cjPackedBegin].cj[0] == s.sci * NCPS)
+        {
+            /* Self energy on the diagonal entry (:381-418). */
+            float sumq2 = 0.0F;
+            for (int a = 0; a < NCPS * CL; a++)
+            {
+                const float qi = xq[4 * (s.sci * NCPS * CL + a) + 3] * epsfac;
+                sumq2 += qi * qi;
+            }
+            E_el += -(sumq2 / epsfac) * beta * (float)c_oneSqrtPi;
+        }
+
+        for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+        {
+            const nbnxm_cj_packed_t* cjp = &list->cjPacked[jp];
+            const unsigned int imask = cjp->imei[0].imask;
+            const nbnxm_excl_t* ex = &list->excl[cjp->imei[0].excl_ind];
+            for (int jm = 0; jm < JG; jm++)
+            {
+                if (!((imask >> (jm * NCPS)) & 0xFFU))
+                {
+                    continue;
+                }
+                const int cj = cjp->cj[jm];
+                for (int im = 0; im < NCPS; im++)
+                {
+                    const int bit = jm * NCPS + im;
+                    if (!((imask >> bit) & 1U))
+                    {
+                        continue;
+                    }
+                    const int ci = s.sci * NCPS + im;
+                    for (int ti = 0; ti < CL; ti++)
+                    {
+                        const int ai = ci * CL + ti;
+                        const float xi = xq[4 * ai + 0] + sh[0];
+                        const float yi = xq[4 * ai + 1] + sh[1];
+                        const float zi = xq[4 * ai + 2] + sh[2];
+                        const float qi = xq[4 * ai + 3] * epsfac;
+                        float fix = 0.0F, fiy = 0.0F, fiz = 0.0F;
+                        for (int tj = 0; tj < CL; tj++)
+                        {
+                            const int aj = cj * CL + tj;
+                            const float dx = xi - xq[4 * aj + 0];
+                            const float dy = yi - xq[4 * aj + 1];
+                            const float dz = zi - xq[4 * aj + 2];
+                            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            const int nonSelf = !(s.shift == NBNXM_CENTRAL_SHIFT && tj <= ti);
+                            if (!(r2 < rcoulomb_sq && (nonSelf || ci != cj)))
+                            {
+                                continue;
+                            }
+                            const float int_bit = ((ex->pair[tj * CL + ti] >> bit) & 1U) ? 1.0F : 0.0F;
+                            const float qj = xq[4 * aj + 3];
+                            float c6, c12;
+                            int lbForceOnly = 0;
+                            float sigma = 0.0F, epsilon = 0.0F;
+                            if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                            {
+                                const int idx = ntypes * nbat->type[ai] + nbat->type[aj];
+                                c6 = nbat->nbfp[2 * idx];
+                                c12 = nbat->nbfp[2 * idx + 1];
+                            }
+                            else if (nbat->vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                            {
+                                c6 = nbat->ljComb[2 * ai] * nbat->ljComb[2 * aj];
+                                c12 = nbat->ljComb[2 * ai + 1] * nbat->ljComb[2 * aj + 1];
+                            }
+                            else
+                            {
+                                sigma = nbat->ljComb[2 * ai] + nbat->ljComb[2 * aj];
+                                epsilon = nbat->ljComb[2 * ai + 1] * nbat->ljComb[2 * aj + 1];
+                                /* convert_sigma_epsilon_to_c6_c12 (nbnxm_cuda_kernel_utils.cuh) */
+                                const float sigma2 = sigma * sigma;
+                                const float sigma6 = sigma2 * sigma2 * sigma2;
+                                c6 = epsilon * sigma6;
+                                c12 = c6 * sigma6;
+                                lbForceOnly = !computeEnergy;
+                            }
+                            r2 = fmaxf(r2, NBNXM_MIN_DISTANCE_SQUARED);
+                            const float inv_r = 1.0F / sqrtf(r2);
+                            const float inv_r2 = inv_r * inv_r;
+                            float F_invr;
+                            float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                            inv_r6 *= int_bit;
+                            if (lbForceOnly)
+                            {
+                                /* LB force-only branch (:555-562) */
+                                const float sig_r = sigma * inv_r;
+                                const float sig_r2 = sig_r * sig_r;
+                                float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+                                sig_r6 *= int_bit;
+                                F_invr = epsilon * sig_r6 * (sig_r6 - 1.0F) * inv_r2;
+                            }
+                            else
+                            {
+                                F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                            }
+                            if (computeEnergy)
+                            {
+                                E_lj += int_bit
+                                        * (c12 * (inv_r6 * inv_r6 + cpot_rep) * oneTwelveth
+                                           - c6 * (inv_r6 + cpot_disp) * oneSixth);
+                            }
+                            F_invr += qi * qj * (int_bit * inv_r2 * inv_r + oracle_pmecorrF(beta2 * r2) * beta3);
+                            if (computeEnergy)
+                            {
+                                E_el += qi * qj * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewald_shift);
+                            }
+                            const float tx = dx * F_invr;
+                            const float ty = dy * F_invr;
+                            const float tz = dz * F_invr;
+                            fix += tx;
+                            fiy += ty;
+                            fiz += tz;
+                            f[3 * aj + 0] -= tx;
+                            f[3 * aj + 1] -= ty;
+                            f[3 * aj + 2] -= tz;
+                        }
+                        f[3 * ai + 0] += fix;
+                        f[3 * ai + 1] += fiy;
+                        f[3 * ai + 2] += fiz;
+                        if (s.shift != NBNXM_CENTRAL_SHIFT)
+                        {
+                            fshift[3 * s.shift + 0] += fix;
+                            fshift[3 * s.shift + 1] += fiy;
+                            fshift[3 * s.shift + 2] += fiz;
+                        }
+                    }
+                }
+            }
+        }
+        vvdwTot += E_lj;
+        vcTot += E_el;
+    }
+    if (computeEnergy)
+    {
+        if (Vvdw)
+        {
+            *Vvdw += vvdwTot;
+        }
+        if (Vc)
+        {
+            *Vc += vcTot;
+        }
+    }
+    return NBNXM_OK;
+}
+
+/* Ewald real-space pair terms in FP64 with exact special functions.
+ * fOverR gets qq*(int_bit/r^3 + [2 beta/sqrt(pi) exp(-beta^2 r^2)/r^2 - erf(beta r)/r^3]),
+ * the exact counterpart of :639-640; vOut qq*((int_bit - erf(beta r))/r - int_bit*sh_ewald) (:656-659). */
+static void ewald_pair_f64(double qq, double r2, double int_bit, double beta, double sh_ewald, double* fOverR, double* vOut)
+{
+    const double r = sqrt(r2);
+    const double inv_r = 1.0 / r;
+    const double inv_r2 = inv_r * inv_r;
+    const double br = beta * r;
+    const double erfbr = erf(br);
+    const double corr = 2.0 * beta * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r;
+    *fOverR = qq * (int_bit * inv_r2 * inv_r + corr);
+    *vOut = qq * (inv_r * (int_bit - erfbr) - int_bit * sh_ewald);
+}
+
+int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
+                      const nbnxm_atomdata_t* nbat,
+                      const nbnxm_interaction_const_t* ic,
+                      int computeEnergy,
+                      double* f,
+                      double* fshift,
+                      double* Vvdw,
+                      double* Vc)
+{
+    int rc = check_inputs(list, nbat);
+    if (rc != NBNXM_OK || !ic || !f || !fshift)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const float* xq = nbat->xq;
+    const double epsfac = ic->epsfac;
+    const double rcoulomb_sq = ic->rcoulomb * ic->rcoulomb;
+    const double beta = ic->ewaldcoeff_q;
+    const int ntypes = nbat->numTypes;
+    double vvdwTot = 0.0;
+    double vcTot = 0.0;
+
+    for (int e = 0; e < list->nsci; e++)
+    {
+        const nbnxm_sci_t s = list->sci[e];
+        const float* sh = nbat->shiftVec + 3 * s.shift;
+
+        if (computeEnergy && s.shift == NBNXM_CENTRAL_SHIFT && s.cjPackedEnd > s.cjPackedBegin
+            && list->cjPacked[s.cjPackedBegin].cj[0] == s.sci * NCPS)
+        {
+            double sumq2 = 0.0;
+            for (int a = 0; a < NCPS * CL; a++)
+            {
+                const double q = xq[4 * (s.sci * NCPS * CL + a) + 3];
+                sumq2 += q * q;
+            }
+            vcTot += -epsfac * sumq2 * beta * c_oneSqrtPi;
+        }
+
+        for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+        {
+            const nbnxm_cj_packed_t* cjp = &list->cjPacked[jp];
+            const unsigned int imask = cjp->imei[0].imask;
+            const nbnxm_excl_t* ex = &list->excl[cjp->imei[0].excl_ind];
+            for (int jm = 0; jm < JG; jm++)
+            {
+                const int cj = cjp->cj[jm];
+                for (int im = 0; im < NCPS; im++)
+                {
+                    const int bit = jm * NCPS + im;
+                    if (!((imask >> bit) & 1U))
+                    {
+                        continue;
+                    }
+                    const int ci = s.sci * NCPS + im;
+                    for (int ti = 0; ti < CL; ti++)
+                    {
+                        const int ai = ci * CL + ti;
+                        const double xi = (double)xq[4 * ai + 0] + (double)sh[0];
+                        const double yi = (double)xq[4 * ai + 1] + (double)sh[1];
+                        const double zi = (double)xq[4 * ai + 2] + (double)sh[2];
+                        const double qi = (double)xq[4 * ai + 3] * epsfac;
+                        double fix = 0.0, fiy = 0.0, fiz = 0.0;
+                        for (int tj = 0; tj < CL; tj++)
+                        {
+                            const int aj = cj * CL + tj;
+                            const double dx = xi - (double)xq[4 * aj + 0];
+                            const double dy = yi - (double)xq[4 * aj + 1];
+                            const double dz = zi - (double)xq[4 * aj + 2];
+                            double r2 = dx * dx + dy * dy + dz * dz;
+                            const int nonSelf = !(s.shift == NBNXM_CENTRAL_SHIFT && tj <= ti);
+                            if (!(r2 < rcoulomb_sq && (nonSelf || ci != cj)))
+                            {
+                                continue;
+                            }
+                            const double int_bit = ((ex->pair[tj * CL + ti] >> bit) & 1U) ? 1.0 : 0.0;
+                            const double qj = xq[4 * aj + 3];
+                            double c6, c12;
+                            if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                            {
+                                const int idx = ntypes * nbat->type[ai] + nbat->type[aj];
+                                c6 = nbat->nbfp[2 * idx];
+                                c12 = nbat->nbfp[2 * idx + 1];
+                            }
+                            else if (nbat->vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                            {
+                                c6 = (double)nbat->ljComb[2 * ai] * (double)nbat->ljComb[2 * aj];
+                                c12 = (double)nbat->ljComb[2 * ai + 1] * (double)nbat->ljComb[2 * aj + 1];
+                            }
+                            else
+                            {
+                                const double sigma = (double)nbat->ljComb[2 * ai] + (double)nbat->ljComb[2 * aj];
+                                const double epsilon = (double)nbat->ljComb[2 * ai + 1] * (double)nbat->ljComb[2 * aj + 1];
+                                const double sigma6 = sigma * sigma * sigma * sigma * sigma * sigma;
+                                c6 = epsilon * sigma6;
+                                c12 = c6 * sigma6;
+                            }
+                            if (r2 < NBNXM_MIN_DISTANCE_SQUARED)
+                            {
+                                r2 = NBNXM_MIN_DISTANCE_SQUARED;
+                            }
+                            const double inv_r2 = 1.0 / r2;
+                            const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
+                            double F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                            double fc, vc;
+                            ewald_pair_f64(qi * qj, r2, int_bit, beta, ic->sh_ewald, &fc, &vc);
+                            F_invr += fc;
+                            if (computeEnergy)
+                            {
+                                vvdwTot += int_bit
+                                           * (c12 * (inv_r6 * inv_r6 + ic->repulsion_shift_cpot) / 12.0
+                                              - c6 * (inv_r6 + ic->dispersion_shift_cpot) / 6.0);
+                                vcTot += vc;
+                            }
+                            const double tx = dx * F_invr;
+                            const double ty = dy * F_invr;
+                            const double tz = dz * F_invr;
+                            fix += tx;
+                            fiy += ty;
+                            fiz += tz;
+                            f[3 * aj + 0] -= tx;
+                            f[3 * aj + 1] -= ty;
+                            f[3 * aj + 2] -= tz;
+                        }
+                        f[3 * ai + 0] += fix;
+                        f[3 * ai + 1] += fiy;
+                        f[3 * ai + 2] += fiz;
+                        if (s.shift != NBNXM_CENTRAL_SHIFT)
+                        {
+                            fshift[3 * s.shift + 0] += fix;
+                            fshift[3 * s.shift + 1] += fiy;
+                            fshift[3 * s.shift + 2] += fiz;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (computeEnergy)
+    {
+        if (Vvdw)
+        {
+            *Vvdw += vvdwTot;
+        }
+        if (Vc)
+        {
+            *Vc += vcTot;
+        }
+    }
+    return NBNXM_OK;
+}
+
+static int is_excluded(int i, int j, const int* ranges, const int* elems)
+{
+    if (!ranges)
+    {
+        return 0;
+    }
+    for (int k = ranges[i]; k < ranges[i + 1]; k++)
+    {
+        if (elems[k] == j)
+        {
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int oracle_bruteforce_f64(int n,
+                          const double* x,
+                          const double* q,
+                          const int* type,
+                          int numTypes,
+                          const double* nbfp,
+                          const int* exclRanges,
+                          const int* exclElements,
+                          const double box[3],
+                          const nbnxm_interaction_const_t* ic,
+                          double* f,
+                          double* Vvdw,
+                          double* Vc,
+                          long long* numPairsInRange)
+{
+    if (n < 0 || !x || !q || !type || !nbfp || !box || !ic || !f)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const double rc = ic->rcoulomb;
+    for (int d = 0; d < 3; d++)
+    {
+        if (!(box[d] > rc))
+        {
+            return NBNXM_ERR_INVALID;
+        }
+    }
+    const double rc2 = rc * rc;
+    const double beta = ic->ewaldcoeff_q;
+    const double epsfac = ic->epsfac;
+    double vvdw = 0.0, vc = 0.0;
+    long long npairs = 0;
+
+    /* Wrap into the unit cell so every separation is within (-L, L). */
+    double* xw = (double*)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    if (!xw)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    for (int i = 0; i < n; i++)
+    {
+        for (int d = 0; d < 3; d++)
+        {
+            double v = fmod(x[3 * i + d], box[d]);
+            if (v < 0)
+            {
+                v += box[d];
+            }
+            xw[3 * i + d] = v;
+        }
+    }
+
+    double sumq2 = 0.0;
+    for (int i = 0; i < n; i++)
+    {
+        sumq2 += q[i] * q[i];
+        for (int j = i + 1; j < n; j++)
+        {
+            const int excluded = is_excluded(i, j, exclRanges, exclElements);
+            const double qq = epsfac * q[i] * q[j];
+            const double c6 = nbfp[2 * (numTypes * type[i] + type[j])];
+            const double c12 = nbfp[2 * (numTypes * type[i] + type[j]) + 1];
+            for (int sz = -1; sz <= 1; sz++)
+            {
+                for (int sy = -1; sy <= 1; sy++)
+                {
+                    for (int sx = -1; sx <= 1; sx++)
+                    {
+                        const double dx = xw[3 * i + 0] - xw[3 * j + 0] + sx * box[0];
+                        const double dy = xw[3 * i + 1] - xw[3 * j + 1] + sy * box[1];
+                        const double dz = xw[3 * i + 2] - xw[3 * j + 2] + sz * box[2];
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 >= rc2)
+                        {
+                            continue;
+                        }
+                        const double int_bit = excluded ? 0.0 : 1.0;
+                        if (!excluded)
+                        {
+                            npairs++;
+                        }
+                        if (r2 < NBNXM_MIN_DISTANCE_SQUARED)
+                        {
+                            r2 = NBNXM_MIN_DISTANCE_SQUARED;
+                        }
+                        const double inv_r2 = 1.0 / r2;
+                        const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
+                        double fr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                        double fcoul, vcoul;
+                        ewald_pair_f64(qq, r2, int_bit, beta, ic->sh_ewald, &fcoul, &vcoul);
+                        fr += fcoul;
+                        vvdw += int_bit
+                                * (c12 * (inv_r6 * inv_r6 + ic->repulsion_shift_cpot) / 12.0
+                                   - c6 * (inv_r6 + ic->dispersion_shift_cpot) / 6.0);
+                        vc += vcoul;
+                        f[3 * i + 0] += fr * dx;
+                        f[3 * i + 1] += fr * dy;
+                        f[3 * i + 2] += fr * dz;
+                        f[3 * j + 0] -= fr * dx;
+                        f[3 * j + 1] -= fr * dy;
+                        f[3 * j + 2] -= fr * dz;
+                    }
+                }
+            }
+        }
+    }
+    vc += -epsfac * beta * c_oneSqrtPi * sumq2;
+    free(xw);
+    if (Vvdw)
+    {
+        *Vvdw += vvdw;
+    }
+    if (Vc)
+    {
+        *Vc += vc;
+    }
+    if (numPairsInRange)
+    {
+        *numPairsInRange = npairs;
+    }
+    return NBNXM_OK;
+}
+
+/* Restates nbnxn_kernel_prune_cuda<haveFreshList>
+ * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343) for split 1:
+ * the any-pair-within-distance test of :283-299 with r2 evaluated as
+ * fmaf(dz,dz,fmaf(dy,dy,dx*dx)) after adding the shift to x_i in FP32. */
+int oracle_prune(const nbnxm_pairlist_gpu_t* list,
+                 const float* xq,
+                 const float* shiftVec,
+                 float rlistOuter_sq,
+                 float rlistInner_sq,
+                 int haveFreshList,
+                 int part,
+                 int numParts,
+                 unsigned int* imaskOuter,
+                 unsigned int* imaskInner,
+                 int* sciCount)
+{
+    if (!list || !xq || !shiftVec || !imaskOuter || !imaskInner || numParts < 1 || part < 0 || part >= numParts)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const int numSciInPart = (list->nsci - part + numParts - 1) / numParts;
+    for (int b = 0; b < numSciInPart; b++)
+    {
+        const int e = b * numParts + part;
+        const nbnxm_sci_t s = list->sci[e];
+        const float* sh = shiftVec + 3 * s.shift;
+        int prunedPairCount = 0;
+        for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+        {
+            unsigned int imaskFull, imaskCheck, imaskNew;
+            if (haveFreshList)
+            {
+                imaskFull = imaskInner[jp];
+                imaskCheck = imaskFull;
+                imaskNew = 0;
+            }
+            else
+            {
+                imaskFull = imaskOuter[jp];
+                imaskNew = imaskInner[jp];
+                imaskCheck = imaskNew ^ imaskFull;
+            }
+            if (!imaskCheck)
+            {
+                continue;
+            }
+            for (int jm = 0; jm < JG; jm++)
+            {
+                const int cj = list->cjPacked[jp].cj[jm];
+                for (int im = 0; im < NCPS; im++)
+                {
+                    const unsigned int mask_ji = 1U << (jm * NCPS + im);
+                    if (!(imaskCheck & mask_ji))
+                    {
+                        continue;
+                    }
+                    const int ci = s.sci * NCPS + im;
+                    int anyOuter = 0, anyInner = 0;
+                    for (int tj = 0; tj < CL; tj++)
+                    {
+                        const int aj = cj * CL + tj;
+                        for (int ti = 0; ti < CL; ti++)
+                        {
+                            const int ai = ci * CL + ti;
+                            const float xi = xq[4 * ai + 0] + sh[0];
+                            const float yi = xq[4 * ai + 1] + sh[1];
+                            const float zi = xq[4 * ai + 2] + sh[2];
+                            const float dx = xi - xq[4 * aj + 0];
+                            const float dy = yi - xq[4 * aj + 1];
+                            const float dz = zi - xq[4 * aj + 2];
+                            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            anyOuter |= (r2 < rlistOuter_sq);
+                            anyInner |= (r2 < rlistInner_sq);
+                        }
+                    }
+                    if (haveFreshList && !anyOuter)
+                    {
+                        imaskFull &= ~mask_ji;
+                    }
+                    if (anyInner)
+                    {
+                        imaskNew |= mask_ji;
+                    }
+                }
+            }
+            if (haveFreshList)
+            {
+                imaskOuter[jp] = imaskFull;
+                prunedPairCount += __builtin_popcount(imaskNew);
+            }
+            imaskInner[jp] = imaskNew;
+        }
+        if (haveFreshList && sciCount)
+        {
+            int index = NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1;
+            sciCount[e] = index > 0 ? index : 0;
+        }
+    }
+    return NBNXM_OK;
+}
+
+int oracle_sci_sort(int nsci, const nbnxm_sci_t* sci, const int* sciCount, nbnxm_sci_t* sorted)
+{
+    if (nsci < 0 || (nsci > 0 && (!sci || !sciCount || !sorted)))
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    int* offset = (int*)calloc(NBNXM_SCI_HISTOGRAM_SIZE + 1, sizeof(int));
+    if (!offset)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    for (int i = 0; i < nsci; i++)
+    {
+        offset[sciCount[i] + 1]++;
+    }
+    for (int b = 0; b < NBNXM_SCI_HISTOGRAM_SIZE; b++)
+    {
+        offset[b + 1] += offset[b];
+    }
+    for (int i = 0; i < nsci; i++)
+    {
+        sorted[offset[sciCount[i]]++] = sci[i];
+    }
+    free(offset);
+    return NBNXM_OK;
+}
+
+long long oracle_count_pairs_f64(const nbnxm_pairlist_gpu_t* list, const nbnxm_atomdata_t* nbat, double rc)
+{
+    if (!list || !nbat || !nbat->xq || !nbat->shiftVec)
+    {
+        return -1;
+    }
+    const float* xq = nbat->xq;
+    const double rc2 = rc * rc;
+    long long count = 0;
+    for (int e = 0; e < list->nsci; e++)
+    {
+        const nbnxm_sci_t s = list->sci[e];
+        const float* sh = nbat->shiftVec + 3 * s.shift;
+        for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+        {
+            const nbnxm_cj_packed_t* cjp = &list->cjPacked[jp];
+            const nbnxm_excl_t* ex = &list->excl[cjp->imei[0].excl_ind];
+            for (int jm = 0; jm < JG; jm++)
+            {
+                for (int im = 0; im < NCPS; im++)
+                {
+                    const int bit = jm * NCPS + im;
+                    if (!((cjp->imei[0].imask >> bit) & 1U))
+                    {
+                        continue;
+                    }
+                    const int ci = s.sci * NCPS + im;
+                    const int cj = cjp->cj[jm];
+                    for (int ti = 0; ti < CL; ti++)
+                    {
+                        const int ai = ci * CL + ti;
+                        if (xq[4 * ai] <= -1.0e5F)
+                        {
+                            continue; /* filler */
+                        }
+                        for (int tj = 0; tj < CL; tj++)
+                        {
+                            const int aj = cj * CL + tj;
+                            if (xq[4 * aj] <= -1.0e5F)
+                            {
+                                continue;
+                            }
+                            if (s.shift == NBNXM_CENTRAL_SHIFT && ci == cj && tj <= ti)
+                            {
+                                continue;
+                            }
+                            if (!((ex->pair[tj * CL + ti] >> bit) & 1U))
+                            {
+                                continue;
+                            }
+                            const double dx = (double)xq[4 * ai] + sh[0] - (double)xq[4 * aj];
+                            const double dy = (double)xq[4 * ai + 1] + sh[1] - (double)xq[4 * aj + 1];
+                            const double dz = (double)xq[4 * ai + 2] + sh[2] - (double)xq[4 * aj + 2];
+                            if (dx * dx + dy * dy + dz * dz < rc2)
+                            {
+                                count++;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return count;
+}
