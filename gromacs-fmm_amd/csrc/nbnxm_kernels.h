@@ -76,6 +76,8 @@ hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t strea
 // Converts the fixed-point forces to float (optionally double) and clears the
 // accumulator for the next evaluation.
 hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream);
+// Converts the fixed-point forces of slots [first, first+count).
+hipError_t launchConvertForcesRange(unsigned long long* fFixed, int first, int count, float* fOut, double* fOutDouble, hipStream_t stream);
 
 // Per-system FP64 reduction of entry partials.  entryRanges: numSystems+1
 // offsets into entryList (entries of each system), shiftRanges: 45*numSystems+1
@@ -91,8 +93,13 @@ hipError_t launchReduceEnergies(const double* entryPartials,
                                 int computeFshift,
                                 hipStream_t stream);
 
-// x_k = x_0 + N(0, sigma) for real atoms (q and fillers kept), Philox4x32-10 counter RNG.
-hipError_t launchPerturb(const float4* x0, float4* xq, const int* atomIsReal, int numAtoms, float sigma, uint64_t seed, uint64_t step, hipStream_t stream);
+// xq = x0 + N(0, sigma) for real atoms (q and fillers kept) over slots
+// [first, first+count); the step number is read from *step on the device so a
+// captured graph can be replayed; Philox4x32-10 counter-based generator keyed
+// by (seed), counter (slot, step).
+hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream);
+// *step += 1 (one thread), ends an evaluation in a captured graph.
+hipError_t launchAdvanceStep(unsigned long long* step, hipStream_t stream);
 
 // Fill kernels (graph-safe clears).
 hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream);

@@ -194,13 +194,30 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
         // added once per i-atom by the tj == 0 lanes.
         if (centralShift && s.cjPackedEnd > s.cjPackedBegin && a.cjPacked[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
         {
-            EAcc q2 = 0;
-#pragma unroll
-            for (int c = 0; c < c_ncps; c++)
+            if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
             {
-                q2 += static_cast<EAcc>(qi[c]) * static_cast<EAcc>(qi[c]);
+                float q2 = 0.0f;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    q2 += qi[c] * qi[c];
+                }
+                eEl -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
             }
-            eEl -= q2 / static_cast<EAcc>(epsfac) * static_cast<EAcc>(beta) * static_cast<EAcc>(c_oneSqrtPi);
+            else
+            {
+                // The self term is the largest energy contribution and cancels
+                // most of the pair sum: evaluate it with FP64 charges and constants.
+                const double* pD = a.sysParamsD + 8 * sys;
+                double q2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    const double q = a.xq[(s.sci * c_ncps + c) * c_cl + ti].w;
+                    q2 += q * q;
+                }
+                eEl -= pD[1] * q2 * pD[2] * c_oneSqrtPi;
+            }
         }
     }
 
@@ -778,17 +795,19 @@ __device__ __forceinline__ float uniformOpen(uint32_t u)
     return (static_cast<float>(u >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
-__global__ void perturbKernel(const float4* x0, float4* xq, const int* atomIsReal, int n, float sigma, uint2 key, uint32_t stepLo, uint32_t stepHi)
+__global__ void perturbKernel(const float4* x0, float4* xq, int first, int count, float sigma, uint2 key, const unsigned long long* step)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count)
     {
         return;
     }
+    const int i = first + k;
     float4 v = x0[i];
-    if (atomIsReal[i])
+    if (v.x > -1.0e5f && sigma > 0.0f)
     {
-        const uint4 r = philox4x32_10(make_uint4(static_cast<uint32_t>(i), 0u, stepLo, stepHi), key);
+        const unsigned long long st = *step;
+        const uint4 r = philox4x32_10(make_uint4(static_cast<uint32_t>(i), 0u, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32)), key);
         const float u1 = uniformOpen(r.x), u2 = uniformOpen(r.y), u3 = uniformOpen(r.z), u4 = uniformOpen(r.w);
         const float m1 = sqrtf(-2.0f * logf(u1));
         const float m2 = sqrtf(-2.0f * logf(u3));
@@ -799,6 +818,14 @@ __global__ void perturbKernel(const float4* x0, float4* xq, const int* atomIsRea
         v.z += sigma * m2 * cosf(a2);
     }
     xq[i] = v;
+}
+
+__global__ void advanceStepKernel(unsigned long long* step)
+{
+    if (threadIdx.x == 0)
+    {
+        *step += 1ull;
+    }
 }
 
 __global__ void fillU64Kernel(unsigned long long* p, size_t n)
@@ -906,6 +933,16 @@ hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* 
     return hipGetLastError();
 }
 
+hipError_t launchConvertForcesRange(unsigned long long* fFixed, int first, int count, float* fOut, double* fOutDouble, hipStream_t stream)
+{
+    if (count <= 0)
+    {
+        return hipSuccess;
+    }
+    const size_t off = 3 * static_cast<size_t>(first);
+    return launchConvertForces(fFixed + off, count, fOut ? fOut + off : nullptr, fOutDouble ? fOutDouble + off : nullptr, stream);
+}
+
 hipError_t launchReduceEnergies(const double* entryPartials,
                                 const int* entryRanges,
                                 const int* entryList,
@@ -925,14 +962,20 @@ hipError_t launchReduceEnergies(const double* entryPartials,
     return hipGetLastError();
 }
 
-hipError_t launchPerturb(const float4* x0, float4* xq, const int* atomIsReal, int numAtoms, float sigma, uint64_t seed, uint64_t step, hipStream_t stream)
+hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream)
 {
-    if (numAtoms == 0)
+    if (count <= 0)
     {
         return hipSuccess;
     }
     const uint2 key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
-    hipLaunchKernelGGL(perturbKernel, dim3((numAtoms + 255) / 256), dim3(256), 0, stream, x0, xq, atomIsReal, numAtoms, sigma, key, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32));
+    hipLaunchKernelGGL(perturbKernel, dim3((count + 255) / 256), dim3(256), 0, stream, x0, xq, first, count, sigma, key, step);
+    return hipGetLastError();
+}
+
+hipError_t launchAdvanceStep(unsigned long long* step, hipStream_t stream)
+{
+    hipLaunchKernelGGL(advanceStepKernel, dim3(1), dim3(64), 0, stream, step);
     return hipGetLastError();
 }
 

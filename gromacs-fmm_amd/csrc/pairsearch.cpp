@@ -680,6 +680,56 @@ void GridSearch::fillLjComb(const int* type, const float* perTypeComb, float* sl
     }
 }
 
+int64_t GridSearch::countPairsWithin(double rc) const
+{
+    const double rc2 = rc * rc;
+    int64_t count = 0;
+    for (const nbnxm_sci_t& s : sci_)
+    {
+        const float* sh = shiftVec_ + 3 * s.shift;
+        const bool central = (s.shift == NBNXM_CENTRAL_SHIFT);
+        for (int g = s.cjPackedBegin; g < s.cjPackedEnd; g++)
+        {
+            const nbnxm_cj_packed_t& p = cjPacked_[g];
+            const nbnxm_excl_t& ex = excl_[p.imei[0].excl_ind];
+            for (int jm = 0; jm < c_jg; jm++)
+            {
+                for (int im = 0; im < c_ncps; im++)
+                {
+                    const int bit = jm * c_ncps + im;
+                    if (!((p.imei[0].imask >> bit) & 1u))
+                    {
+                        continue;
+                    }
+                    const int ci = s.sci * c_ncps + im;
+                    const int cj = p.cj[jm];
+                    for (int ti = 0; ti < c_cl; ti++)
+                    {
+                        const int a = atomIndices_[ci * c_cl + ti];
+                        if (a < 0)
+                        {
+                            continue;
+                        }
+                        for (int tj = 0; tj < c_cl; tj++)
+                        {
+                            const int b = atomIndices_[cj * c_cl + tj];
+                            if (b < 0 || (central && ci == cj && tj <= ti) || !((ex.pair[tj * c_cl + ti] >> bit) & 1u))
+                            {
+                                continue;
+                            }
+                            const double dx = static_cast<double>(xw_[3 * a]) + sh[0] - xw_[3 * b];
+                            const double dy = static_cast<double>(xw_[3 * a + 1]) + sh[1] - xw_[3 * b + 1];
+                            const double dz = static_cast<double>(xw_[3 * a + 2]) + sh[2] - xw_[3 * b + 2];
+                            count += (dx * dx + dy * dy + dz * dz < rc2);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return count;
+}
+
 nbnxm_pairlist_gpu_t GridSearch::listView() const
 {
     nbnxm_pairlist_gpu_t v;

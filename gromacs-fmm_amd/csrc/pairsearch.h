@@ -70,6 +70,10 @@ public:
 
     nbnxm_pairlist_gpu_t listView() const;
 
+    // Non-excluded atom pairs within rc in the current list (FP64 distances on
+    // the wrapped search coordinates); requires the list to cover rc.
+    int64_t countPairsWithin(double rc) const;
+
 private:
     int placeAtoms();
     void computeBoundingBoxes();

@@ -117,7 +117,39 @@ SIGNATURES = [
     ("nbnxm_gpu_get_sci_order", _c_int, [_vp, _vp, _vp]),
     ("nbnxm_gpu_set_waves_per_block", _c_int, [_vp, _c_int]),
     ("nbnxm_gpu_free", None, [_vp]),
+    ("nbnxm_ensemble_create", _c_int, [_c_int, _vp, _vp, _PTR(_vp)]),
+    ("nbnxm_ensemble_set_split", _c_int, [_vp, _c_int, _vp, _vp, _c_int]),
+    ("nbnxm_ensemble_autotune", _c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    ("nbnxm_ensemble_run", _c_int, [_vp, _c_int, _c_int, ctypes.c_float, ctypes.c_uint64, _c_int, _c_int, _c_int]),
+    ("nbnxm_ensemble_sync", _c_int, [_vp]),
+    ("nbnxm_ensemble_set_step", _c_int, [_vp, ctypes.c_uint64]),
+    ("nbnxm_ensemble_time", _c_int, [_vp, _c_int, _c_int, ctypes.c_float, ctypes.c_uint64, _c_int, _c_int, _vp, _vp]),
+    ("nbnxm_ensemble_get_forces", _c_int, [_vp, _c_int, _vp]),
+    ("nbnxm_ensemble_get_energies", _c_int, [_vp, _vp, _vp]),
+    ("nbnxm_ensemble_get_positions", _c_int, [_vp, _c_int, _vp]),
+    ("nbnxm_ensemble_stats", _c_int, [_vp, _vp]),
+    ("nbnxm_ensemble_system_list", _c_int, [_vp, _c_int, _PTR(PairlistGpu), _vp]),
+    ("nbnxm_ensemble_count_pairs", _c_int, [_vp, _c_double, _vp, _vp]),
+    ("nbnxm_ensemble_free", None, [_vp]),
 ]
+
+
+class SystemDesc(ctypes.Structure):
+    _fields_ = [("numAtoms", ctypes.c_int), ("x", ctypes.c_void_p), ("q", ctypes.c_void_p),
+                ("type", ctypes.c_void_p), ("exclRanges", ctypes.c_void_p),
+                ("exclElements", ctypes.c_void_p), ("box", ctypes.c_double * 3)]
+
+
+class EnsembleOptions(ctypes.Structure):
+    _fields_ = [("numTypes", ctypes.c_int), ("c6c12", ctypes.c_void_p), ("vdwType", ctypes.c_int),
+                ("rcoulomb", ctypes.c_double), ("ewaldRtol", ctypes.c_double), ("epsilonR", ctypes.c_double),
+                ("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
+                ("accumulation", ctypes.c_int), ("maxPairsPerEntry", ctypes.c_int),
+                ("wavesPerBlock", ctypes.c_int), ("numHostThreads", ctypes.c_int)]
+
+
+MODE_BATCHED = 0
+MODE_PLAIN = 1
 
 _lib = None
 

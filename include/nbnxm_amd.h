@@ -241,6 +241,85 @@ int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock);
 /* gpu_free. */
 void nbnxm_gpu_free(nbnxm_gpu_t* nb);
 
+/* ---------------------------------------------------------------------------
+ * Ensemble: many independent systems of uneven size in one launch.
+ * No single reference counterpart: it replaces one nonbonded_verlet_t + one
+ * gpu_launch_kernel per system (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:478) with
+ * one list over all systems, one nonbonded launch per evaluation and a hipGraph
+ * over many evaluations.  Systems share the LJ type table.
+ * ------------------------------------------------------------------------- */
+typedef struct nbnxm_ensemble nbnxm_ensemble_t;
+
+typedef struct nbnxm_system_desc
+{
+    int numAtoms;
+    const double* x;          /* 3*numAtoms */
+    const double* q;          /* numAtoms, e */
+    const int* type;          /* numAtoms, index into the shared type table */
+    const int* exclRanges;    /* numAtoms+1, or NULL */
+    const int* exclElements;  /* exclusion partners (self included or not) */
+    double box[3];            /* rectangular, every edge > rlistOuter */
+} nbnxm_system_desc_t;
+
+typedef struct nbnxm_ensemble_options
+{
+    int numTypes;             /* real types; a dummy type is appended internally */
+    const double* c6c12;      /* 2*numTypes*numTypes: C6, C12 per type pair */
+    int vdwType;              /* enum nbnxm_vdw_type; combination rules use the diagonal */
+    double rcoulomb;          /* = rvdw */
+    double ewaldRtol;
+    double epsilonR;
+    double rlistOuter;        /* list cut-off */
+    double rlistInner;        /* dynamic-prune cut-off, rc <= rlistInner <= rlistOuter */
+    int accumulation;         /* enum nbnxm_accumulation */
+    int maxPairsPerEntry;     /* initial entry split (cluster pairs), 0 = 64 */
+    int wavesPerBlock;        /* 1..4, 0 = 4 */
+    int numHostThreads;       /* list construction threads, 0 = hardware concurrency */
+} nbnxm_ensemble_options_t;
+
+#define NBNXM_MODE_BATCHED 0  /* one launch per kernel for all systems */
+#define NBNXM_MODE_PLAIN 1    /* one launch per kernel per system, in turn */
+
+/* Builds every system's grid and list on the host, uploads all data and
+ * prunes the fresh lists (rlistOuter -> rlistInner) with the entry ordering. */
+int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, const nbnxm_ensemble_options_t* opt, nbnxm_ensemble_t** out);
+/* Re-splits the entries of the systems in each size bucket: bucketEdges
+ * (numBuckets-1 ascending atom counts) and maxPairs (numBuckets, 0 = no split). */
+int nbnxm_ensemble_set_split(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, const int* maxPairs, int wavesPerBlock);
+/* Times candidate splits per size bucket and keeps the fastest; writes the
+ * chosen maxPairs per bucket and waves per block when the pointers are set. */
+int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int* chosenMaxPairs, int* chosenWavesPerBlock);
+/* Runs numSteps evaluations: x = x_search + N(0, sigma) per atom and step
+ * (sigma 0: the search positions), nonbonded forces, FP64 energies and shift
+ * forces.  useGraph: capture stepsPerGraph evaluations in a hipGraph and replay
+ * it (batched mode only).  Asynchronous; nbnxm_ensemble_sync waits. */
+int nbnxm_ensemble_run(nbnxm_ensemble_t* e, int mode, int numSteps, float sigma, uint64_t seed, int computeEnergy, int useGraph, int stepsPerGraph);
+int nbnxm_ensemble_sync(nbnxm_ensemble_t* e);
+/* Sets the step counter that keys the perturbation of the next evaluation. */
+int nbnxm_ensemble_set_step(nbnxm_ensemble_t* e, uint64_t step);
+/* Times numSteps evaluations between HIP events on the ensemble's stream;
+ * writes total milliseconds, and the mean nonbonded-kernel duration in ms when
+ * kernelMs is set (batched mode, measured in a separate pass of events around
+ * each nonbonded launch). */
+int nbnxm_ensemble_time(nbnxm_ensemble_t* e, int mode, int numSteps, float sigma, uint64_t seed, int computeEnergy, int useGraph, double* totalMs, double* kernelMs);
+/* Forces of one system after the last evaluation, original atom order (3*numAtoms). */
+int nbnxm_ensemble_get_forces(nbnxm_ensemble_t* e, int system, double* f);
+/* Energies per system: 2*numSystems (E_lj, E_el); fshift: 3*45*numSystems (may be NULL). */
+int nbnxm_ensemble_get_energies(nbnxm_ensemble_t* e, double* energies, double* fshift);
+/* Coordinates used by the last evaluation, original order (3*numAtoms). */
+int nbnxm_ensemble_get_positions(nbnxm_ensemble_t* e, int system, double* x);
+/* Counts: [0] atoms, [1] grid atoms, [2] entries, [3] packed groups,
+ * [4] cluster pairs in the list, [5] cluster pairs after the inner prune. */
+int nbnxm_ensemble_stats(nbnxm_ensemble_t* e, int64_t* stats);
+/* Pair list of one system as uploaded (before pruning), for checks. */
+int nbnxm_ensemble_system_list(nbnxm_ensemble_t* e, int system, nbnxm_pairlist_gpu_t* list, int* atomIndices);
+/* Useful pairs at the search positions: non-excluded atom pairs (every
+ * periodic image) with r < rc, counted in FP64 over each system's list
+ * (the pair counts of src/gromacs/nbnxm/pairlist.cpp:4143-4172 restricted to
+ * the cut-off).  perSystem may be NULL. */
+int nbnxm_ensemble_count_pairs(nbnxm_ensemble_t* e, double rc, int64_t* perSystem, int64_t* total);
+void nbnxm_ensemble_free(nbnxm_ensemble_t* e);
+
 #ifdef __cplusplus
 }
 #endif

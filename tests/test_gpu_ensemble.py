@@ -1,0 +1,128 @@
+"""Batched ensemble on the GPU: per-system forces against the FP64 brute-force
+sum, batched == plain bitwise, graph replays == eager bitwise, perturbed
+positions stay covered by the pruned list."""
+import copy
+
+import numpy as np
+import pytest
+
+import nbnxm_amd as nb
+import oracle
+from conftest import announce
+from helpers import rel_rms, wrapped_float_system
+
+pytestmark = pytest.mark.gpu
+
+RC, RTOL = 1.0, 1e-5
+
+
+@pytest.fixture(scope="module")
+def systems():
+    # Smallest box (648 atoms, L = 1.86 nm < 2 rc: several images of a pair in range) up to 6000 atoms.
+    sizes = [216, 300, 420, 1000, 2000]
+    return [wrapped_float_system(nb.systems.water_box(m, None, seed=40 + k)) for k, m in enumerate(sizes)]
+
+
+@pytest.fixture(scope="module")
+def ens(systems):
+    announce("ensemble create fixed64")
+    return nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_COMB_GEOM, nb.ACCUM_FIXED64, max_pairs_per_entry=48)
+
+
+def _bruteforce(s):
+    ic = nb.interaction_const(RC, RTOL)
+    f, elj, eel, npairs = oracle.bruteforce(s, ic)
+    return f, elj, eel, npairs
+
+
+def test_ensemble_vs_bruteforce(systems, ens):
+    announce("ensemble vs brute force, sigma 0")
+    ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
+    e, fshift = ens.energies()
+    total, per = ens.count_pairs()
+    for k, s in enumerate(systems):
+        f_ref, elj, eel, npairs = _bruteforce(s)
+        assert per[k] == npairs
+        assert rel_rms(ens.forces(k), f_ref) <= 5e-6, k
+        assert abs(e[k, 0] - elj) <= 1e-5 * abs(elj)
+        # FP32 pair energies sum to a small difference of large terms (self
+        # energy vs pair sum): the relative error of the total is ~1e-5.
+        assert abs(e[k, 1] - eel) <= 5e-5 * abs(eel)
+
+
+def test_fp64_mode_vs_bruteforce(systems):
+    announce("ensemble fp64 mode vs brute force")
+    ens64 = nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_COMB_GEOM, nb.ACCUM_FP64)
+    ens64.run(1, nb.MODE_BATCHED, 0.0, 1, True)
+    e, _ = ens64.energies()
+    for k, s in enumerate(systems):
+        f_ref, elj, eel, _ = _bruteforce(s)
+        # Only the FP32 storage of coordinates' shift and LJ parameters differs.
+        assert rel_rms(ens64.forces(k), f_ref) <= 2e-7, k
+        assert abs(e[k, 1] - eel) <= 1e-7 * abs(eel)
+
+
+def test_batched_equals_plain_bitwise(systems, ens):
+    announce("batched vs plain bitwise")
+    ens.set_step(3)
+    ens.run(1, nb.MODE_BATCHED, 0.002, 9, True)
+    fb = [ens.forces(k) for k in range(len(systems))]
+    eb, fsb = ens.energies()
+    ens.set_step(3)
+    ens.run(1, nb.MODE_PLAIN, 0.002, 9, True)
+    for k in range(len(systems)):
+        assert np.array_equal(ens.forces(k), fb[k])
+    ep, fsp = ens.energies()
+    assert np.array_equal(ep, eb) and np.array_equal(fsp, fsb)
+
+
+def test_graph_replays_equal_eager(systems, ens):
+    announce("graph replay vs eager")
+    ens.set_step(7)
+    ens.run(1, nb.MODE_BATCHED, 0.002, 5, True, use_graph=False)
+    ref = [ens.forces(k) for k in range(len(systems))]
+    eref, _ = ens.energies()
+    for rep in range(3):
+        ens.set_step(7)
+        ens.run(1, nb.MODE_BATCHED, 0.002, 5, True, use_graph=True, steps_per_graph=1)
+        ens.sync()
+        for k in range(len(systems)):
+            assert np.array_equal(ens.forces(k), ref[k]), (rep, k)
+        e, _ = ens.energies()
+        assert np.array_equal(e, eref)
+        # eager work between replays
+        ens.run(1, nb.MODE_BATCHED, 0.0, 1, False)
+
+
+def test_perturbed_positions_covered_by_list(systems, ens):
+    announce("perturbed positions vs brute force")
+    ens.set_step(123)
+    ens.run(1, nb.MODE_BATCHED, 0.002, 77, True)
+    e, _ = ens.energies()
+    for k, s in enumerate(systems):
+        x = ens.positions(k)
+        moved = np.abs(x - s.x).max()
+        assert 0.0 < moved < 0.02
+        sp = copy.copy(s)
+        sp.x = x
+        f_ref, elj, eel, _ = _bruteforce(sp)
+        assert rel_rms(ens.forces(k), f_ref) <= 5e-6, k
+        assert abs(e[k, 1] - eel) <= 5e-5 * abs(eel)
+
+
+def test_split_and_autotune_keep_forces(systems, ens):
+    announce("autotune keeps forces")
+    ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
+    ref = [ens.forces(k) for k in range(len(systems))]
+    chosen, wpb = ens.autotune([1000, 3000])
+    assert len(chosen) == 3 and 1 <= wpb <= 4
+    ens.set_split([1000, 3000], [0, 32, 16], 2)
+    ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
+    for k in range(len(systems)):
+        # A different split regroups the FP32 i-force partial sums: equal to rounding.
+        assert rel_rms(ens.forces(k), ref[k]) <= 1e-6
+    # For a fixed split the result is bitwise reproducible.
+    again = [ens.forces(k) for k in range(len(systems))]
+    ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
+    for k in range(len(systems)):
+        assert np.array_equal(ens.forces(k), again[k])
