@@ -199,6 +199,65 @@ def sci_sort_order(sci_count):
     return out["sci"].copy()
 
 
+_cpu4xm = None
+CPU4XM_PATH = os.path.join(_HERE, "lib", "libcpu4xm.so")
+
+
+def _load_cpu4xm():
+    global _cpu4xm
+    if _cpu4xm is None:
+        lib = ctypes.CDLL(CPU4XM_PATH)
+        lib.cpu4xm_create.restype = _vp
+        lib.cpu4xm_create.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]
+        lib.cpu4xm_compute.restype = ctypes.c_int
+        lib.cpu4xm_compute.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
+        lib.cpu4xm_num_threads.restype = ctypes.c_int
+        lib.cpu4xm_num_threads.argtypes = [_vp]
+        lib.cpu4xm_num_cluster_pairs.restype = ctypes.c_longlong
+        lib.cpu4xm_num_cluster_pairs.argtypes = [_vp]
+        lib.cpu4xm_free.restype = None
+        lib.cpu4xm_free.argtypes = [_vp]
+        _cpu4xm = lib
+    return _cpu4xm
+
+
+def cpu4xm_available():
+    return os.path.exists(CPU4XM_PATH)
+
+
+class Cpu4xm:
+    """The 4xM SIMD CPU baseline kernel on one system's list (geometric LJ)."""
+
+    def __init__(self, plist, atoms, ic, r_prune, threads=0):
+        self._lib = _load_cpu4xm()
+        self._lv = _list(plist.sci, plist.cj_packed, plist.excl, plist.rlist)
+        self._av = _atoms(atoms)
+        self._ic = _ic(ic)
+        self._keep = (plist, atoms)
+        self._h = self._lib.cpu4xm_create(ctypes.byref(self._lv), ctypes.byref(self._av), ctypes.byref(self._ic), float(r_prune), int(threads))
+        if not self._h:
+            raise RuntimeError("cpu4xm_create failed (geometric LJ parameters required)")
+        self.n = atoms.num_atoms
+
+    @property
+    def threads(self):
+        return self._lib.cpu4xm_num_threads(self._h)
+
+    def compute(self, compute_energy=True):
+        f = np.zeros(3 * self.n, dtype=np.float32)
+        vlj = ctypes.c_double(0.0)
+        vc = ctypes.c_double(0.0)
+        rc = self._lib.cpu4xm_compute(self._h, int(compute_energy), _p(f), ctypes.byref(vlj), ctypes.byref(vc))
+        if rc != 0:
+            raise RuntimeError("cpu4xm_compute failed")
+        return f.reshape(-1, 3), vlj.value, vc.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.cpu4xm_free(self._h)
+            self._h = None
+
+
 def count_pairs(plist, atoms, rc):
     lv = _list(plist.sci, plist.cj_packed, plist.excl, plist.rlist)
     av = _atoms(atoms)

@@ -24,6 +24,7 @@ def wrapped_float_system(system):
     xw = xw.astype(np.float32)
     xw[xw >= box.astype(np.float32)] = 0.0
     s.x = xw.astype(np.float64)
+    s.q = np.asarray(system.q, dtype=np.float32).astype(np.float64)
     # LJ parameters as the FP32 type table holds them (6*C6, 12*C12 in FP32).
     c = np.array(system.c6c12, dtype=np.float64)
     c[..., 0] = (6.0 * c[..., 0]).astype(np.float32).astype(np.float64) / 6.0

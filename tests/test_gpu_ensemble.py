@@ -26,7 +26,9 @@ def systems():
 @pytest.fixture(scope="module")
 def ens(systems):
     announce("ensemble create fixed64")
-    return nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_COMB_GEOM, nb.ACCUM_FIXED64, max_pairs_per_entry=48)
+    # Type-table LJ: the FP32 table holds exactly the (rounded) parameters the
+    # brute-force sum uses, so only FP32 arithmetic separates the two.
+    return nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_TYPE_TABLE, nb.ACCUM_FIXED64, max_pairs_per_entry=48)
 
 
 def _bruteforce(s):
@@ -44,22 +46,23 @@ def test_ensemble_vs_bruteforce(systems, ens):
         f_ref, elj, eel, npairs = _bruteforce(s)
         assert per[k] == npairs
         assert rel_rms(ens.forces(k), f_ref) <= 5e-6, k
-        assert abs(e[k, 0] - elj) <= 1e-5 * abs(elj)
-        # FP32 pair energies sum to a small difference of large terms (self
-        # energy vs pair sum): the relative error of the total is ~1e-5.
+        # FP32 pair energies sum to a small difference of large terms
+        # (repulsion vs dispersion, self energy vs pair sum): ~1e-5 relative.
+        assert abs(e[k, 0] - elj) <= 5e-5 * abs(elj)
         assert abs(e[k, 1] - eel) <= 5e-5 * abs(eel)
 
 
 def test_fp64_mode_vs_bruteforce(systems):
     announce("ensemble fp64 mode vs brute force")
-    ens64 = nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_COMB_GEOM, nb.ACCUM_FP64)
+    ens64 = nb.Ensemble(systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_TYPE_TABLE, nb.ACCUM_FP64)
     ens64.run(1, nb.MODE_BATCHED, 0.0, 1, True)
     e, _ = ens64.energies()
     for k, s in enumerate(systems):
         f_ref, elj, eel, _ = _bruteforce(s)
-        # Only the FP32 storage of coordinates' shift and LJ parameters differs.
-        assert rel_rms(ens64.forces(k), f_ref) <= 2e-7, k
-        assert abs(e[k, 1] - eel) <= 1e-7 * abs(eel)
+        # Same inputs, FP64 arithmetic on both sides; 2^-32 fixed-point force resolution.
+        assert rel_rms(ens64.forces(k), f_ref) <= 1e-9, k
+        assert abs(e[k, 0] - elj) <= 1e-9 * abs(elj)
+        assert abs(e[k, 1] - eel) <= 1e-9 * abs(eel)
 
 
 def test_batched_equals_plain_bitwise(systems, ens):

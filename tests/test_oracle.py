@@ -132,9 +132,8 @@ def test_f32_restatement_close_to_f64():
 
 
 def test_newton_shift_forces():
-    """Sum of forces equals minus the shift-force sum? No: with pair forces the
-    total force is zero, and the virial-relevant shift forces balance the
-    image contributions: sum_i f_i = 0 and sum_s fshift_s = sum over image pairs."""
+    """Pair forces cancel (sum_i f_i = 0) and the central shift collects no
+    shift force (nbnxm_cuda_kernel.cuh:694-698)."""
     s = nb.systems.water_box(300, None, seed=2)
     ic = nb.interaction_const(1.0, 1e-5)
     search, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_COMB_GEOM)
@@ -204,3 +203,19 @@ def test_sci_sort_stable_descending():
     counts = np.array([8190, 8000, 8190, 0, 8000, 5], dtype=np.int32)
     order = oracle.sci_sort_order(counts)
     assert order.tolist() == [3, 5, 1, 4, 0, 2]
+
+
+def test_cpu4xm_baseline_matches_fp64():
+    """The CPU 4xM SIMD baseline computes the same forces and energies."""
+    s = nb.systems.water_box(900, None, seed=31)
+    ic = nb.interaction_const(1.0, 1e-5)
+    search, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_COMB_GEOM)
+    f64, _, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    k = oracle.Cpu4xm(plist, atoms, ic, 1.04, threads=2)
+    f, vlj, vc = k.compute(True)
+    real = atoms.atom_indices >= 0
+    assert rel_rms(f[real], f64[real]) < 1e-5
+    assert vlj == pytest.approx(elj64, rel=1e-5)
+    assert vc == pytest.approx(eel64, rel=1e-4)
+    f2, _, _ = k.compute(False)
+    assert rel_rms(f2[real], f[real]) < 1e-6
