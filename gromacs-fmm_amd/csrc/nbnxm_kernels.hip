@@ -121,25 +121,123 @@ __device__ __forceinline__ float norm2Fixed(float dx, float dy, float dz)
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
+// Read-only list and parameter data through the constant address space: with
+// wave-uniform indices the loads become scalar (s_load, counted by lgkmcnt),
+// so waiting for them never waits on the vector-memory counter that also
+// tracks the in-flight force atomics.
+template<typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+template<typename T>
+__device__ __forceinline__ ConstPtr<T> asConst(const T* p)
+{
+    return (ConstPtr<T>)(p);
+}
+
+__device__ __forceinline__ nbnxm_sci_t loadSci(ConstPtr<nbnxm_sci_t> p, int i)
+{
+    const ConstPtr<int> q = (ConstPtr<int>)(p + i);
+    nbnxm_sci_t s;
+    s.sci = q[0];
+    s.shift = q[1];
+    s.cjPackedBegin = q[2];
+    s.cjPackedEnd = q[3];
+    return s;
+}
+
+__device__ __forceinline__ float4 loadF4(ConstPtr<float4> p, int i)
+{
+    const ConstPtr<float> q = (ConstPtr<float>)(p + i);
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ float2 loadF2(ConstPtr<float2> p, int i)
+{
+    const ConstPtr<float> q = (ConstPtr<float>)(p + i);
+    return make_float2(q[0], q[1]);
+}
+
+// erf(z)/z as a rational in z^2: the single-precision pmePotentialCorrection
+// (src/gromacs/simd/include/gromacs/simd/simd_math.h:1687-1720), the form the
+// reference CPU kernels use for the analytical Ewald energy; abs. error ~1e-6.
+__device__ __forceinline__ float erfOverZ(float z2)
+{
+    const float VN6 = 1.9296833005951166339e-8F;
+    const float VN5 = -1.4213390571557850962e-6F;
+    const float VN4 = 0.000041603292906656984871F;
+    const float VN3 = -0.00013134036773265025626F;
+    const float VN2 = 0.038657983986041781264F;
+    const float VN1 = 0.11285044772717598220F;
+    const float VN0 = 1.1283802385263030286F;
+    const float VD3 = 0.0066752224023576045451F;
+    const float VD2 = 0.078647795836373922256F;
+    const float VD1 = 0.43336185284710920150F;
+    const float VD0 = 1.0F;
+
+    const float z4 = z2 * z2;
+    float polyVD1 = __builtin_fmaf(VD3, z4, VD1);
+    float polyVD0 = __builtin_fmaf(VD2, z4, VD0);
+    polyVD0 = __builtin_fmaf(polyVD1, z2, polyVD0);
+    polyVD0 = __builtin_amdgcn_rcpf(polyVD0);
+    float polyVN0 = __builtin_fmaf(VN6, z4, VN4);
+    float polyVN1 = __builtin_fmaf(VN5, z4, VN3);
+    polyVN0 = __builtin_fmaf(polyVN0, z4, VN2);
+    polyVN1 = __builtin_fmaf(polyVN1, z4, VN1);
+    polyVN0 = __builtin_fmaf(polyVN0, z4, VN0);
+    polyVN0 = __builtin_fmaf(polyVN1, z2, polyVN0);
+    return polyVN0 * polyVD0;
+}
+
+// erfc(z) with relative accuracy for z in [0, 4.5]: erfc(z) = exp(-z^2) R(t),
+// t = 1/(1 + 0.7 z), R a degree-9 polynomial fitted to erfc(z) exp(z^2) by
+// relative least squares in FP64 (max relative error 2.7e-7 evaluated in FP32).
+// Used for pairs without exclusions, where 1 - erf(z) would lose the small
+// erfc(z) near the cut-off.
+__device__ __forceinline__ float erfcxPoly(float z)
+{
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.7f, z, 1.0f));
+    float p = 0.06597486883401871f;
+    p = __builtin_fmaf(p, t, -0.3136824071407318f);
+    p = __builtin_fmaf(p, t, 0.5114253163337708f);
+    p = __builtin_fmaf(p, t, -0.12743264436721802f);
+    p = __builtin_fmaf(p, t, -0.5417962670326233f);
+    p = __builtin_fmaf(p, t, 0.4081534445285797f);
+    p = __builtin_fmaf(p, t, 0.18638932704925537f);
+    p = __builtin_fmaf(p, t, 0.4186546206474304f);
+    p = __builtin_fmaf(p, t, 0.39217787981033325f);
+    p = __builtin_fmaf(p, t, 0.00013602453691419214f);
+    return p;
+}
+
+__device__ __forceinline__ float erfcRelative(float z)
+{
+    return __expf(-z * z) * erfcxPoly(z);
+}
+
+// Force-only kernels fit 4 waves per SIMD (128 VGPRs); the energy kernels need
+// more registers and run at the occupancy their allocation allows.
 template<bool computeEnergy, int vdwType, int accumulation>
-__global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a, int wavesPerBlock)
+__global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKernel(const NbKernelArgs a, int wavesPerBlock)
 {
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7;
     const int tj = lane >> 3;
-    const int work = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    // Wave-uniform by construction; readfirstlane lets the compiler keep it scalar.
+    const int work = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + (threadIdx.x >> 6));
     if (work >= a.nsci)
     {
         return;
     }
-    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
-    const nbnxm_sci_t s = a.sci[entry];
+    const ConstPtr<nbnxm_cj_packed_t> cjList = asConst(a.cjPacked);
+    const ConstPtr<float4> xqc = asConst(a.xq);
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? asConst(a.sciOrder)[work] : work);
+    const nbnxm_sci_t s = loadSci(asConst(a.sci), entry);
     const int sys = s.shift / NBNXM_NUM_SHIFTS;
     const int shiftInSys = s.shift - sys * NBNXM_NUM_SHIFTS;
     const bool centralShift = (shiftInSys == NBNXM_CENTRAL_SHIFT);
 
-    const float4 pA = a.sysParamsA[sys];
-    const float2 pB = a.sysParamsB[sys];
+    const float4 pA = loadF4(asConst(a.sysParamsA), sys);
+    const float2 pB = loadF2(asConst(a.sysParamsB), sys);
     const float rcoulomb_sq = pA.x;
     const float epsfac = pA.y;
     const float beta = pA.z;
@@ -148,31 +246,32 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
     const float ewaldShift = pA.w;
     const float cpotRep = pB.x;
     const float cpotDisp = pB.y;
+    const float4 sh = loadF4(asConst(a.shiftVec), s.shift);
 
-    const float4 sh = a.shiftVec[s.shift];
-
-    // i-atom data for all 8 clusters of the super-cluster, in registers.
+    // i-atom coordinates and charges of all 8 clusters in registers; their LJ
+    // parameters (type row offset or combination parameters) in per-wave LDS.
+    __shared__ float2 sLi[4][c_ncps * c_cl];
+    float2* stageLi = sLi[threadIdx.x >> 6];
     float xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
-    int typeiOff[c_ncps];
-    float ljci0[c_ncps], ljci1[c_ncps];
 #pragma unroll
     for (int c = 0; c < c_ncps; c++)
     {
         const int ai = (s.sci * c_ncps + c) * c_cl + ti;
-        const float4 v = a.xq[ai];
+        const float4 v = loadF4(xqc, ai);
         xi[c] = v.x + sh.x;
         yi[c] = v.y + sh.y;
         zi[c] = v.z + sh.z;
         qi[c] = v.w * epsfac;
+    }
+    {
+        const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
         if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
         {
-            typeiOff[c] = a.numTypes * a.atomTypes[ai];
+            stageLi[lane] = make_float2(__int_as_float(a.numTypes * asConst(a.atomTypes)[ai]), 0.0f);
         }
         else
         {
-            const float2 lc = a.ljComb[ai];
-            ljci0[c] = lc.x;
-            ljci1[c] = lc.y;
+            stageLi[lane] = loadF2(asConst(a.ljComb), ai);
         }
     }
 
@@ -192,7 +291,7 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
     {
         // Self energy on the diagonal entry (:381-418): -epsfac*beta/sqrt(pi)*sum q^2,
         // added once per i-atom by the tj == 0 lanes.
-        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && a.cjPacked[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
+        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && cjList[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
         {
             if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
             {
@@ -213,7 +312,7 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
 #pragma unroll
                 for (int c = 0; c < c_ncps; c++)
                 {
-                    const double q = a.xq[(s.sci * c_ncps + c) * c_cl + ti].w;
+                    const double q = loadF4(xqc, (s.sci * c_ncps + c) * c_cl + ti).w;
                     q2 += q * q;
                 }
                 eEl -= pD[1] * q2 * pD[2] * c_oneSqrtPi;
@@ -223,15 +322,30 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
 
     // Newton / self masking on the diagonal of the central shift (:423).
     const bool nonSelfLane = !(centralShift && tj <= ti);
+    const int firstCluster = s.sci * c_ncps;
 
-    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
-    {
-        const unsigned int imask = a.cjPacked[jp].imei[0].imask;
-        if (imask == 0)
-        {
-            continue;
-        }
-        const unsigned int wexcl = a.excl[a.cjPacked[jp].imei[0].excl_ind].pair[lane];
+    // Per-wave LDS staging of one packed group: 32 j-atoms' xq and LJ data.
+    // The next group is fetched into registers (one coalesced load per lane)
+    // before the current group's force atomics are issued, so waiting for
+    // j-data never waits for those atomics (vmcnt counts both, in order).
+    __shared__ float4 sXq[4][c_jg * c_cl];
+    __shared__ float2 sLj[4][c_jg * c_cl];
+    float4* stageXq = sXq[threadIdx.x >> 6];
+    float2* stageLj = sLj[threadIdx.x >> 6];
+    const int stageAtom = lane & 31; // j-cluster stageAtom/8, atom stageAtom%8
+
+    // One packed group of 4 j-clusters against the 8 i-clusters.  Instantiated
+    // with and without exclusion handling: groups without exclusions (the
+    // large majority) skip the interaction-bit arithmetic and take the
+    // Coulomb energy from erfc directly.
+    // Deferred j-force atomics: the reduced force component of lane ti < 3
+    // for each j-cluster of the group, issued at the start of the next group.
+    float pendV[c_jg];
+    int pendCj[c_jg];
+    unsigned int pendMask = 0u;
+
+    auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
+        constexpr bool hasExcl = decltype(withExcl)::value;
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)
         {
@@ -239,21 +353,14 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
             {
                 continue;
             }
-            const int cj = a.cjPacked[jp].cj[jm];
-            const int aj = cj * c_cl + tj;
-            const float4 xqj = a.xq[aj];
-            int typej = 0;
-            float2 ljcj = make_float2(0.0f, 0.0f);
-            if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
-            {
-                typej = a.atomTypes[aj];
-            }
-            else
-            {
-                ljcj = a.ljComb[aj];
-            }
+            const int cj = (jm == 0) ? cj0 : (jm == 1) ? cj1 : (jm == 2) ? cj2 : cj3;
+            const float4 xqj = stageXq[jm * c_cl + tj];
+            const float2 ljStaged = stageLj[jm * c_cl + tj];
+            const int typej = __float_as_int(ljStaged.x);
+            const float2 ljcj = ljStaged;
             float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
-
+            float eljC = 0.0f, eelC = 0.0f;
+            bool anyInRange = false;
 #pragma unroll
             for (int c = 0; c < c_ncps; c++)
             {
@@ -266,44 +373,52 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
                 const float dy = yi[c] - xqj.y;
                 const float dz = zi[c] - xqj.z;
                 float r2 = norm2Fixed(dx, dy, dz);
-                const bool notDiag = (s.sci * c_ncps + c) != cj;
+                const bool notDiag = (firstCluster + c) != cj;
                 if (r2 < rcoulomb_sq && (nonSelfLane || notDiag))
                 {
-                    const float int_bit = ((wexcl >> bit) & 1u) ? 1.0f : 0.0f;
+                    anyInRange = true;
+                    const float int_bit = hasExcl ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
                     float c6, c12;
                     float sigma = 0.0f, epsilon = 0.0f;
+                    const float2 ljci = stageLi[c * c_cl + ti];
                     if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
                     {
-                        const float2 p = a.nbfp[typeiOff[c] + typej];
+                        const float2 p = loadF2(asConst(a.nbfp), __float_as_int(ljci.x) + typej);
                         c6 = p.x;
                         c12 = p.y;
                     }
                     else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
                     {
-                        c6 = ljci0[c] * ljcj.x;
-                        c12 = ljci1[c] * ljcj.y;
+                        c6 = ljci.x * ljcj.x;
+                        c12 = ljci.y * ljcj.y;
                     }
                     else
                     {
-                        sigma = ljci0[c] + ljcj.x;
-                        epsilon = ljci1[c] * ljcj.y;
+                        sigma = ljci.x + ljcj.x;
+                        epsilon = ljci.y * ljcj.y;
                         const float sigma2 = sigma * sigma;
                         const float sigma6 = sigma2 * sigma2 * sigma2;
                         c6 = epsilon * sigma6;
                         c12 = c6 * sigma6;
                     }
-                    r2 = fmaxf(r2, NBNXM_MIN_DISTANCE_SQUARED);
+                    r2 = __builtin_amdgcn_fmed3f(r2, NBNXM_MIN_DISTANCE_SQUARED, rcoulomb_sq);
                     const float inv_r = __builtin_amdgcn_rsqf(r2);
                     const float inv_r2 = inv_r * inv_r;
                     float F_invr;
                     float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                    inv_r6 *= int_bit;
+                    if constexpr (hasExcl)
+                    {
+                        inv_r6 *= int_bit;
+                    }
                     if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
                     {
                         const float sig_r = sigma * inv_r;
                         const float sig_r2 = sig_r * sig_r;
                         float sig_r6 = sig_r2 * sig_r2 * sig_r2;
-                        sig_r6 *= int_bit;
+                        if constexpr (hasExcl)
+                        {
+                            sig_r6 *= int_bit;
+                        }
                         F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
                     }
                     else
@@ -311,15 +426,31 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
                         F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
                     }
                     const float qq = qi[c] * xqj.w;
-                    F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
-                    if constexpr (computeEnergy)
+                    const float z2 = beta2 * r2;
+                    if constexpr (computeEnergy && !hasExcl)
                     {
-                        const float elj = int_bit
-                                          * (c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f)
-                                             - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f));
-                        const float eel = qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
-                        eLJ += static_cast<EAcc>(elj);
-                        eEl += static_cast<EAcc>(eel);
+                        // Force and energy from one exp(-z^2): with erfc(z) = e R,
+                        // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2 and
+                        // V/qq = e R / r - sh_ewald; no cancellation in either.
+                        const float e = __expf(-z2);
+                        const float rPoly = erfcxPoly(r2 * inv_r * beta);
+                        const float eR = e * rPoly;
+                        F_invr += qq * e * __builtin_fmaf(rPoly, inv_r, beta * static_cast<float>(2.0 * c_oneSqrtPi)) * inv_r2;
+                        eljC += c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f) - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f);
+                        eelC += qq * __builtin_fmaf(eR, inv_r, -ewaldShift);
+                    }
+                    else
+                    {
+                        const float coulR3 = hasExcl ? int_bit * inv_r2 * inv_r : inv_r2 * inv_r;
+                        F_invr += qq * (coulR3 + pmecorrF(z2) * beta3);
+                        if constexpr (computeEnergy)
+                        {
+                            // Groups with exclusions: the reference form
+                            // qq*((int_bit - erf(beta r))/r - int_bit*sh_ewald) (:656-659).
+                            const float ljPot = c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f) - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f);
+                            eljC += int_bit * ljPot;
+                            eelC += qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
+                        }
                     }
                     const float tx = dx * F_invr;
                     const float ty = dy * F_invr;
@@ -332,25 +463,131 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernel(const NbKernelArgs a
                     fciZ[c] += tz;
                 }
             }
-
-            // j-force: sum over the 8 i-lanes, lanes ti = 0,1,2 store x,y,z.
-            fcjX = reduceOver8Lanes(fcjX);
-            fcjY = reduceOver8Lanes(fcjY);
-            fcjZ = reduceOver8Lanes(fcjZ);
-            if (ti < 3)
+            if constexpr (computeEnergy)
             {
-                const float v = (ti == 0) ? fcjX : ((ti == 1) ? fcjY : fcjZ);
-                if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                eLJ += static_cast<EAcc>(eljC);
+                eEl += static_cast<EAcc>(eelC);
+            }
+            // j-force: sum over the 8 i-lanes; lanes ti = 0,1,2 keep x,y,z for
+            // the deferred atomic.  Skipped when no pair was inside the cut-off.
+            if (__ballot(anyInRange) != 0)
+            {
+                fcjX = reduceOver8Lanes(fcjX);
+                fcjY = reduceOver8Lanes(fcjY);
+                fcjZ = reduceOver8Lanes(fcjZ);
+                pendV[jm] = (ti == 0) ? fcjX : ((ti == 1) ? fcjY : fcjZ);
+                pendCj[jm] = cj;
+                pendMask |= 1u << jm;
+            }
+        }
+    };
+
+    auto flushPending = [&]() {
+#pragma unroll
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if ((pendMask >> jm) & 1u)
+            {
+                if (ti < 3)
                 {
-                    atomicAdd(a.fFloat + 3 * aj + ti, v);
-                }
-                else
-                {
-                    atomicAdd(a.fFixed + 3 * aj + ti, toFixed(v));
+                    const int aj = pendCj[jm] * c_cl + tj;
+                    if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                    {
+                        atomicAdd(a.fFloat + 3 * aj + ti, pendV[jm]);
+                    }
+                    else
+                    {
+                        atomicAdd(a.fFixed + 3 * aj + ti, toFixed(pendV[jm]));
+                    }
                 }
             }
         }
+        pendMask = 0u;
+    };
+
+
+    auto nextNonEmpty = [&](int g) {
+        while (g < s.cjPackedEnd && cjList[g].imei[0].imask == 0u)
+        {
+            g++;
+        }
+        return g;
+    };
+    // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
+    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    unsigned int pfExcl = 0xFFFFFFFFu;
+    auto prefetch = [&](int g) {
+        const int k = stageAtom >> 3;
+        const int cjl = cjList[g].cj[0] * (k == 0) + cjList[g].cj[1] * (k == 1) + cjList[g].cj[2] * (k == 2) + cjList[g].cj[3] * (k == 3);
+        const int atom = cjl * c_cl + (stageAtom & 7);
+        if (lane < 32)
+        {
+            pf = loadF4(xqc, atom);
+        }
+        else if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+        {
+            pf = make_float4(__int_as_float(asConst(a.atomTypes)[atom]), 0.0f, 0.0f, 0.0f);
+        }
+        else
+        {
+            const float2 lj = loadF2(asConst(a.ljComb), atom);
+            pf = make_float4(lj.x, lj.y, 0.0f, 0.0f);
+        }
+        const int ex = cjList[g].imei[0].excl_ind;
+        pfExcl = (ex != 0) ? asConst(a.excl)[ex].pair[lane] : 0xFFFFFFFFu;
+    };
+    unsigned int stagedExcl = 0xFFFFFFFFu;
+    auto stage = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 32)
+        {
+            stageXq[stageAtom] = pf;
+        }
+        else
+        {
+            stageLj[stageAtom] = make_float2(pf.x, pf.y);
+        }
+        stagedExcl = pfExcl;
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    int jp = nextNonEmpty(s.cjPackedBegin);
+    if (jp < s.cjPackedEnd)
+    {
+        prefetch(jp);
+        stage();
     }
+    while (jp < s.cjPackedEnd)
+    {
+        const unsigned int imask = cjList[jp].imei[0].imask;
+        const int exclInd = cjList[jp].imei[0].excl_ind;
+        const int cj0 = cjList[jp].cj[0];
+        const int cj1 = cjList[jp].cj[1];
+        const int cj2 = cjList[jp].cj[2];
+        const int cj3 = cjList[jp].cj[3];
+        const unsigned int wexcl = stagedExcl;
+        // Atomics of the previous group, then the next group's loads.
+        flushPending();
+        const int jn = nextNonEmpty(jp + 1);
+        if (jn < s.cjPackedEnd)
+        {
+            prefetch(jn);
+        }
+        if (exclInd != 0)
+        {
+            group(cj0, cj1, cj2, cj3, imask, wexcl, std::true_type{});
+        }
+        else
+        {
+            group(cj0, cj1, cj2, cj3, imask, 0xFFFFFFFFu, std::false_type{});
+        }
+        if (jn < s.cjPackedEnd)
+        {
+            stage();
+        }
+        jp = jn;
+    }
+    flushPending();
 
     // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
     double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
