@@ -170,9 +170,14 @@ struct nbnxm_gpu
     DeviceBuffer<int> entryList;
     DeviceBuffer<int> shiftRanges;
     DeviceBuffer<int> shiftEntryList;
+    DeviceBuffer<int> slotRange;          // {0, numAtoms}
+    DeviceBuffer<double> virialPartials;  // 9 per block
+    DeviceBuffer<double> virial;          // 9
+    int virialBlocks = 1;
 
     double* hEnergies = nullptr; // pinned, 2
     double* hFshift = nullptr;   // pinned, 3*45
+    double* hVirial = nullptr;   // pinned, 9
     bool lastComputeEnergy = false;
     bool lastComputeVirial = false;
 
@@ -185,6 +190,10 @@ struct nbnxm_gpu
         if (hFshift)
         {
             (void)hipHostFree(hFshift);
+        }
+        if (hVirial)
+        {
+            (void)hipHostFree(hVirial);
         }
         if (stream)
         {
@@ -437,6 +446,7 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
     NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&nb->stream, hipStreamNonBlocking));
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hEnergies), 2 * sizeof(double), hipHostMallocDefault));
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hFshift), 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipHostMallocDefault));
+    NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hVirial), 9 * sizeof(double), hipHostMallocDefault));
     nb->ic = *ic;
     nb->listParams = *listParams;
     nb->accumulation = accumulation;
@@ -509,6 +519,13 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
     }
     NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
+    // Virial reduction geometry: about 1024 slots per block, at most 512 blocks.
+    nb->virialBlocks = std::min(512, std::max(1, (n + 1023) / 1024));
+    const int range[2] = { 0, n };
+    NBNXM_HIP_CHECK(nb->slotRange.resize(2));
+    NBNXM_HIP_CHECK(nb->virialPartials.resize(9 * static_cast<size_t>(nb->virialBlocks)));
+    NBNXM_HIP_CHECK(nb->virial.resize(9));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->slotRange.ptr, range, sizeof(range), hipMemcpyHostToDevice, nb->stream));
     // Host buffers of the caller may be reused after return.
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     return NBNXM_OK;
@@ -761,6 +778,12 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         return fail(NBNXM_ERR_INVALID, "gpu_launch_cpyback: null handle");
     }
     const size_t n3 = 3 * static_cast<size_t>(nb->numAtoms);
+    if (nb->lastComputeVirial)
+    {
+        // x (x) f partials before the fixed-point accumulator is cleared.
+        const bool fixed = nb->accumulation != NBNXM_ACCUM_FLOAT;
+        NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, fixed ? nb->fFixed.ptr : nullptr, fixed ? nullptr : nb->fFloat.ptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
+    }
     if (nb->accumulation == NBNXM_ACCUM_FLOAT)
     {
         if (f && n3)
@@ -789,6 +812,29 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         NBNXM_HIP_CHECK(launchReduceEnergies(nb->partials.ptr, nb->entryRanges.ptr, nb->entryList.ptr, nb->shiftRanges.ptr, nb->shiftEntryList.ptr, 1, nb->energies.ptr, nb->fshift.ptr, nb->lastComputeVirial ? 1 : 0, nb->stream));
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hEnergies, nb->energies.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hFshift, nb->fshift.ptr, 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
+    }
+    if (nb->lastComputeVirial)
+    {
+        NBNXM_HIP_CHECK(launchVirialFinish(nb->virialPartials.ptr, 1, nb->virialBlocks, nb->shiftVec.ptr, nb->fshift.ptr, nb->virial.ptr, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hVirial, nb->virial.ptr, 9 * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_get_virial(nbnxm_gpu_t* nb, double* virial)
+{
+    if (!nb || !virial)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_virial: null pointer");
+    }
+    if (!nb->lastComputeVirial)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_get_virial: the last launch did not compute the virial");
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    for (int k = 0; k < 9; k++)
+    {
+        virial[k] += nb->hVirial[k];
     }
     return NBNXM_OK;
 }

@@ -93,6 +93,21 @@ hipError_t launchReduceEnergies(const double* entryPartials,
                                 int computeFshift,
                                 hipStream_t stream);
 
+// Single-sum virial in FP64.  Partials: x (x) f summed over the slots
+// [slotRanges[s], slotRanges[s+1]) of each system s by blocksPerSystem blocks
+// (fixed-point forces if fFixed, else fFloat; run before the fixed-point
+// accumulator is cleared).  Finish: virial[9*s] = -1/2 (sum x (x) f +
+// sum_k shiftVec[45*s+k] (x) fshift[45*s+k]).
+hipError_t launchVirialPartials(const float4* xq,
+                                const unsigned long long* fFixed,
+                                const float* fFloat,
+                                const int* slotRanges,
+                                int numSystems,
+                                int blocksPerSystem,
+                                double* partials,
+                                hipStream_t stream);
+hipError_t launchVirialFinish(const double* partials, int numSystems, int blocksPerSystem, const float4* shiftVec, const double* fshift, double* virial, hipStream_t stream);
+
 // xq = x0 + N(0, sigma) for real atoms (q and fillers kept) over slots
 // [first, first+count); the step number is read from *step on the device so a
 // captured graph can be replayed; Philox4x32-10 counter-based generator keyed

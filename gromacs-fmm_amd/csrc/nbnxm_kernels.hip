@@ -1007,6 +1007,105 @@ __global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partial
     }
 }
 
+// Single-sum virial, atom part: per block FP64 partial sums of x (x) f over the
+// slots of one system (calc_x_times_f, src/gromacs/mdlib/calcvir.cpp:57-85).
+// Block (b, sys) covers slots slotRanges[sys] + b*256 + t + k*blocksPerSystem*256;
+// the wave and block reductions run in a fixed order, so the result is
+// bitwise reproducible for fixed-point forces.  Filler slots are skipped.
+__global__ __launch_bounds__(256) void virialPartialKernel(const float4* xq,
+                                                           const unsigned long long* fFixed,
+                                                           const float* fFloat,
+                                                           const int* slotRanges,
+                                                           int blocksPerSystem,
+                                                           double* partials)
+{
+    const int sys = blockIdx.y;
+    const int b = blockIdx.x;
+    const int begin = slotRanges[sys];
+    const int end = slotRanges[sys + 1];
+    double acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++)
+    {
+        acc[c] = 0.0;
+    }
+    for (int s = begin + b * 256 + static_cast<int>(threadIdx.x); s < end; s += blocksPerSystem * 256)
+    {
+        const float4 p = xq[s];
+        if (p.x < -1.0e5f)
+        {
+            continue;
+        }
+        double f[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+        {
+            f[d] = fFixed ? static_cast<double>(static_cast<long long>(fFixed[3 * static_cast<size_t>(s) + d])) * (1.0 / c_fixedScaleD)
+                          : static_cast<double>(fFloat[3 * static_cast<size_t>(s) + d]);
+        }
+        const double x[3] = { p.x, p.y, p.z };
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+        {
+#pragma unroll
+            for (int n = 0; n < 3; n++)
+            {
+                acc[3 * d + n] = fma(x[d], f[n], acc[3 * d + n]);
+            }
+        }
+    }
+    __shared__ double red[4][9];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < 9; c++)
+    {
+        const double v = reduceWaveD(acc[c]);
+        if (lane == 0)
+        {
+            red[wave][c] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 9)
+    {
+        const int c = threadIdx.x;
+        partials[(static_cast<size_t>(sys) * blocksPerSystem + b) * 9 + c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    }
+}
+
+// Virial per system: -1/2 (sum_i x_i (x) f_i + sum_s shift_s (x) fshift_s)
+// (calc_vir on shift forces and on forces, src/gromacs/mdlib/calcvir.cpp:89-133,
+// as api/nblib/virials.cpp:55-84 combines them).  One block per system.
+__global__ __launch_bounds__(64) void virialFinishKernel(const double* partials,
+                                                         int blocksPerSystem,
+                                                         const float4* shiftVec,
+                                                         const double* fshift,
+                                                         double* virial)
+{
+    const int sys = blockIdx.x;
+    const int c = threadIdx.x;
+    if (c >= 9)
+    {
+        return;
+    }
+    const int d = c / 3;
+    const int n = c % 3;
+    double xf = 0.0;
+    for (int b = 0; b < blocksPerSystem; b++)
+    {
+        xf += partials[(static_cast<size_t>(sys) * blocksPerSystem + b) * 9 + c];
+    }
+    double sf = 0.0;
+    for (int s = 0; s < NBNXM_NUM_SHIFTS; s++)
+    {
+        const float4 sv = shiftVec[sys * NBNXM_NUM_SHIFTS + s];
+        const double svd = (d == 0) ? sv.x : ((d == 1) ? sv.y : sv.z);
+        sf = fma(svd, fshift[3 * (sys * NBNXM_NUM_SHIFTS + s) + n], sf);
+    }
+    virial[9 * sys + c] = -0.5 * (sf + xf);
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used as a counter-based
 // generator: key = seed, counter = (atom, step).
 __device__ __forceinline__ uint4 philox4x32_10(uint4 ctr, uint2 key)
@@ -1196,6 +1295,33 @@ hipError_t launchReduceEnergies(const double* entryPartials,
         return hipSuccess;
     }
     hipLaunchKernelGGL(reduceEnergiesKernel, dim3(numSystems * (1 + NBNXM_NUM_SHIFTS)), dim3(64), 0, stream, entryPartials, entryRanges, entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    return hipGetLastError();
+}
+
+hipError_t launchVirialPartials(const float4* xq,
+                                const unsigned long long* fFixed,
+                                const float* fFloat,
+                                const int* slotRanges,
+                                int numSystems,
+                                int blocksPerSystem,
+                                double* partials,
+                                hipStream_t stream)
+{
+    if (numSystems <= 0 || blocksPerSystem <= 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(virialPartialKernel, dim3(blocksPerSystem, numSystems), dim3(256), 0, stream, xq, fFixed, fFloat, slotRanges, blocksPerSystem, partials);
+    return hipGetLastError();
+}
+
+hipError_t launchVirialFinish(const double* partials, int numSystems, int blocksPerSystem, const float4* shiftVec, const double* fshift, double* virial, hipStream_t stream)
+{
+    if (numSystems <= 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(virialFinishKernel, dim3(numSystems), dim3(64), 0, stream, partials, blocksPerSystem, shiftVec, fshift, virial);
     return hipGetLastError();
 }
 

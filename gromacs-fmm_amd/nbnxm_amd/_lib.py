@@ -111,6 +111,7 @@ SIGNATURES = [
     ("nbnxm_gpu_launch_kernel_pruneonly", _c_int, [_vp, _c_int]),
     ("nbnxm_gpu_launch_cpyback", _c_int, [_vp, _vp, _vp]),
     ("nbnxm_gpu_wait_finish_task", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_get_virial", _c_int, [_vp, _vp]),
     ("nbnxm_gpu_min_ci_balanced", _c_int, [_vp]),
     ("nbnxm_gpu_is_kernel_ewald_analytical", _c_int, [_vp]),
     ("nbnxm_gpu_get_masks", _c_int, [_vp, _vp, _vp]),
@@ -131,7 +132,24 @@ SIGNATURES = [
     ("nbnxm_ensemble_system_list", _c_int, [_vp, _c_int, _PTR(PairlistGpu), _vp]),
     ("nbnxm_ensemble_count_pairs", _c_int, [_vp, _c_double, _vp, _vp]),
     ("nbnxm_ensemble_free", None, [_vp]),
+    ("nbnxm_detect_comb_rule", _c_int, [_c_int, _vp, _c_double]),
+    ("nbnxm_force_calculator_create", _c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _PTR(_vp)]),
+    ("nbnxm_force_calculator_update_pairlist", _c_int, [_vp, _c_int, _vp, _vp]),
+    ("nbnxm_force_calculator_compute", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("nbnxm_force_calculator_free", None, [_vp]),
 ]
+
+COULOMB_PME = 0
+COULOMB_CUTOFF = 1
+COULOMB_REACTION_FIELD = 2
+NUM_ENERGY_TERMS = 5
+
+
+class KernelOptions(ctypes.Structure):
+    _fields_ = [("useGpu", ctypes.c_int), ("numOpenMPThreads", ctypes.c_int),
+                ("pairlistCutoff", ctypes.c_double), ("coulombType", ctypes.c_int),
+                ("useTabulatedEwaldCorr", ctypes.c_int), ("ewaldRtol", ctypes.c_double),
+                ("epsilonR", ctypes.c_double), ("rlist", ctypes.c_double), ("accumulation", ctypes.c_int)]
 
 
 class SystemDesc(ctypes.Structure):

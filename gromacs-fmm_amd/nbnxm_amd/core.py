@@ -267,6 +267,12 @@ class GpuTask:
         check(self._lib.nbnxm_gpu_wait_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift)), "gpu_wait_finish_task")
         return f, fd, elj.value, eel.value, fshift
 
+    def virial(self):
+        """Single-sum virial (3, 3) of the last launch made with compute_virial."""
+        v = np.zeros(9, dtype=np.float64)
+        check(self._lib.nbnxm_gpu_get_virial(self._h, ptr(v)), "gpu_get_virial")
+        return v.reshape(3, 3)
+
     def masks(self):
         n = len(self.plist.cj_packed)
         inner = np.zeros(n, dtype=np.uint32)

@@ -199,3 +199,128 @@ def spc27_system(golden, comb_rule):
     return System(box=box, x=x, q=q, types=types, c6c12=c6c12, excl_ranges=ranges,
                   excl_elements=elements, comb_rule=comb_rule, per_type_sigma_eps=sig_eps,
                   name=f"spc27_{comb_rule}")
+
+
+SOLUTE_BOND = 0.15
+SOLUTE_ATOM_MASS_AMU = 14.027  # united-atom CH2: the mass that sets the solvated box volume
+SOLUTE_MIN_DISTANCE = 0.2      # self-avoidance of the random walk (non-bonded solute atoms)
+WATER_SOLUTE_MIN_DISTANCE = 0.3
+
+
+def _random_unit(rng):
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def _random_walk_chains(rng, n_chains, chain_len, box):
+    """Self-avoiding-by-rejection random walks with fixed bond length in a
+    periodic box: every new atom keeps SOLUTE_MIN_DISTANCE from all earlier
+    solute atoms except its bonded predecessor."""
+    cell = SOLUTE_MIN_DISTANCE
+    ncell = np.maximum((box / cell).astype(int), 1)
+    grid = {}
+    placed = np.zeros((n_chains * chain_len, 3))
+
+    def key(p):
+        return tuple((np.floor(p / box * ncell).astype(int)) % ncell)
+
+    def min_dist2(p, skip):
+        k = key(p)
+        best = np.inf
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    for a in grid.get(((k[0] + dx) % ncell[0], (k[1] + dy) % ncell[1], (k[2] + dz) % ncell[2]), ()):
+                        if a == skip:
+                            continue
+                        d = p - placed[a]
+                        d -= np.round(d / box) * box
+                        best = min(best, float(d @ d))
+        return best
+
+    for c in range(n_chains):
+        for m in range(chain_len):
+            a = c * chain_len + m
+            best_p, best_d = None, -1.0
+            for _ in range(50):
+                if m == 0:
+                    p = rng.uniform(0.0, 1.0, 3) * box
+                else:
+                    p = placed[a - 1] + SOLUTE_BOND * _random_unit(rng)
+                p = p - np.floor(p / box) * box
+                d2 = min_dist2(p, a - 1 if m > 0 else -1)
+                if d2 > best_d:
+                    best_p, best_d = p, d2
+                if d2 >= SOLUTE_MIN_DISTANCE ** 2:
+                    break
+            placed[a] = best_p
+            grid.setdefault(key(best_p), []).append(a)
+    return placed
+
+
+def hetero_system(n_chains=480, chain_len=50, total_atoms=1_000_000, n_types=20, seed=6, name=""):
+    """Solute of random-walk chains (bond 0.15 nm) with n_types LJ types
+    (sigma U[0.25, 0.40] nm, epsilon U[0.1, 1.0] kJ/mol, Lorentz-Berthelot),
+    charges N(0, 0.3 e) neutralised per chain and 1-2/1-3 exclusions,
+    solvated with the rigid water to 1.0 g/cm^3 (solute atoms weigh
+    SOLUTE_ATOM_MASS_AMU) up to total_atoms.  Types 0..n_types-1 are solute,
+    n_types the water O and n_types+1 the water H."""
+    rng = np.random.default_rng(seed)
+    n_sol = n_chains * chain_len
+    n_wat = (total_atoms - n_sol) // 3
+    volume = n_wat * WATER_MASS_AMU / AMU_PER_NM3_AT_1G_CM3 + n_sol * SOLUTE_ATOM_MASS_AMU / AMU_PER_NM3_AT_1G_CM3
+    edge = volume ** (1.0 / 3.0)
+    box = np.array([edge] * 3)
+
+    sig = rng.uniform(0.25, 0.40, n_types)
+    eps = rng.uniform(0.1, 1.0, n_types)
+    sig_eps = np.zeros((n_types + 2, 2))
+    sig_eps[:n_types, 0] = sig
+    sig_eps[:n_types, 1] = eps
+    sig_eps[n_types] = (SIGMA_O, EPSILON_O)
+    c6c12 = combine_params(sig_eps, "lb")
+
+    xs = _random_walk_chains(rng, n_chains, chain_len, box)
+    ts = rng.integers(0, n_types, n_sol).astype(np.int32)
+    qs = rng.normal(0.0, 0.3, (n_chains, chain_len))
+    qs -= qs.mean(axis=1, keepdims=True)
+    qs = qs.reshape(-1)
+
+    # Water: lattice candidates away from the solute, then n_wat of them at random.
+    from scipy.spatial import cKDTree
+    tree = cKDTree(xs, boxsize=edge)
+    n_cand = int(n_wat * volume / (n_wat / WATER_MOLECULES_PER_NM3) * 1.05) + 8
+    while True:
+        xw = water_positions(n_cand, edge, rng).reshape(-1, 3, 3)
+        xw -= np.floor(xw / box) * box
+        d, _ = tree.query(xw[:, 0, :], k=1)
+        free = np.flatnonzero(d >= WATER_SOLUTE_MIN_DISTANCE)
+        if free.size >= n_wat:
+            break
+        n_cand = int(n_cand * 1.1) + 8
+    keep = np.sort(rng.choice(free, size=n_wat, replace=False))
+    xw = xw[keep].reshape(-1, 3)
+
+    x = np.concatenate([xs, xw])
+    q = np.concatenate([qs, np.tile([Q_O, Q_H, Q_H], n_wat)])
+    types = np.concatenate([ts, np.tile(np.array([n_types, n_types + 1], dtype=np.int32)[[0, 1, 1]], n_wat)]).astype(np.int32)
+
+    # Exclusions: self, 1-2 and 1-3 along each chain; 3x3 per water.
+    elements = []
+    ranges = [0]
+    for a in range(n_sol):
+        m = a % chain_len
+        lo = a - min(m, 2)
+        hi = a + min(chain_len - 1 - m, 2)
+        elements.extend(range(lo, hi + 1))
+        ranges.append(len(elements))
+    wr, we = water_exclusions(n_wat, offset=n_sol)
+    ranges = np.concatenate([np.array(ranges, dtype=np.int64), wr[1:].astype(np.int64) + len(elements)]).astype(np.int32)
+    elements = np.concatenate([np.array(elements, dtype=np.int32), we]).astype(np.int32)
+    return System(box=box, x=x, q=q, types=types, c6c12=c6c12, excl_ranges=ranges, excl_elements=elements,
+                  comb_rule="lb", per_type_sigma_eps=sig_eps, name=name or f"hetero{x.shape[0]}")
+
+
+def config_hetero_1m(seed=6):
+    """480 chains x 50 atoms, 20 LJ types (LB), solvated to 1.0 M atoms."""
+    return hetero_system(480, 50, 1_000_000, 20, seed=seed, name="hetero1M")

@@ -226,6 +226,12 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble);
 /* gpu_wait_finish_task (src/gromacs/nbnxm/gpu_common.h:345): waits and adds the
  * energies and shift forces (3*45) of the last launch to the outputs. */
 int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift);
+/* Adds the single-sum virial (9, row-major) of the last launch made with
+ * computeVirial to virial, after waiting.  Replaces the host calc_vir over the
+ * nbat forces and shift forces (src/gromacs/mdlib/calcvir.cpp:89-133, as
+ * api/nblib/virials.cpp:55-84 uses it): here the sum over atoms and shifts is
+ * reduced on the GPU in FP64 in a fixed order. */
+int nbnxm_gpu_get_virial(nbnxm_gpu_t* nb, double* virial);
 /* gpu_min_ci_balanced (gpu_data_mgmt.h:119). */
 int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb);
 /* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): always 1 here. */
@@ -319,6 +325,91 @@ int nbnxm_ensemble_system_list(nbnxm_ensemble_t* e, int system, nbnxm_pairlist_g
  * the cut-off).  perSystem may be NULL. */
 int nbnxm_ensemble_count_pairs(nbnxm_ensemble_t* e, double rc, int64_t* perSystem, int64_t* total);
 void nbnxm_ensemble_free(nbnxm_ensemble_t* e);
+
+/* ---------------------------------------------------------------------------
+ * nblib-style force calculator: nblib::GmxNBForceCalculatorCpu
+ * (api/nblib/include/nblib/gmxcalculatorcpu.h:67-105,
+ * api/nblib/gmxcalculatorcpu.cpp:108-251) backed by the GPU kernel, the
+ * counterpart the reference only sketches in createNbnxmGPU
+ * (api/nblib/nbnxmsetuphelpers.cpp:313-357).
+ * ------------------------------------------------------------------------- */
+
+/* Combination-rule detection of nbnxn_atomdata_t::Params
+ * (src/gromacs/nbnxm/atomdata.cpp:421-540, enbnxninitcombruleDETECT):
+ * nbfp holds (6*C6, 12*C12) per type pair, numTypes^2 pairs; tol is the
+ * relative tolerance (the reference's default is 1e-5).  Returns the
+ * nbnxm_vdw_type (geometric preferred over Lorentz-Berthelot), or a negative
+ * error code. */
+int nbnxm_detect_comb_rule(int numTypes, const double* nbfp, double tol);
+
+/* nblib::CoulombType (api/nblib/include/nblib/kerneloptions.h:73-79). */
+enum nbnxm_coulomb_type
+{
+    NBNXM_COULOMB_PME = 0,           /* Ewald real space, analytical correction (supported) */
+    NBNXM_COULOMB_CUTOFF = 1,        /* not supported by this kernel */
+    NBNXM_COULOMB_REACTION_FIELD = 2 /* not supported by this kernel */
+};
+
+/* Energy terms of the output, in NonBondedEnergyTerms order
+ * (src/gromacs/mdtypes/enerdata.h:52-60): CoulombSR, LJSR, BuckinghamSR,
+ * Coulomb14, LJ14 (one energy-group pair on the GPU). */
+#define NBNXM_NUM_ENERGY_TERMS 5
+
+/* nblib::NBKernelOptions (api/nblib/include/nblib/kerneloptions.h:84-102),
+ * the fields that apply to the GPU kernel.  The nblib defaults are
+ * pairlistCutoff 1.0, coulombType PME, analytical Ewald, Ewald rtol 1e-5,
+ * epsilon_r 1 (api/nblib/nbnxmsetuphelpers.cpp:219-272). */
+typedef struct nbnxm_kernel_options
+{
+    int useGpu;                /* must be nonzero: the kernel runs on the GPU only */
+    int numOpenMPThreads;      /* host threads for the search (0 = hardware) */
+    double pairlistCutoff;     /* rc = rvdw = rcoulomb; also the list radius unless rlist > 0 */
+    int coulombType;           /* enum nbnxm_coulomb_type: PME only */
+    int useTabulatedEwaldCorr; /* must be 0: the kernel is analytical */
+    double ewaldRtol;          /* 0 -> 1e-5 */
+    double epsilonR;           /* 0 -> 1 */
+    double rlist;              /* 0 -> pairlistCutoff (nblib); larger gives a buffered list */
+    int accumulation;          /* enum nbnxm_accumulation */
+} nbnxm_kernel_options_t;
+
+typedef struct nbnxm_force_calculator nbnxm_force_calculator_t;
+
+/* GmxNBForceCalculatorCpu(particleTypeIdOfAllParticles, nonBondedParams,
+ * charges, particleInteractionFlags, exclusionRanges, exclusionElements,
+ * options).  nonBondedParams: (6*C6, 12*C12) per type pair, numTypes^2
+ * pairs (createNonBondedParameters, api/nblib/nbnxmsetuphelpers.cpp:153-177).
+ * Exclusions as a list of lists: exclusionRanges[numParticles+1] offsets into
+ * exclusionElements (either may be NULL for none).  One energy group: the
+ * reference's particleInteractionFlags carry energy groups, which the GPU path
+ * does not support (nbnxmsetuphelpers.cpp:328), so there is no such
+ * argument.  The combination rule is detected as in nbnxm_detect_comb_rule. */
+int nbnxm_force_calculator_create(int numParticles,
+                                  const int* particleTypeIdOfAllParticles,
+                                  int numTypes,
+                                  const double* nonBondedParams,
+                                  const double* charges,
+                                  const int* exclusionRanges,
+                                  const int* exclusionElements,
+                                  const nbnxm_kernel_options_t* options,
+                                  nbnxm_force_calculator_t** out);
+/* updatePairlist(coordinates, box) (gmxcalculatorcpu.cpp:125-171): puts the
+ * coordinates in the rectangular box in place (put_atoms_in_box_omp), builds
+ * the grid and the GPU pair list and uploads them.  box: the three edges;
+ * each edge must exceed the list radius. */
+int nbnxm_force_calculator_update_pairlist(nbnxm_force_calculator_t* c, int numCoordinates, double* coordinates, const double box[3]);
+/* compute(coordinates, box, forces[, virial[, energies]])
+ * (gmxcalculatorcpu.cpp:173-251): forces (3*numParticles) are ADDED to, as
+ * atomdata_add_nbat_f_to_f does; virial (9, or NULL) is overwritten with the
+ * single-sum virial; energies (NBNXM_NUM_ENERGY_TERMS, or NULL) are
+ * overwritten.  Fails with NBNXM_ERR_STATE before the first update_pairlist. */
+int nbnxm_force_calculator_compute(nbnxm_force_calculator_t* c,
+                                   int numCoordinates,
+                                   const double* coordinates,
+                                   const double box[3],
+                                   double* forces,
+                                   double* virial,
+                                   double* energies);
+void nbnxm_force_calculator_free(nbnxm_force_calculator_t* c);
 
 #ifdef __cplusplus
 }

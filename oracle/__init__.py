@@ -60,6 +60,8 @@ def load():
         fn.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]
     lib.oracle_bruteforce_f64.restype = ctypes.c_int
     lib.oracle_bruteforce_f64.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.oracle_bruteforce_virial_f64.restype = ctypes.c_int
+    lib.oracle_bruteforce_virial_f64.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     lib.oracle_prune.restype = ctypes.c_int
     lib.oracle_prune.argtypes = [_vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     lib.oracle_sci_sort.restype = ctypes.c_int
@@ -138,9 +140,7 @@ def kernel(plist, atoms, ic, compute_energy=True, precision="f64"):
     return f.reshape(-1, 3), fs.reshape(-1, 3), elj.value, eel.value
 
 
-def bruteforce(system, ic):
-    """All-image FP64 sum in original atom order: (f (n,3), e_lj, e_el, pairs in range)."""
-    lib = load()
+def _bruteforce_inputs(system):
     n = system.n
     x = np.ascontiguousarray(system.x, dtype=np.float64)
     q = np.ascontiguousarray(system.q, dtype=np.float64)
@@ -153,6 +153,13 @@ def bruteforce(system, ic):
     r = np.ascontiguousarray(system.excl_ranges, dtype=np.int32)
     e = np.ascontiguousarray(system.excl_elements, dtype=np.int32)
     box = np.ascontiguousarray(system.box, dtype=np.float64)
+    return n, x, q, t, nt, nbfp, r, e, box
+
+
+def bruteforce(system, ic):
+    """All-image FP64 sum in original atom order: (f (n,3), e_lj, e_el, pairs in range)."""
+    lib = load()
+    n, x, q, t, nt, nbfp, r, e, box = _bruteforce_inputs(system)
     f = np.zeros(3 * n)
     elj = ctypes.c_double(0.0)
     eel = ctypes.c_double(0.0)
@@ -164,6 +171,30 @@ def bruteforce(system, ic):
     if rc != 0:
         raise RuntimeError(f"oracle bruteforce failed: {rc}")
     return f.reshape(-1, 3), elj.value, eel.value, npairs.value
+
+
+def bruteforce_virial(system, ic):
+    """All-image FP64 sum: (f (n,3), pair virial (3,3) = -1/2 sum r_ij (x) F_ij)."""
+    lib = load()
+    n, x, q, t, nt, nbfp, r, e, box = _bruteforce_inputs(system)
+    f = np.zeros(3 * n)
+    vir = np.zeros(9)
+    icv = _ic(ic)
+    rc = lib.oracle_bruteforce_virial_f64(n, _p(x), _p(q), _p(t), nt, _p(nbfp), _p(r), _p(e) if e.size else None,
+                                          _p(box), ctypes.byref(icv), _p(f), _p(vir))
+    if rc != 0:
+        raise RuntimeError(f"oracle bruteforce failed: {rc}")
+    return f.reshape(-1, 3), vir.reshape(3, 3)
+
+
+def single_sum_virial(x, f, shift_vec, fshift):
+    """-1/2 (sum_i x_i (x) f_i + sum_s shift_s (x) fshift_s), the single-sum
+    virial of calc_vir (src/gromacs/mdlib/calcvir.cpp:89-133) in FP64."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    f = np.asarray(f, dtype=np.float64).reshape(-1, 3)
+    sv = np.asarray(shift_vec, dtype=np.float64).reshape(-1, 3)
+    fs = np.asarray(fshift, dtype=np.float64).reshape(-1, 3)
+    return -0.5 * (x.T @ f + sv.T @ fs)
 
 
 def prune(plist, xq, shift_vec, rlist_outer, rlist_inner, fresh, part=0, num_parts=1,

@@ -485,7 +485,7 @@ static int is_excluded(int i, int j, const int* ranges, const int* elems)
     return 0;
 }
 
-int oracle_bruteforce_f64(int n,
+static int bruteforce_f64(int n,
                           const double* x,
                           const double* q,
                           const int* type,
@@ -498,7 +498,8 @@ int oracle_bruteforce_f64(int n,
                           double* f,
                           double* Vvdw,
                           double* Vc,
-                          long long* numPairsInRange)
+                          long long* numPairsInRange,
+                          double* virial)
 {
     if (n < 0 || !x || !q || !type || !nbfp || !box || !ic || !f)
     {
@@ -586,6 +587,18 @@ int oracle_bruteforce_f64(int n,
                         f[3 * j + 0] -= fr * dx;
                         f[3 * j + 1] -= fr * dy;
                         f[3 * j + 2] -= fr * dz;
+                        if (virial)
+                        {
+                            /* pair virial -1/2 r_ij (x) F_ij, F_ij = fr r_ij */
+                            const double r[3] = { dx, dy, dz };
+                            for (int a = 0; a < 3; a++)
+                            {
+                                for (int b = 0; b < 3; b++)
+                                {
+                                    virial[3 * a + b] -= 0.5 * r[a] * fr * r[b];
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -606,6 +619,44 @@ int oracle_bruteforce_f64(int n,
         *numPairsInRange = npairs;
     }
     return NBNXM_OK;
+}
+
+int oracle_bruteforce_f64(int n,
+                          const double* x,
+                          const double* q,
+                          const int* type,
+                          int numTypes,
+                          const double* nbfp,
+                          const int* exclRanges,
+                          const int* exclElements,
+                          const double box[3],
+                          const nbnxm_interaction_const_t* ic,
+                          double* f,
+                          double* Vvdw,
+                          double* Vc,
+                          long long* numPairsInRange)
+{
+    return bruteforce_f64(n, x, q, type, numTypes, nbfp, exclRanges, exclElements, box, ic, f, Vvdw, Vc, numPairsInRange, NULL);
+}
+
+int oracle_bruteforce_virial_f64(int n,
+                                 const double* x,
+                                 const double* q,
+                                 const int* type,
+                                 int numTypes,
+                                 const double* nbfp,
+                                 const int* exclRanges,
+                                 const int* exclElements,
+                                 const double box[3],
+                                 const nbnxm_interaction_const_t* ic,
+                                 double* f,
+                                 double* virial)
+{
+    if (!virial)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    return bruteforce_f64(n, x, q, type, numTypes, nbfp, exclRanges, exclElements, box, ic, f, NULL, NULL, NULL, virial);
 }
 
 /* Restates nbnxn_kernel_prune_cuda<haveFreshList>

@@ -18,6 +18,9 @@
  *   oracle_prune               double-precision any-pair-within-r predicate away
  *                              from the threshold; masks subset of input masks
  *   oracle_sci_sort            counts non-increasing, permutation of input
+ *   oracle_bruteforce_virial_f64  hand value for two atoms; equality with the
+ *                              single-sum virial from oracle_kernel_f64 forces
+ *                              and shift forces (an independent formulation)
  */
 #ifndef NBNXM_ORACLE_H
 #define NBNXM_ORACLE_H
@@ -73,6 +76,22 @@ int oracle_bruteforce_f64(int n,
                           double* Vvdw,
                           double* Vc,
                           long long* numPairsInRange);
+
+/* Same all-image sum, accumulating forces and the pair virial
+ * -1/2 sum_pairs r_ij (x) F_ij (9, row-major; added to virial), the quantity
+ * the single-sum virial of src/gromacs/mdlib/calcvir.cpp:89-133 equals. */
+int oracle_bruteforce_virial_f64(int n,
+                                 const double* x,
+                                 const double* q,
+                                 const int* type,
+                                 int numTypes,
+                                 const double* nbfp,
+                                 const int* exclRanges,
+                                 const int* exclElements,
+                                 const double box[3],
+                                 const nbnxm_interaction_const_t* ic,
+                                 double* f,
+                                 double* virial);
 
 /* Prune restatement (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343)
  * with split 1.  imaskOuter / imaskInner are per packed group.  On a fresh

@@ -1,0 +1,136 @@
+"""GPU: the nblib-style force calculator, the FP64 virial reduction and the
+heterogeneous chain system, against the CPU oracle and the golden vectors.
+Calls go through the C ABI."""
+import numpy as np
+import pytest
+
+import nbnxm_amd as nb
+import oracle
+from conftest import announce
+from helpers import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+REF_FORCE_TOL = 0.0119  # kernel_test.cpp:663-698 in FP32
+
+
+def nb_params(system):
+    """(6*C6, 12*C12) per type pair, as nblib's createNonBondedParameters."""
+    return np.stack([6.0 * system.c6c12[..., 0], 12.0 * system.c6c12[..., 1]], axis=-1)
+
+
+@pytest.fixture(scope="module")
+def water9k():
+    return nb.systems.config_water_small()
+
+
+@pytest.fixture(scope="module")
+def water9k_oracle(water9k):
+    ic = nb.interaction_const(1.0, 1e-5)
+    search, plist, atoms = nb.build(water9k, 1.1, nb.VDW_CUT_COMB_GEOM)
+    f, fs, elj, eel = oracle.kernel(plist, atoms, ic, True, "f64")
+    real = atoms.atom_indices >= 0
+    xq = atoms.xq.reshape(-1, 4)[:, :3].astype(np.float64)
+    vir = oracle.single_sum_virial(xq[real], f[real], atoms.shift_vec, fs)
+    return dict(f=atoms.to_original_order(f, water9k.n), elj=elj, eel=eel, vir=vir, atoms=atoms, plist=plist, ic=ic)
+
+
+def test_calculator_water9k(water9k, water9k_oracle):
+    announce("calculator water9k fixed64")
+    opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1)
+    calc = nb.ForceCalculator(water9k.types, nb_params(water9k), water9k.q, water9k.excl_ranges,
+                              water9k.excl_elements, opt)
+    x = calc.update_pairlist(water9k.x.copy(), water9k.box)
+    f, vir, en = calc.compute(x, water9k.box, virial=True, energies=True)
+    ref = water9k_oracle
+    assert rel_rms(f, ref["f"]) <= 5e-6
+    assert en[1] == pytest.approx(ref["elj"], rel=1e-5)
+    assert en[0] == pytest.approx(ref["eel"], rel=1e-5)
+    assert en[2:].tolist() == [0.0, 0.0, 0.0]
+    assert np.abs(vir - ref["vir"]).max() <= 1e-5 * np.abs(ref["vir"]).max()
+    # Forces are added to the output buffer (atomdata_add_nbat_f_to_f); the
+    # same kernel flavour (F+E) gives bitwise the same fixed-point forces.
+    f2, vir2, en2 = calc.compute(x, water9k.box, forces=f.copy(), virial=True, energies=True)
+    assert np.array_equal(f2, 2.0 * f)
+    assert np.array_equal(vir2, vir) and np.array_equal(en2, en)
+    # The force-only flavour agrees within FP32 rounding.
+    f3, _, _ = calc.compute(x, water9k.box)
+    assert rel_rms(f3, f) <= 1e-6
+
+
+def test_calculator_fp64_mode(water9k, water9k_oracle):
+    announce("calculator water9k fp64")
+    opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1, accumulation=nb.ACCUM_FP64)
+    calc = nb.ForceCalculator(water9k.types, nb_params(water9k), water9k.q, water9k.excl_ranges,
+                              water9k.excl_elements, opt)
+    x = calc.update_pairlist(water9k.x.copy(), water9k.box)
+    f, vir, en = calc.compute(x, water9k.box, virial=True, energies=True)
+    ref = water9k_oracle
+    # FP64 on both sides; the kernel reads FP32 coordinates and parameters like the oracle.
+    assert rel_rms(f, ref["f"]) <= 1e-9
+    assert en[0] == pytest.approx(ref["eel"], rel=1e-9)
+    assert np.abs(vir - ref["vir"]).max() <= 1e-8 * np.abs(ref["vir"]).max()
+
+
+@pytest.mark.parametrize("rule", ["geom", "lb"])
+def test_calculator_golden(golden, rule):
+    """The reference kernel test through the calculator: the combination rule is
+    detected from the type table, as enbnxninitcombruleDETECT does."""
+    announce(f"calculator golden {rule}")
+    system = nb.systems.spc27_system(golden, rule)
+    case = {"geom": "CoulombEwald_VdwCutCombGeom", "lb": "CoulombEwald_VdwCutCombLB"}[rule]
+    assert nb.detect_comb_rule(nb_params(system)) == (nb.VDW_CUT_COMB_GEOM if rule == "geom" else nb.VDW_CUT_COMB_LB)
+    opt = nb.NBKernelOptions(pairlistCutoff=0.9, ewaldRtol=1e-6)
+    calc = nb.ForceCalculator(system.types, nb_params(system), system.q, system.excl_ranges,
+                              system.excl_elements, opt)
+    x = calc.update_pairlist(system.x.copy(), system.box)
+    f, _, en = calc.compute(x, system.box, energies=True)
+    ref = np.array(golden["cases"][case]["forces"])
+    assert np.abs(f - ref).max() < REF_FORCE_TOL
+    assert abs(en[1] - golden["cases"][case]["vdw_energy"]) < REF_FORCE_TOL
+    assert abs(en[0] - golden["cases"][case]["coulomb_energy"]) < 10 * REF_FORCE_TOL
+
+
+def test_gpu_task_virial(water9k, water9k_oracle):
+    announce("gpu task virial water9k")
+    ref = water9k_oracle
+    for acc, tol in [(nb.ACCUM_FIXED64, 1e-5), (nb.ACCUM_FLOAT, 1e-5), (nb.ACCUM_FP64, 1e-8)]:
+        task = nb.GpuTask(ref["ic"], ref["atoms"], 1.1, None, acc)
+        task.init_pairlist(ref["plist"])
+        task.launch(compute_energy=False, compute_virial=True)
+        task.fetch()
+        vir = task.virial()
+        assert np.abs(vir - ref["vir"]).max() <= tol * np.abs(ref["vir"]).max(), acc
+    # Without computeVirial in the last launch there is no virial to return.
+    task.launch(compute_energy=True, compute_virial=False)
+    task.fetch()
+    with pytest.raises(nb.NbnxmError, match="virial"):
+        task.virial()
+
+
+@pytest.fixture(scope="module")
+def hetero_small():
+    return nb.systems.hetero_system(12, 50, 6000, 20, seed=12)
+
+
+@pytest.mark.parametrize("vdw", [nb.VDW_CUT_TYPE_TABLE, nb.VDW_CUT_COMB_LB])
+def test_hetero_chains(hetero_small, vdw):
+    """20 LB types from the LDS-staged type table (or the LB rule), 1-2/1-3
+    exclusions inside the chains."""
+    announce(f"hetero chains vdw={vdw}")
+    s = hetero_small
+    ic = nb.interaction_const(1.0, 1e-5)
+    search, plist, atoms = nb.build(s, 1.1, vdw)
+    f64, fs64, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    f32, _, _, _ = oracle.kernel(plist, atoms, ic, True, "f32")
+    task = nb.GpuTask(ic, atoms, 1.1, None, nb.ACCUM_FIXED64)
+    task.init_pairlist(plist)
+    task.launch(compute_energy=True, compute_virial=True)
+    f, fd, elj, eel, fshift = task.fetch()
+    real = atoms.atom_indices >= 0
+    gpu = fd.reshape(-1, 3)[real]
+    assert rel_rms(gpu, f64[real]) <= 5e-6
+    assert rel_rms(gpu, f32[real]) <= 1e-5
+    assert elj == pytest.approx(elj64, rel=1e-5)
+    assert eel == pytest.approx(eel64, rel=1e-5)
+    assert np.abs(fshift.reshape(-1, 3) - fs64).max() <= 1e-5 * np.abs(fs64).max()
