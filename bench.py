@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--split", type=str, default="",
+                    help="fixed schedule instead of the autotuner: max pruned cluster pairs per work item for each "
+                         "size bucket, comma-separated (0 = no split), e.g. 512,512,512,0")
+    ap.add_argument("--wpb", type=int, default=4, help="waves per block with --split")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -116,7 +120,14 @@ def main():
     log(f"lists built and pruned in {time.perf_counter() - t0:.1f}s: {stats}, useful pairs {useful}")
 
     tuned = None
-    if not args.no_autotune:
+    if args.split:
+        split = [int(v) for v in args.split.split(",")]
+        if len(split) != len(BUCKET_EDGES) + 1:
+            sys.exit(f"--split needs {len(BUCKET_EDGES) + 1} values")
+        ens.set_split(BUCKET_EDGES, split, args.wpb)
+        tuned = {"bucket_edges_atoms": BUCKET_EDGES, "max_pairs_per_item": split, "waves_per_block": args.wpb,
+                 "fixed": True}
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         split, wpb = ens.autotune(BUCKET_EDGES)
         tuned = {"bucket_edges_atoms": BUCKET_EDGES, "max_pairs_per_item": split, "waves_per_block": wpb}

@@ -29,7 +29,7 @@ constexpr int c_ncps = NBNXM_NUM_CLUSTER_PER_SUPERCLUSTER;
 constexpr int c_jg = NBNXM_JGROUP_SIZE;
 constexpr float c_fixedScale = 4294967296.0f;        // 2^NBNXM_FIXED_FRAC_BITS
 constexpr double c_fixedScaleD = 4294967296.0;
-constexpr float c_fixedClamp = 9.0e18f;              // keeps the int64 conversion defined
+constexpr float c_fixedMaxF = 1073741824.0f;         // 2^30: keeps both 32-bit halves defined
 constexpr double c_oneSqrtPi = 0.56418958354775628695;
 
 template<int ctrl>
@@ -67,11 +67,19 @@ __device__ __forceinline__ double reduceWaveD(double v)
     return v;
 }
 
+// FP32 force -> int64 fixed point (2^-32 units) in two exact 32-bit halves:
+// hi = floor(v), lo = (v - floor(v)) * 2^32 in [0, 2^32), truncated.  Six
+// VALU operations instead of a generic float-to-int64 conversion; a
+// deterministic function of v, which is all that order-independent
+// accumulation needs.  |v| is clamped to 2^30 kJ/mol/nm.
 __device__ __forceinline__ unsigned long long toFixed(float v)
 {
-    float s = v * c_fixedScale;
-    s = fminf(fmaxf(s, -c_fixedClamp), c_fixedClamp);
-    return static_cast<unsigned long long>(__float2ll_rn(s));
+    v = __builtin_amdgcn_fmed3f(v, -c_fixedMaxF, c_fixedMaxF);
+    const float hi = __builtin_floorf(v);
+    const float lo = (v - hi) * c_fixedScale;
+    const unsigned int loBits = static_cast<unsigned int>(lo);
+    const unsigned int hiBits = static_cast<unsigned int>(static_cast<int>(hi));
+    return (static_cast<unsigned long long>(hiBits) << 32) | loBits;
 }
 
 __device__ __forceinline__ unsigned long long toFixedD(double v)
@@ -157,45 +165,13 @@ __device__ __forceinline__ float2 loadF2(ConstPtr<float2> p, int i)
     return make_float2(q[0], q[1]);
 }
 
-// erf(z)/z as a rational in z^2: the single-precision pmePotentialCorrection
-// (src/gromacs/simd/include/gromacs/simd/simd_math.h:1687-1720), the form the
-// reference CPU kernels use for the analytical Ewald energy; abs. error ~1e-6.
-__device__ __forceinline__ float erfOverZ(float z2)
-{
-    const float VN6 = 1.9296833005951166339e-8F;
-    const float VN5 = -1.4213390571557850962e-6F;
-    const float VN4 = 0.000041603292906656984871F;
-    const float VN3 = -0.00013134036773265025626F;
-    const float VN2 = 0.038657983986041781264F;
-    const float VN1 = 0.11285044772717598220F;
-    const float VN0 = 1.1283802385263030286F;
-    const float VD3 = 0.0066752224023576045451F;
-    const float VD2 = 0.078647795836373922256F;
-    const float VD1 = 0.43336185284710920150F;
-    const float VD0 = 1.0F;
-
-    const float z4 = z2 * z2;
-    float polyVD1 = __builtin_fmaf(VD3, z4, VD1);
-    float polyVD0 = __builtin_fmaf(VD2, z4, VD0);
-    polyVD0 = __builtin_fmaf(polyVD1, z2, polyVD0);
-    polyVD0 = __builtin_amdgcn_rcpf(polyVD0);
-    float polyVN0 = __builtin_fmaf(VN6, z4, VN4);
-    float polyVN1 = __builtin_fmaf(VN5, z4, VN3);
-    polyVN0 = __builtin_fmaf(polyVN0, z4, VN2);
-    polyVN1 = __builtin_fmaf(polyVN1, z4, VN1);
-    polyVN0 = __builtin_fmaf(polyVN0, z4, VN0);
-    polyVN0 = __builtin_fmaf(polyVN1, z2, polyVN0);
-    return polyVN0 * polyVD0;
-}
-
 // erfc(z) with relative accuracy for z in [0, 4.5]: erfc(z) = exp(-z^2) R(t),
 // t = 1/(1 + 0.7 z), R a degree-9 polynomial fitted to erfc(z) exp(z^2) by
 // relative least squares in FP64 (max relative error 2.7e-7 evaluated in FP32).
 // Used for pairs without exclusions, where 1 - erf(z) would lose the small
-// erfc(z) near the cut-off.
-__device__ __forceinline__ float erfcxPoly(float z)
+// erfc(z) near the cut-off.  Takes t; the caller forms it with beta folded in.
+__device__ __forceinline__ float erfcxPolyT(float t)
 {
-    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.7f, z, 1.0f));
     float p = 0.06597486883401871f;
     p = __builtin_fmaf(p, t, -0.3136824071407318f);
     p = __builtin_fmaf(p, t, 0.5114253163337708f);
@@ -207,11 +183,6 @@ __device__ __forceinline__ float erfcxPoly(float z)
     p = __builtin_fmaf(p, t, 0.39217787981033325f);
     p = __builtin_fmaf(p, t, 0.00013602453691419214f);
     return p;
-}
-
-__device__ __forceinline__ float erfcRelative(float z)
-{
-    return __expf(-z * z) * erfcxPoly(z);
 }
 
 // Force-only kernels fit 4 waves per SIMD (128 VGPRs); the energy kernels need
@@ -285,8 +256,8 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
     }
 
     using EAcc = typename std::conditional<accumulation == NBNXM_ACCUM_FLOAT, float, double>::type;
-    EAcc eLJ = 0;
-    EAcc eEl = 0;
+    EAcc eLJAcc = 0;
+    EAcc eElAcc = 0;
     if constexpr (computeEnergy)
     {
         // Self energy on the diagonal entry (:381-418): -epsfac*beta/sqrt(pi)*sum q^2,
@@ -301,7 +272,7 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                 {
                     q2 += qi[c] * qi[c];
                 }
-                eEl -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
+                eElAcc -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
             }
             else
             {
@@ -315,14 +286,19 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                     const double q = loadF4(xqc, (s.sci * c_ncps + c) * c_cl + ti).w;
                     q2 += q * q;
                 }
-                eEl -= pD[1] * q2 * pD[2] * c_oneSqrtPi;
+                eElAcc -= pD[1] * q2 * pD[2] * c_oneSqrtPi;
             }
         }
     }
 
-    // Newton / self masking on the diagonal of the central shift (:423).
-    const bool nonSelfLane = !(centralShift && tj <= ti);
-    const int firstCluster = s.sci * c_ncps;
+    // Newton / self masking on the diagonal of the central shift (:423): on a
+    // cluster pair with ci == cj only lanes with tj > ti interact.
+    const bool upperLane = tj > ti;
+    const int firstCluster = __builtin_amdgcn_readfirstlane(s.sci * c_ncps);
+    // Constants folded for the exclusion-free path.
+    const float negBeta2Log2e = -beta2 * 1.44269504088896340736f; // exp(-b^2 r^2) = exp2(r^2 * this)
+    const float beta07 = 0.7f * beta;                              // t = 1/(1 + 0.7 beta r)
+    const float twoBetaSqrtPi = beta * static_cast<float>(2.0 * c_oneSqrtPi);
 
     // Per-wave LDS staging of one packed group: 32 j-atoms' xq and LJ data.
     // The next group is fetched into registers (one coalesced load per lane)
@@ -334,18 +310,23 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
     float2* stageLj = sLj[threadIdx.x >> 6];
     const int stageAtom = lane & 31; // j-cluster stageAtom/8, atom stageAtom%8
 
-    // One packed group of 4 j-clusters against the 8 i-clusters.  Instantiated
-    // with and without exclusion handling: groups without exclusions (the
-    // large majority) skip the interaction-bit arithmetic and take the
-    // Coulomb energy from erfc directly.
     // Deferred j-force atomics: the reduced force component of lane ti < 3
     // for each j-cluster of the group, issued at the start of the next group.
     float pendV[c_jg];
     int pendCj[c_jg];
     unsigned int pendMask = 0u;
 
+    // One packed group of 4 j-clusters against the 8 i-clusters, instantiated
+    // with and without exclusion handling.  In a group with exclusions, only
+    // cluster pairs that hold an excluded lane take the interaction-bit path;
+    // the others, like every pair of an exclusion-free group, take the erfc
+    // path, where one exp(-z^2) serves both Coulomb force and energy.
+    // Energies are summed in FP32 over the group and added in FP64 per group;
+    // the LJ energy is kept as 12*E without its constant shift, plus the
+    // sums of C6 and C12 that carry the shift.
     auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
         constexpr bool hasExcl = decltype(withExcl)::value;
+        float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)
         {
@@ -359,8 +340,7 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
             const int typej = __float_as_int(ljStaged.x);
             const float2 ljcj = ljStaged;
             float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
-            float eljC = 0.0f, eelC = 0.0f;
-            bool anyInRange = false;
+            unsigned long long anyInRange = 0ull;
 #pragma unroll
             for (int c = 0; c < c_ncps; c++)
             {
@@ -373,11 +353,27 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                 const float dy = yi[c] - xqj.y;
                 const float dz = zi[c] - xqj.z;
                 float r2 = norm2Fixed(dx, dy, dz);
-                const bool notDiag = (firstCluster + c) != cj;
-                if (r2 < rcoulomb_sq && (nonSelfLane || notDiag))
+                // Cluster pairs on the diagonal of the central shift always carry
+                // the Newton mask in their exclusion words, so they take the
+                // interaction-bit path, the only one that needs the j > i test.
+                bool exclHere = false;
+                if constexpr (hasExcl)
                 {
-                    anyInRange = true;
-                    const float int_bit = hasExcl ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
+                    exclHere = __builtin_amdgcn_ballot_w64(((wexcl >> bit) & 1u) == 0u) != 0ull;
+                }
+                bool inRange;
+                if (hasExcl && exclHere)
+                {
+                    const bool diag = centralShift && (firstCluster + c == cj);
+                    inRange = (r2 < rcoulomb_sq) && (!diag || upperLane);
+                }
+                else
+                {
+                    inRange = r2 < rcoulomb_sq;
+                }
+                anyInRange |= __builtin_amdgcn_ballot_w64(inRange);
+                if (inRange)
+                {
                     float c6, c12;
                     float sigma = 0.0f, epsilon = 0.0f;
                     const float2 ljci = stageLi[c * c_cl + ti];
@@ -404,73 +400,79 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                     r2 = __builtin_amdgcn_fmed3f(r2, NBNXM_MIN_DISTANCE_SQUARED, rcoulomb_sq);
                     const float inv_r = __builtin_amdgcn_rsqf(r2);
                     const float inv_r2 = inv_r * inv_r;
-                    float F_invr;
-                    float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                    if constexpr (hasExcl)
-                    {
-                        inv_r6 *= int_bit;
-                    }
-                    if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
-                    {
-                        const float sig_r = sigma * inv_r;
-                        const float sig_r2 = sig_r * sig_r;
-                        float sig_r6 = sig_r2 * sig_r2 * sig_r2;
-                        if constexpr (hasExcl)
-                        {
-                            sig_r6 *= int_bit;
-                        }
-                        F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
-                    }
-                    else
-                    {
-                        F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-                    }
                     const float qq = qi[c] * xqj.w;
-                    const float z2 = beta2 * r2;
-                    if constexpr (computeEnergy && !hasExcl)
+                    float F_invr;
+                    if (hasExcl && exclHere)
                     {
-                        // Force and energy from one exp(-z^2): with erfc(z) = e R,
-                        // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2 and
-                        // V/qq = e R / r - sh_ewald; no cancellation in either.
-                        const float e = __expf(-z2);
-                        const float rPoly = erfcxPoly(r2 * inv_r * beta);
-                        const float eR = e * rPoly;
-                        F_invr += qq * e * __builtin_fmaf(rPoly, inv_r, beta * static_cast<float>(2.0 * c_oneSqrtPi)) * inv_r2;
-                        eljC += c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f) - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f);
-                        eelC += qq * __builtin_fmaf(eR, inv_r, -ewaldShift);
-                    }
-                    else
-                    {
-                        const float coulR3 = hasExcl ? int_bit * inv_r2 * inv_r : inv_r2 * inv_r;
-                        F_invr += qq * (coulR3 + pmecorrF(z2) * beta3);
+                        // The reference form with the interaction bit (:548-659).
+                        const float int_bit = static_cast<float>((wexcl >> bit) & 1u);
+                        const float inv_r6 = int_bit * (inv_r2 * inv_r2 * inv_r2);
+                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                        {
+                            const float sig_r = sigma * inv_r;
+                            const float sig_r2 = sig_r * sig_r;
+                            const float sig_r6 = int_bit * (sig_r2 * sig_r2 * sig_r2);
+                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
+                        }
+                        else
+                        {
+                            F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                        }
+                        const float z2 = beta2 * r2;
+                        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(z2) * beta3);
                         if constexpr (computeEnergy)
                         {
-                            // Groups with exclusions: the reference form
-                            // qq*((int_bit - erf(beta r))/r - int_bit*sh_ewald) (:656-659).
-                            const float ljPot = c12 * (inv_r6 * inv_r6 + cpotRep) * (1.0f / 12.0f) - c6 * (inv_r6 + cpotDisp) * (1.0f / 6.0f);
-                            eljC += int_bit * ljPot;
-                            eelC += qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
+                            eLJ12 = __builtin_fmaf(inv_r6, __builtin_fmaf(c12, inv_r6, -2.0f * c6), eLJ12);
+                            eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
+                            eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
+                            eElG += qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
                         }
                     }
-                    const float tx = dx * F_invr;
-                    const float ty = dy * F_invr;
-                    const float tz = dz * F_invr;
-                    fcjX -= tx;
-                    fcjY -= ty;
-                    fcjZ -= tz;
-                    fciX[c] += tx;
-                    fciY[c] += ty;
-                    fciZ[c] += tz;
+                    else
+                    {
+                        const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                        {
+                            const float sig_r = sigma * inv_r;
+                            const float sig_r2 = sig_r * sig_r;
+                            const float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2 + qq * (inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
+                        }
+                        else if constexpr (computeEnergy)
+                        {
+                            // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2, V/qq = e R / r - sh_ewald,
+                            // with e = exp(-beta^2 r^2) and erfc(beta r) = e R.
+                            const float r = r2 * inv_r;
+                            const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(beta07, r, 1.0f));
+                            const float R = erfcxPolyT(t);
+                            const float e = __builtin_amdgcn_exp2f(r2 * negBeta2Log2e);
+                            const float w = qq * e;
+                            const float u = __builtin_fmaf(R, inv_r, twoBetaSqrtPi);
+                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                            F_invr = __builtin_fmaf(inv_r6, tl, w * u) * inv_r2;
+                            eLJ12 = __builtin_fmaf(inv_r6, tl - c6, eLJ12);
+                            eSumC6 += c6;
+                            eSumC12 += c12;
+                            eElG = __builtin_fmaf(qq, __builtin_fmaf(e * R, inv_r, -ewaldShift), eElG);
+                        }
+                        else
+                        {
+                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                            const float coul = qq * __builtin_fmaf(pmecorrF(beta2 * r2), beta3, inv_r2 * inv_r);
+                            F_invr = __builtin_fmaf(inv_r6 * tl, inv_r2, coul);
+                        }
+                    }
+                    fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
+                    fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
+                    fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
+                    fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
+                    fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
+                    fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
                 }
-            }
-            if constexpr (computeEnergy)
-            {
-                eLJ += static_cast<EAcc>(eljC);
-                eEl += static_cast<EAcc>(eelC);
             }
             // j-force: sum over the 8 i-lanes; lanes ti = 0,1,2 keep x,y,z for
             // the deferred atomic.  Skipped when no pair was inside the cut-off.
-            if (__ballot(anyInRange) != 0)
+            if (anyInRange != 0ull)
             {
                 fcjX = reduceOver8Lanes(fcjX);
                 fcjY = reduceOver8Lanes(fcjY);
@@ -479,6 +481,12 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                 pendCj[jm] = cj;
                 pendMask |= 1u << jm;
             }
+        }
+        if constexpr (computeEnergy)
+        {
+            const float eLJ = (eLJ12 + cpotRep * eSumC12 - 2.0f * cpotDisp * eSumC6) * (1.0f / 12.0f);
+            eLJAcc += static_cast<EAcc>(eLJ);
+            eElAcc += static_cast<EAcc>(eElG);
         }
     };
 
@@ -625,8 +633,8 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
         double v0 = 0.0, v1 = 0.0;
         if constexpr (computeEnergy)
         {
-            v0 = reduceWaveD(static_cast<double>(eLJ));
-            v1 = reduceWaveD(static_cast<double>(eEl));
+            v0 = reduceWaveD(static_cast<double>(eLJAcc));
+            v1 = reduceWaveD(static_cast<double>(eElAcc));
         }
         const bool doShift = a.computeFshift && !centralShift;
         const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
