@@ -470,8 +470,9 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                     fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
                 }
             }
-            // j-force: sum over the 8 i-lanes; lanes ti = 0,1,2 keep x,y,z for
-            // the deferred atomic.  Skipped when no pair was inside the cut-off.
+            // j-force: sum over the 8 i-lanes, skipped when no pair was inside
+            // the cut-off; lanes ti = 0, 1, 2 keep x, y, z for the deferred
+            // atomic (consecutive lanes on consecutive addresses coalesce).
             if (anyInRange != 0ull)
             {
                 fcjX = reduceOver8Lanes(fcjX);
@@ -490,29 +491,33 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
         }
     };
 
+    // Lanes that issue the j-force atomics (ti = 0, 1, 2 for x, y, z) and the
+    // lane part of the address: component ti of j-atom tj of a cluster.
+    const bool jForceLane = ti < 3;
+    const int jForceLaneOffset = 3 * tj + ti;
     auto flushPending = [&]() {
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)
         {
             if ((pendMask >> jm) & 1u)
             {
-                if (ti < 3)
+                if (jForceLane)
                 {
-                    const int aj = pendCj[jm] * c_cl + tj;
+                    // Uniform cluster base (scalar) plus a 32-bit lane offset.
+                    const size_t base = static_cast<size_t>(pendCj[jm]) * (3 * c_cl);
                     if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
                     {
-                        atomicAdd(a.fFloat + 3 * aj + ti, pendV[jm]);
+                        atomicAdd(a.fFloat + base + jForceLaneOffset, pendV[jm]);
                     }
                     else
                     {
-                        atomicAdd(a.fFixed + 3 * aj + ti, toFixed(pendV[jm]));
+                        atomicAdd(a.fFixed + base + jForceLaneOffset, toFixed(pendV[jm]));
                     }
                 }
             }
         }
         pendMask = 0u;
     };
-
 
     auto nextNonEmpty = [&](int g) {
         while (g < s.cjPackedEnd && cjList[g].imei[0].imask == 0u)
@@ -523,10 +528,19 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
     };
     // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
     float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool stageK1 = (stageAtom >> 3) == 1;
+    const bool stageK2 = (stageAtom >> 3) == 2;
+    const bool stageK3 = (stageAtom >> 3) == 3;
     unsigned int pfExcl = 0xFFFFFFFFu;
     auto prefetch = [&](int g) {
-        const int k = stageAtom >> 3;
-        const int cjl = cjList[g].cj[0] * (k == 0) + cjList[g].cj[1] * (k == 1) + cjList[g].cj[2] * (k == 2) + cjList[g].cj[3] * (k == 3);
+        // Scalar loads first, then a per-lane select (a select of the
+        // addresses would become a vector load whose wait also waits for the
+        // in-flight force atomics).
+        const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
+        const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
+        const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
+        const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
+        const int cjl = stageK1 ? g1 : (stageK2 ? g2 : (stageK3 ? g3 : g0));
         const int atom = cjl * c_cl + (stageAtom & 7);
         if (lane < 32)
         {
