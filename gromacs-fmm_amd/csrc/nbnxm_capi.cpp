@@ -171,6 +171,11 @@ struct nbnxm_gpu
     DeviceBuffer<int> shiftRanges;
     DeviceBuffer<int> shiftEntryList;
     DeviceBuffer<int> slotRange;          // {0, numAtoms}
+    DeviceBuffer<int> atomIndex;          // grid slot -> atom or -1 (buffer ops)
+    DeviceBuffer<int> cellIndex;          // atom -> grid slot (buffer ops)
+    int numRealAtoms = -1;                // atoms of the buffer-op map, -1 = none
+    bool forcesConsumed = false;          // add_nbat_f_to_f took the forces of the last launch
+    bool haveLaunched = false;
     DeviceBuffer<double> virialPartials;  // 9 per block
     DeviceBuffer<double> virial;          // 9
     int virialBlocks = 1;
@@ -768,6 +773,8 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
     nb->lastComputeEnergy = computeEnergy != 0;
     nb->lastComputeVirial = computeVirial != 0;
+    nb->forcesConsumed = false;
+    nb->haveLaunched = true;
     return NBNXM_OK;
 }
 
@@ -778,33 +785,45 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         return fail(NBNXM_ERR_INVALID, "gpu_launch_cpyback: null handle");
     }
     const size_t n3 = 3 * static_cast<size_t>(nb->numAtoms);
-    if (nb->lastComputeVirial)
+    if (nb->forcesConsumed)
     {
-        // x (x) f partials before the fixed-point accumulator is cleared.
-        const bool fixed = nb->accumulation != NBNXM_ACCUM_FLOAT;
-        NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, fixed ? nb->fFixed.ptr : nullptr, fixed ? nullptr : nb->fFloat.ptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
-    }
-    if (nb->accumulation == NBNXM_ACCUM_FLOAT)
-    {
-        if (f && n3)
+        // add_nbat_f_to_f already reduced the forces and the virial partials;
+        // only energies, shift forces and the virial remain.
+        if (f || fDouble)
         {
-            NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fFloat.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
-        }
-        if (fDouble)
-        {
-            return fail(NBNXM_ERR_UNSUPPORTED, "gpu_launch_cpyback: double forces need fixed-point accumulation");
+            return fail(NBNXM_ERR_STATE, "gpu_launch_cpyback: the forces of this launch were taken by gpu_add_nbat_f_to_f");
         }
     }
     else
     {
-        NBNXM_HIP_CHECK(launchConvertForces(nb->fFixed.ptr, nb->numAtoms, nb->fOut.ptr, nb->fOutDouble.ptr, nb->stream));
-        if (f && n3)
+        if (nb->lastComputeVirial)
         {
-            NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fOut.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+            // x (x) f partials before the fixed-point accumulator is cleared.
+            const bool fixed = nb->accumulation != NBNXM_ACCUM_FLOAT;
+            NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, fixed ? nb->fFixed.ptr : nullptr, fixed ? nullptr : nb->fFloat.ptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
         }
-        if (fDouble && n3)
+        if (nb->accumulation == NBNXM_ACCUM_FLOAT)
         {
-            NBNXM_HIP_CHECK(hipMemcpyAsync(fDouble, nb->fOutDouble.ptr, sizeof(double) * n3, hipMemcpyDeviceToHost, nb->stream));
+            if (f && n3)
+            {
+                NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fFloat.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+            }
+            if (fDouble)
+            {
+                return fail(NBNXM_ERR_UNSUPPORTED, "gpu_launch_cpyback: double forces need fixed-point accumulation");
+            }
+        }
+        else
+        {
+            NBNXM_HIP_CHECK(launchConvertForces(nb->fFixed.ptr, nb->numAtoms, nb->fOut.ptr, nb->fOutDouble.ptr, nb->stream));
+            if (f && n3)
+            {
+                NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fOut.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+            }
+            if (fDouble && n3)
+            {
+                NBNXM_HIP_CHECK(hipMemcpyAsync(fDouble, nb->fOutDouble.ptr, sizeof(double) * n3, hipMemcpyDeviceToHost, nb->stream));
+            }
         }
     }
     if (nb->lastComputeEnergy || nb->lastComputeVirial)
@@ -818,6 +837,89 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         NBNXM_HIP_CHECK(launchVirialFinish(nb->virialPartials.ptr, 1, nb->virialBlocks, nb->shiftVec.ptr, nb->fshift.ptr, nb->virial.ptr, nb->stream));
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hVirial, nb->virial.ptr, 9 * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
     }
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_init_x_to_nbat_x(nbnxm_gpu_t* nb, int numAtoms, int numGridAtoms, const int* atomIndices)
+{
+    if (!nb || numAtoms < 0 || (numGridAtoms > 0 && !atomIndices))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init_x_to_nbat_x: null pointer or negative size");
+    }
+    if (numGridAtoms != nb->numAtoms)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_init_x_to_nbat_x: numGridAtoms differs from the uploaded atom data");
+    }
+    std::vector<int> cells(static_cast<size_t>(numAtoms), -1);
+    for (int slot = 0; slot < numGridAtoms; slot++)
+    {
+        const int a = atomIndices[slot];
+        if (a < -1 || a >= numAtoms || (a >= 0 && cells[a] >= 0))
+        {
+            return fail(NBNXM_ERR_INVALID, "gpu_init_x_to_nbat_x: atomIndices is not a map of slots onto distinct atoms");
+        }
+        if (a >= 0)
+        {
+            cells[a] = slot;
+        }
+    }
+    for (int a = 0; a < numAtoms; a++)
+    {
+        if (cells[a] < 0)
+        {
+            return fail(NBNXM_ERR_INVALID, "gpu_init_x_to_nbat_x: an atom has no grid slot");
+        }
+    }
+    NBNXM_HIP_CHECK(nb->atomIndex.resize(std::max(numGridAtoms, 1)));
+    NBNXM_HIP_CHECK(nb->cellIndex.resize(std::max(numAtoms, 1)));
+    if (numGridAtoms > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->atomIndex.ptr, atomIndices, sizeof(int) * numGridAtoms, hipMemcpyHostToDevice, nb->stream));
+    }
+    if (numAtoms > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->cellIndex.ptr, cells.data(), sizeof(int) * numAtoms, hipMemcpyHostToDevice, nb->stream));
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    nb->numRealAtoms = numAtoms;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_x_to_nbat_x(nbnxm_gpu_t* nb, const float* d_x)
+{
+    if (!nb || (!d_x && nb->numRealAtoms > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_x_to_nbat_x: null pointer");
+    }
+    if (nb->numRealAtoms < 0)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_x_to_nbat_x: call gpu_init_x_to_nbat_x first");
+    }
+    NBNXM_HIP_CHECK(launchXToXq(d_x, nb->atomIndex.ptr, nb->numAtoms, nb->xq.ptr, nb->stream));
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_add_nbat_f_to_f(nbnxm_gpu_t* nb, float* d_f, const float* d_fToAdd, int accumulate)
+{
+    if (!nb || (!d_f && nb->numRealAtoms > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_add_nbat_f_to_f: null pointer");
+    }
+    if (nb->numRealAtoms < 0)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_add_nbat_f_to_f: call gpu_init_x_to_nbat_x first");
+    }
+    if (!nb->haveLaunched || nb->forcesConsumed)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_add_nbat_f_to_f: no nonbonded launch whose forces are still pending");
+    }
+    const bool fixed = nb->accumulation != NBNXM_ACCUM_FLOAT;
+    if (nb->lastComputeVirial)
+    {
+        NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, fixed ? nb->fFixed.ptr : nullptr, fixed ? nullptr : nb->fFloat.ptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
+    }
+    NBNXM_HIP_CHECK(launchReduceForces(fixed ? nb->fFixed.ptr : nullptr, nb->fFloat.ptr, nb->cellIndex.ptr, nb->numRealAtoms, d_fToAdd, d_f, accumulate != 0, nb->stream));
+    nb->forcesConsumed = true;
     return NBNXM_OK;
 }
 

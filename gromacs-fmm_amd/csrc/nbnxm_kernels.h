@@ -108,6 +108,13 @@ hipError_t launchVirialPartials(const float4* xq,
                                 hipStream_t stream);
 hipError_t launchVirialFinish(const double* partials, int numSystems, int blocksPerSystem, const float4* shiftVec, const double* fshift, double* virial, hipStream_t stream);
 
+// Buffer ops around the kernel.  xq.xyz of real slots from coordinates in atom
+// order (3 floats per atom; atomIndices: slot -> atom or -1), w kept.
+hipError_t launchXToXq(const float* x, const int* atomIndices, int numSlots, float4* xq, hipStream_t stream);
+// f[3a..] (+)= nonbonded force of slot cells[a] (+ fToAdd[3a..] if non-null);
+// fixed-point slots (fFixed non-null) are converted and cleared, else fFloat is read.
+hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, const int* cells, int numAtoms, const float* fToAdd, float* f, bool accumulate, hipStream_t stream);
+
 // xq = x0 + N(0, sigma) for real atoms (q and fillers kept) over slots
 // [first, first+count); the step number is read from *step on the device so a
 // captured graph can be replayed; Philox4x32-10 counter-based generator keyed

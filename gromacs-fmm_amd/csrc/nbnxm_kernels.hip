@@ -1128,6 +1128,63 @@ __global__ __launch_bounds__(64) void virialFinishKernel(const double* partials,
     virial[9 * sys + c] = -0.5 * (sf + xf);
 }
 
+// Coordinate buffer op: xq.xyz of every real grid slot from the coordinates
+// in atom order (nbnxn_gpu_x_to_nbat_x_kernel,
+// src/gromacs/nbnxm/cuda/nbnxm_gpu_buffer_ops_internal.cu:66-105); the charge
+// (w) and the filler slots are left as they are.
+__global__ void xToXqKernel(const float* x, const int* atomIndices, int numSlots, float4* xq)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= numSlots)
+    {
+        return;
+    }
+    const int atom = atomIndices[slot];
+    if (atom < 0)
+    {
+        return;
+    }
+    const float w = xq[slot].w;
+    xq[slot] = make_float4(x[3 * static_cast<size_t>(atom)], x[3 * static_cast<size_t>(atom) + 1], x[3 * static_cast<size_t>(atom) + 2], w);
+}
+
+// Force reduction into atom order (reduceKernel,
+// src/gromacs/mdlib/gpuforcereduction_impl_internal.cu:62-120): f[a] is set
+// to (or, with accumulate, incremented by) the nonbonded force of slot
+// cell[a], then fToAdd[a] is added when given.  Fixed-point forces are
+// converted exactly and their accumulator slots cleared for the next launch.
+template<bool fixedPoint, bool accumulate, bool addRvec>
+__global__ void reduceForcesKernel(unsigned long long* fFixed, const float* fFloat, const int* cells, int numAtoms, const float* fToAdd, float* f)
+{
+    const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+    if (atom >= numAtoms)
+    {
+        return;
+    }
+    const size_t slot = static_cast<size_t>(cells[atom]);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+    {
+        float v;
+        if constexpr (fixedPoint)
+        {
+            v = static_cast<float>(static_cast<double>(static_cast<long long>(fFixed[3 * slot + d])) * (1.0 / c_fixedScaleD));
+            fFixed[3 * slot + d] = 0ull;
+        }
+        else
+        {
+            v = fFloat[3 * slot + d];
+        }
+        const size_t k = 3 * static_cast<size_t>(atom) + d;
+        float out = accumulate ? f[k] + v : v;
+        if constexpr (addRvec)
+        {
+            out += fToAdd[k];
+        }
+        f[k] = out;
+    }
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used as a counter-based
 // generator: key = seed, counter = (atom, step).
 __device__ __forceinline__ uint4 philox4x32_10(uint4 ctr, uint2 key)
@@ -1344,6 +1401,51 @@ hipError_t launchVirialFinish(const double* partials, int numSystems, int blocks
         return hipSuccess;
     }
     hipLaunchKernelGGL(virialFinishKernel, dim3(numSystems), dim3(64), 0, stream, partials, blocksPerSystem, shiftVec, fshift, virial);
+    return hipGetLastError();
+}
+
+hipError_t launchXToXq(const float* x, const int* atomIndices, int numSlots, float4* xq, hipStream_t stream)
+{
+    if (numSlots <= 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(xToXqKernel, dim3((numSlots + 255) / 256), dim3(256), 0, stream, x, atomIndices, numSlots, xq);
+    return hipGetLastError();
+}
+
+hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, const int* cells, int numAtoms, const float* fToAdd, float* f, bool accumulate, hipStream_t stream)
+{
+    if (numAtoms <= 0)
+    {
+        return hipSuccess;
+    }
+    const dim3 grid((numAtoms + 255) / 256), block(256);
+    const bool fixedPoint = fFixed != nullptr;
+    const bool addRvec = fToAdd != nullptr;
+    auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, stream, fFixed, fFloat, cells, numAtoms, fToAdd, f); };
+    if (fixedPoint)
+    {
+        if (accumulate)
+        {
+            addRvec ? launch(reduceForcesKernel<true, true, true>) : launch(reduceForcesKernel<true, true, false>);
+        }
+        else
+        {
+            addRvec ? launch(reduceForcesKernel<true, false, true>) : launch(reduceForcesKernel<true, false, false>);
+        }
+    }
+    else
+    {
+        if (accumulate)
+        {
+            addRvec ? launch(reduceForcesKernel<false, true, true>) : launch(reduceForcesKernel<false, true, false>);
+        }
+        else
+        {
+            addRvec ? launch(reduceForcesKernel<false, false, true>) : launch(reduceForcesKernel<false, false, false>);
+        }
+    }
     return hipGetLastError();
 }
 

@@ -112,6 +112,9 @@ SIGNATURES = [
     ("nbnxm_gpu_launch_cpyback", _c_int, [_vp, _vp, _vp]),
     ("nbnxm_gpu_wait_finish_task", _c_int, [_vp, _vp, _vp, _vp]),
     ("nbnxm_gpu_get_virial", _c_int, [_vp, _vp]),
+    ("nbnxm_gpu_init_x_to_nbat_x", _c_int, [_vp, _c_int, _c_int, _vp]),
+    ("nbnxm_gpu_x_to_nbat_x", _c_int, [_vp, _vp]),
+    ("nbnxm_gpu_add_nbat_f_to_f", _c_int, [_vp, _vp, _vp, _c_int]),
     ("nbnxm_gpu_min_ci_balanced", _c_int, [_vp]),
     ("nbnxm_gpu_is_kernel_ewald_analytical", _c_int, [_vp]),
     ("nbnxm_gpu_get_masks", _c_int, [_vp, _vp, _vp]),

@@ -267,6 +267,19 @@ class GpuTask:
         check(self._lib.nbnxm_gpu_wait_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift)), "gpu_wait_finish_task")
         return f, fd, elj.value, eel.value, fshift
 
+    # GPU buffer ops: device pointers (ints) to float32 arrays in atom order.
+    def init_x_to_nbat_x(self, num_atoms, atom_indices):
+        ai = np.ascontiguousarray(atom_indices, dtype=np.int32)
+        check(self._lib.nbnxm_gpu_init_x_to_nbat_x(self._h, int(num_atoms), ai.size, ptr(ai)), "gpu_init_x_to_nbat_x")
+
+    def x_to_nbat_x(self, d_x):
+        check(self._lib.nbnxm_gpu_x_to_nbat_x(self._h, ctypes.c_void_p(d_x)), "gpu_x_to_nbat_x")
+
+    def add_nbat_f_to_f(self, d_f, d_f_to_add=None, accumulate=False):
+        check(self._lib.nbnxm_gpu_add_nbat_f_to_f(self._h, ctypes.c_void_p(d_f),
+                                                  None if d_f_to_add is None else ctypes.c_void_p(d_f_to_add),
+                                                  int(accumulate)), "gpu_add_nbat_f_to_f")
+
     def virial(self):
         """Single-sum virial (3, 3) of the last launch made with compute_virial."""
         v = np.zeros(9, dtype=np.float64)

@@ -232,6 +232,22 @@ int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, doub
  * api/nblib/virials.cpp:55-84 uses it): here the sum over atoms and shifts is
  * reduced on the GPU in FP64 in a fixed order. */
 int nbnxm_gpu_get_virial(nbnxm_gpu_t* nb, double* virial);
+/* GPU buffer ops, for callers whose coordinates and forces live on the device.
+ * nbnxn_gpu_init_x_to_nbat_x (src/gromacs/nbnxm/nbnxm_gpu_data_mgmt.cpp:1076):
+ * uploads the grid's slot -> atom map (numGridAtoms entries, -1 for fillers;
+ * nbnxm_search_atom_indices gives it) and derives the atom -> slot map. */
+int nbnxm_gpu_init_x_to_nbat_x(nbnxm_gpu_t* nb, int numAtoms, int numGridAtoms, const int* atomIndices);
+/* nbnxn_gpu_x_to_nbat_x (src/gromacs/nbnxm/nbnxm_gpu_buffer_ops.cpp:63,
+ * kernel cuda/nbnxm_gpu_buffer_ops_internal.cu:66-105): d_x is a DEVICE array
+ * of 3*numAtoms floats in atom order; writes x, y, z of every real slot of xq
+ * and keeps the charges.  Asynchronous on the task's stream. */
+int nbnxm_gpu_x_to_nbat_x(nbnxm_gpu_t* nb, const float* d_x);
+/* GPU force reduction (reduceKernel, src/gromacs/mdlib/gpuforcereduction_impl_internal.cu:62-120):
+ * d_f, a DEVICE array of 3*numAtoms floats in atom order, is set to (accumulate
+ * = 0) or incremented by the nonbonded forces of the last launch, then d_fToAdd
+ * (DEVICE, may be NULL) is added.  Takes the forces of that launch: a following
+ * gpu_launch_cpyback then only reduces energies, shift forces and virial. */
+int nbnxm_gpu_add_nbat_f_to_f(nbnxm_gpu_t* nb, float* d_f, const float* d_fToAdd, int accumulate);
 /* gpu_min_ci_balanced (gpu_data_mgmt.h:119). */
 int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb);
 /* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): always 1 here. */
