@@ -7,7 +7,7 @@ lib/libnbnxm_amd.so (include/nbnxm_amd.h).  Import with the package directory
 from . import _lib
 from ._lib import (ACCUM_FIXED64, ACCUM_FLOAT, ACCUM_FP64, CENTRAL_SHIFT, NUM_SHIFTS, VDW_CUT_COMB_GEOM,
                    VDW_CUT_COMB_LB, VDW_CUT_TYPE_TABLE, NbnxmError, load)
-from .core import (AtomData, GpuTask, PairSearch, Pairlist, build, comb_params, ewald_coeff,
+from .core import (AtomData, GpuPairSearch, GpuTask, PairSearch, Pairlist, build, comb_params, ewald_coeff,
                    interaction_const, nbfp_table)
 from . import systems
 from .ensemble import MODE_BATCHED, MODE_PLAIN, Ensemble
@@ -16,7 +16,7 @@ from .calculator import ForceCalculator, NBKernelOptions, detect_comb_rule
 __all__ = [
     "ACCUM_FIXED64", "ACCUM_FLOAT", "ACCUM_FP64", "CENTRAL_SHIFT", "NUM_SHIFTS",
     "VDW_CUT_COMB_GEOM", "VDW_CUT_COMB_LB", "VDW_CUT_TYPE_TABLE", "NbnxmError", "load",
-    "AtomData", "GpuTask", "PairSearch", "Pairlist", "build", "comb_params", "ewald_coeff",
+    "AtomData", "GpuPairSearch", "GpuTask", "PairSearch", "Pairlist", "build", "comb_params", "ewald_coeff",
     "interaction_const", "nbfp_table", "systems", "_lib", "Ensemble", "MODE_BATCHED", "MODE_PLAIN",
     "ForceCalculator", "NBKernelOptions", "detect_comb_rule",
 ]

@@ -140,6 +140,11 @@ SIGNATURES = [
     ("nbnxm_force_calculator_update_pairlist", _c_int, [_vp, _c_int, _vp, _vp]),
     ("nbnxm_force_calculator_compute", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("nbnxm_force_calculator_free", None, [_vp]),
+    ("nbnxm_gpu_search_create", _c_int, [_PTR(_vp)]),
+    ("nbnxm_gpu_search_build", _c_int, [_vp, _c_int, _vp, _c_double, _c_int]),
+    ("nbnxm_gpu_search_num_entries", _c_int, [_vp, _vp]),
+    ("nbnxm_gpu_search_system_list", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_search_free", None, [_vp]),
 ]
 
 COULOMB_PME = 0

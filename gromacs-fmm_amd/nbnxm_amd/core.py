@@ -299,3 +299,43 @@ class GpuTask:
         count = np.zeros(n, dtype=np.int32)
         check(self._lib.nbnxm_gpu_get_sci_order(self._h, ptr(order), ptr(count)), "gpu_get_sci_order")
         return order, count
+
+
+class GpuPairSearch:
+    """Grids and GPU pair lists of one or many systems built on the device
+    (nbnxm_gpu_search_*); each system's list equals PairSearch's."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.nbnxm_gpu_search_create(ctypes.byref(h)), "gpu_search_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.nbnxm_gpu_search_free(self._h)
+            self._h = None
+
+    def build(self, systems, rlist, max_pairs_per_entry=0):
+        from .ensemble import system_descs
+        descs, keep = system_descs(systems)
+        check(self._lib.nbnxm_gpu_search_build(self._h, len(systems), ctypes.cast(descs, ctypes.c_void_p), float(rlist),
+                                               int(max_pairs_per_entry)), "gpu_search_build")
+        self.rlist = float(rlist)
+        del keep
+
+    def num_entries(self):
+        c = np.zeros(4, dtype=np.int64)
+        check(self._lib.nbnxm_gpu_search_num_entries(self._h, ptr(c)), "gpu_search_num_entries")
+        return dict(zip(["grid_slots", "entries", "groups", "excl"], (int(v) for v in c)))
+
+    def system_list(self, k):
+        """(Pairlist, slot -> atom map) of system k in single-system numbering."""
+        sizes = np.zeros(4, dtype=np.int32)
+        check(self._lib.nbnxm_gpu_search_system_list(self._h, int(k), None, None, None, None, ptr(sizes)), "gpu_search_system_list")
+        sci = np.zeros(max(int(sizes[1]), 1), dtype=SCI_DTYPE)
+        cj = np.zeros(max(int(sizes[2]), 1), dtype=CJ_DTYPE)
+        ex = np.zeros(int(sizes[3]), dtype=EXCL_DTYPE)
+        ai = np.zeros(int(sizes[0]), dtype=np.int32)
+        check(self._lib.nbnxm_gpu_search_system_list(self._h, int(k), ptr(sci), ptr(cj), ptr(ex), ptr(ai), ptr(sizes)), "gpu_search_system_list")
+        return Pairlist(sci[:sizes[1]], cj[:sizes[2]], ex, self.rlist), ai

@@ -10,6 +10,28 @@ from ._lib import (ACCUM_FIXED64, MODE_BATCHED, MODE_PLAIN, NUM_SHIFTS, VDW_CUT_
 __all__ = ["Ensemble", "MODE_BATCHED", "MODE_PLAIN"]
 
 
+def system_descs(systems):
+    """nbnxm_system_desc_t array for the systems, plus the arrays it points to."""
+    keep = []
+    descs = (SystemDesc * len(systems))()
+    for k, s in enumerate(systems):
+        x = np.ascontiguousarray(s.x, dtype=np.float64)
+        q = np.ascontiguousarray(s.q, dtype=np.float64)
+        t = np.ascontiguousarray(s.types, dtype=np.int32)
+        r = np.ascontiguousarray(s.excl_ranges, dtype=np.int32)
+        e = np.ascontiguousarray(s.excl_elements, dtype=np.int32)
+        keep += [x, q, t, r, e]
+        descs[k].numAtoms = s.n
+        descs[k].x = ptr(x)
+        descs[k].q = ptr(q)
+        descs[k].type = ptr(t)
+        descs[k].exclRanges = ptr(r)
+        descs[k].exclElements = ptr(e) if e.size else ptr(np.zeros(1, dtype=np.int32))
+        for d in range(3):
+            descs[k].box[d] = float(s.box[d])
+    return descs, keep
+
+
 class Ensemble:
     """All systems must share one (C6, C12) type table."""
 
@@ -22,23 +44,7 @@ class Ensemble:
         for s in systems[1:]:
             if s.c6c12.shape != c6c12.shape or not np.array_equal(s.c6c12, c6c12):
                 raise ValueError("all systems of an ensemble share the type table")
-        self._keep = []
-        descs = (SystemDesc * len(systems))()
-        for k, s in enumerate(systems):
-            x = np.ascontiguousarray(s.x, dtype=np.float64)
-            q = np.ascontiguousarray(s.q, dtype=np.float64)
-            t = np.ascontiguousarray(s.types, dtype=np.int32)
-            r = np.ascontiguousarray(s.excl_ranges, dtype=np.int32)
-            e = np.ascontiguousarray(s.excl_elements, dtype=np.int32)
-            self._keep += [x, q, t, r, e]
-            descs[k].numAtoms = s.n
-            descs[k].x = ptr(x)
-            descs[k].q = ptr(q)
-            descs[k].type = ptr(t)
-            descs[k].exclRanges = ptr(r)
-            descs[k].exclElements = ptr(e) if e.size else ptr(np.zeros(1, dtype=np.int32))
-            for d in range(3):
-                descs[k].box[d] = float(s.box[d])
+        descs, self._keep = system_descs(systems)
         tab = np.ascontiguousarray(c6c12.reshape(-1), dtype=np.float64)
         self._keep.append(tab)
         opt = EnsembleOptions()

@@ -427,6 +427,31 @@ int nbnxm_force_calculator_compute(nbnxm_force_calculator_t* c,
                                    double* energies);
 void nbnxm_force_calculator_free(nbnxm_force_calculator_t* c);
 
+/* ---------------------------------------------------------------------------
+ * GPU pair search: the grid and the GPU cluster-pair list built on the device,
+ * for one or many systems in one pass.  Replaces the host-side
+ * putAtomsOnGrid / constructPairlist of the GPU list (src/gromacs/nbnxm/grid.cpp:1101-1240,
+ * src/gromacs/nbnxm/pairlist.cpp:914-939,1122-1270,1948-2046,2150-2262) and
+ * produces, for every system, exactly the list nbnxm_search_* builds.
+ * ------------------------------------------------------------------------- */
+typedef struct nbnxm_gpu_search nbnxm_gpu_search_t;
+int nbnxm_gpu_search_create(nbnxm_gpu_search_t** out);
+/* Grids and lists of numSystems systems (coordinates, boxes and exclusions as
+ * in nbnxm_ensemble_create; q and type are not used).  Every box edge must
+ * exceed rlist; maxPairsPerEntry > 0 splits entries as nbnxm_search does.
+ * Fails with NBNXM_ERR_UNSUPPORTED when an entry exceeds 1024 j-clusters or
+ * needs more than 32 exclusion masks. */
+int nbnxm_gpu_search_build(nbnxm_gpu_search_t* s, int numSystems, const nbnxm_system_desc_t* systems, double rlist, int maxPairsPerEntry);
+/* Totals over all systems: [0] grid slots, [1] entries, [2] packed groups,
+ * [3] exclusion masks (including the shared all-ones mask 0). */
+int nbnxm_gpu_search_num_entries(const nbnxm_gpu_search_t* s, int64_t* counts);
+/* One system's list in single-system numbering (cells, groups and masks from
+ * 0, its own all-ones mask 0) and its slot -> atom map.  sizes[4] receives the
+ * grid slots, entries, groups and masks; any output pointer may be NULL to
+ * query the sizes first. */
+int nbnxm_gpu_search_system_list(const nbnxm_gpu_search_t* s, int system, nbnxm_sci_t* sci, nbnxm_cj_packed_t* cjPacked, nbnxm_excl_t* excl, int* atomIndices, int* sizes);
+void nbnxm_gpu_search_free(nbnxm_gpu_search_t* s);
+
 #ifdef __cplusplus
 }
 #endif
