@@ -14,6 +14,7 @@
 
 #include "capi_common.h"
 #include "nbnxm_kernels.h"
+#include "gpu_search.h"
 #include "pairsearch.h"
 
 using namespace nbnxm_amd;
@@ -525,10 +526,32 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     e->launchConfig.wavesPerBlock = (opt->wavesPerBlock >= 1 && opt->wavesPerBlock <= 4) ? opt->wavesPerBlock : 4;
     NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
 
-    // Grids and unsplit lists, in parallel over systems.
+    // Grids and unsplit lists: on the device in one pass, or on host threads.
     e->sys.resize(numSystems);
-    const int nthreads = std::max(1, std::min(numSystems, opt->numHostThreads > 0 ? opt->numHostThreads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()))));
+    if (opt->gpuSearch)
     {
+        nbnxm_gpu_search_t* gs = nullptr;
+        rc = nbnxm_gpu_search_create(&gs);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+        std::unique_ptr<nbnxm_gpu_search_t, void (*)(nbnxm_gpu_search_t*)> guard(gs, nbnxm_gpu_search_free);
+        rc = nbnxm_gpu_search_build(gs, numSystems, systems, opt->rlistOuter, 0);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+        for (int k = 0; k < numSystems; k++)
+        {
+            SystemHost& s = e->sys[k];
+            s.numAtoms = systems[k].numAtoms;
+            s.buildStatus = gpuSearchExportSystem(gs, k, s.grid);
+        }
+    }
+    else
+    {
+        const int nthreads = std::max(1, std::min(numSystems, opt->numHostThreads > 0 ? opt->numHostThreads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()))));
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; t++)
         {

@@ -27,7 +27,13 @@
 #include <vector>
 
 #include "capi_common.h"
+#include "gpu_search.h"
 #include "nbnxm_amd.h"
+#include "pairsearch.h"
+
+// No contraction of a multiply and an add into an FMA anywhere in this file:
+// the rounded operations must be the host builder's, one by one.
+#pragma clang fp contract(off)
 
 namespace nbnxm_amd
 {
@@ -383,7 +389,6 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
     __shared__ int sExclSlot[c_maxEntryGroups];
     __shared__ int sEventGroups[c_maxEntryExcl];
     __shared__ unsigned int sExcl[c_maxEntryExcl][NBNXM_EXCL_SIZE];
-    __shared__ int sLen;
     __shared__ int sNumEvents;
     __shared__ int sOverflow;
 
@@ -839,6 +844,7 @@ struct nbnxm_gpu_search
     std::vector<int> sysCellStart;   // numSystems + 1 (after the build)
     std::vector<int> sysSciStart, sysGroupStart, sysExclStart; // numSystems + 1
     int numAtoms = 0, numCols = 0, numCells = 0, nsci = 0, ngroups = 0, nexcl = 0;
+    float rlistF = 0.0f;
 
     DeviceBuffer<double> x;
     DeviceBuffer<SysGrid> grids;
@@ -862,6 +868,7 @@ struct nbnxm_gpu_search
     std::vector<nbnxm_cj_packed_t> hCj;
     std::vector<nbnxm_excl_t> hExcl;
     std::vector<int> hAtomIndex;
+    std::vector<float4> hXw;
 
     ~nbnxm_gpu_search()
     {
@@ -1221,7 +1228,10 @@ extern "C" int nbnxm_gpu_search_build(nbnxm_gpu_search_t* s, int numSystems, con
     }
     NBNXM_HIP_CHECK(hipMemcpyAsync(s->hExcl.data(), s->excl.ptr, sizeof(nbnxm_excl_t) * s->nexcl, hipMemcpyDeviceToHost, st));
     NBNXM_HIP_CHECK(hipMemcpyAsync(s->hAtomIndex.data(), s->atomIndex.ptr, sizeof(int) * s->hAtomIndex.size(), hipMemcpyDeviceToHost, st));
+    s->hXw.resize(N);
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hXw.data(), s->xw.ptr, sizeof(float4) * N, hipMemcpyDeviceToHost, st));
     NBNXM_HIP_CHECK(hipStreamSynchronize(st));
+    s->rlistF = static_cast<float>(rlist);
     return NBNXM_OK;
 }
 
@@ -1300,3 +1310,38 @@ extern "C" int nbnxm_gpu_search_system_list(const nbnxm_gpu_search_t* s, int sys
     }
     return NBNXM_OK;
 }
+
+namespace nbnxm_amd
+{
+
+int gpuSearchExportSystem(const nbnxm_gpu_search_t* s, int system, GridSearch& grid)
+{
+    int sizes[4];
+    int rc = nbnxm_gpu_search_system_list(s, system, nullptr, nullptr, nullptr, nullptr, sizes);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    std::vector<nbnxm_sci_t> sci(sizes[1]);
+    std::vector<nbnxm_cj_packed_t> cj(sizes[2]);
+    std::vector<nbnxm_excl_t> excl(sizes[3]);
+    std::vector<int> ai(sizes[0]);
+    rc = nbnxm_gpu_search_system_list(s, system, sci.data(), cj.data(), excl.data(), ai.data(), sizes);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    const SysGrid& g = s->hostGrids[system];
+    std::vector<float> xw(3 * static_cast<size_t>(g.numAtoms));
+    for (int i = 0; i < g.numAtoms; i++)
+    {
+        const float4 v = s->hXw[g.atomOffset + i];
+        xw[3 * i] = v.x;
+        xw[3 * i + 1] = v.y;
+        xw[3 * i + 2] = v.z;
+    }
+    grid.adopt(g.box, g.numAtoms, std::move(xw), sizes[0] / c_cellAtoms, std::move(ai), s->rlistF, std::move(sci), std::move(cj), std::move(excl));
+    return NBNXM_OK;
+}
+
+} // namespace nbnxm_amd

@@ -730,6 +730,55 @@ int64_t GridSearch::countPairsWithin(double rc) const
     return count;
 }
 
+void GridSearch::adopt(const double box[3],
+                       int numAtoms,
+                       std::vector<float>&& wrappedX,
+                       int numCells,
+                       std::vector<int>&& atomIndices,
+                       float rlist,
+                       std::vector<nbnxm_sci_t>&& sci,
+                       std::vector<nbnxm_cj_packed_t>&& cjPacked,
+                       std::vector<nbnxm_excl_t>&& excl)
+{
+    for (int d = 0; d < 3; d++)
+    {
+        box_[d] = box[d];
+    }
+    for (int tz = -1, k = 0; tz <= 1; tz++)
+    {
+        for (int ty = -1; ty <= 1; ty++)
+        {
+            for (int tx = -2; tx <= 2; tx++, k++)
+            {
+                shiftVec_[3 * k + 0] = static_cast<float>(tx) * static_cast<float>(box_[0]);
+                shiftVec_[3 * k + 1] = static_cast<float>(ty) * static_cast<float>(box_[1]);
+                shiftVec_[3 * k + 2] = static_cast<float>(tz) * static_cast<float>(box_[2]);
+            }
+        }
+    }
+    numAtoms_ = numAtoms;
+    xw_ = std::move(wrappedX);
+    numCells_ = numCells;
+    atomIndices_ = std::move(atomIndices);
+    cells_.assign(numAtoms, -1);
+    for (size_t slot = 0; slot < atomIndices_.size(); slot++)
+    {
+        if (atomIndices_[slot] >= 0)
+        {
+            cells_[atomIndices_[slot]] = static_cast<int>(slot);
+        }
+    }
+    rlist_ = rlist;
+    sci_ = std::move(sci);
+    cjPacked_ = std::move(cjPacked);
+    excl_ = std::move(excl);
+    numClusterPairs_ = 0;
+    for (const nbnxm_cj_packed_t& p : cjPacked_)
+    {
+        numClusterPairs_ += __builtin_popcount(p.imei[0].imask);
+    }
+}
+
 nbnxm_pairlist_gpu_t GridSearch::listView() const
 {
     nbnxm_pairlist_gpu_t v;

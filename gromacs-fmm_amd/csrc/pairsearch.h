@@ -73,6 +73,19 @@ public:
     // Non-excluded atom pairs within rc in the current list (FP64 distances on
     // the wrapped search coordinates); requires the list to cover rc.
     int64_t countPairsWithin(double rc) const;
+    // Takes over a grid and list built elsewhere (the device search): the box,
+    // wrapped FP32 coordinates in atom order, the slot -> atom map of numCells
+    // cells and the list; afterwards every accessor behaves as after
+    // putAtomsOnGrid + constructPairlist with the same inputs.
+    void adopt(const double box[3],
+               int numAtoms,
+               std::vector<float>&& wrappedX,
+               int numCells,
+               std::vector<int>&& atomIndices,
+               float rlist,
+               std::vector<nbnxm_sci_t>&& sci,
+               std::vector<nbnxm_cj_packed_t>&& cjPacked,
+               std::vector<nbnxm_excl_t>&& excl);
 
 private:
     int placeAtoms();

@@ -171,7 +171,7 @@ class EnsembleOptions(ctypes.Structure):
                 ("rcoulomb", ctypes.c_double), ("ewaldRtol", ctypes.c_double), ("epsilonR", ctypes.c_double),
                 ("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
                 ("accumulation", ctypes.c_int), ("maxPairsPerEntry", ctypes.c_int),
-                ("wavesPerBlock", ctypes.c_int), ("numHostThreads", ctypes.c_int)]
+                ("wavesPerBlock", ctypes.c_int), ("numHostThreads", ctypes.c_int), ("gpuSearch", ctypes.c_int)]
 
 
 MODE_BATCHED = 0

@@ -37,7 +37,7 @@ class Ensemble:
 
     def __init__(self, systems, rc=1.0, ewald_rtol=1e-5, rlist_outer=1.1, rlist_inner=1.04,
                  vdw_type=VDW_CUT_COMB_GEOM, accumulation=ACCUM_FIXED64, max_pairs_per_entry=64,
-                 waves_per_block=4, epsilon_r=1.0):
+                 waves_per_block=4, epsilon_r=1.0, gpu_search=False):
         self._lib = _lib.load()
         self.systems = systems
         c6c12 = systems[0].c6c12
@@ -60,6 +60,7 @@ class Ensemble:
         opt.maxPairsPerEntry = max_pairs_per_entry
         opt.wavesPerBlock = waves_per_block
         opt.numHostThreads = 0
+        opt.gpuSearch = 1 if gpu_search else 0
         h = ctypes.c_void_p()
         check(self._lib.nbnxm_ensemble_create(len(systems), ctypes.cast(descs, ctypes.c_void_p), ctypes.byref(opt), ctypes.byref(h)), "ensemble_create")
         self._h = h

@@ -297,6 +297,7 @@ typedef struct nbnxm_ensemble_options
     int maxPairsPerEntry;     /* initial entry split (cluster pairs), 0 = 64 */
     int wavesPerBlock;        /* 1..4, 0 = 4 */
     int numHostThreads;       /* list construction threads, 0 = hardware concurrency */
+    int gpuSearch;            /* nonzero: grids and lists built on the device (nbnxm_gpu_search_*) */
 } nbnxm_ensemble_options_t;
 
 #define NBNXM_MODE_BATCHED 0  /* one launch per kernel for all systems */

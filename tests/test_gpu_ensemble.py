@@ -129,3 +129,21 @@ def test_split_and_autotune_keep_forces(systems, ens):
     ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
     for k in range(len(systems)):
         assert np.array_equal(ens.forces(k), again[k])
+
+
+def test_gpu_search_ensemble_bitwise(systems):
+    """Lists built on the device give the same ensemble, bit for bit."""
+    announce("ensemble with device-built lists")
+    args = (systems, RC, RTOL, 1.1, 1.04, nb.VDW_CUT_TYPE_TABLE, nb.ACCUM_FIXED64)
+    host = nb.Ensemble(*args, max_pairs_per_entry=48)
+    dev = nb.Ensemble(*args, max_pairs_per_entry=48, gpu_search=True)
+    assert dev.stats() == host.stats()
+    assert dev.count_pairs()[0] == host.count_pairs()[0]
+    for e in (host, dev):
+        e.run(1, nb.MODE_BATCHED, 0.0, 5, True)
+        e.sync()
+    for k in range(len(systems)):
+        assert np.array_equal(dev.forces(k), host.forces(k))
+    e1, f1 = host.energies()
+    e2, f2 = dev.energies()
+    assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
