@@ -275,13 +275,151 @@ int nbnxm_init_interaction_const(double rc, double ewaldRtol, double epsilonR, n
     {
         return fail(NBNXM_ERR_INVALID, "init_interaction_const: need rc > 0 and 0 < rtol < 1");
     }
-    ic->rcoulomb = rc;
-    ic->rvdw = rc;
-    ic->ewaldcoeff_q = nbnxm_calc_ewaldcoeff_q(rc, ewaldRtol);
-    ic->sh_ewald = std::erfc(ic->ewaldcoeff_q * rc) / rc;
-    ic->epsfac = (epsilonR != 0) ? one4PiEps0() / epsilonR : 0.0;
-    ic->dispersion_shift_cpot = -1.0 / std::pow(rc, 6);
-    ic->repulsion_shift_cpot = -1.0 / std::pow(rc, 12);
+    nbnxm_ic_options_t o{};
+    o.eeltype = NBNXM_ELEC_EWALD;
+    o.vdwModifier = NBNXM_VDW_MOD_POTSHIFT;
+    o.rcoulomb = rc;
+    o.rvdw = rc;
+    o.ewaldRtol = ewaldRtol;
+    o.epsilonR = epsilonR;
+    return nbnxm_init_interaction_const_ex(&o, ic);
+}
+
+double nbnxm_calc_ewaldcoeff_lj(double rc, double rtol)
+{
+    // Root of exp(-x^2)(1 + x^2 + x^4/2) = rtol at x = beta*rc, found like
+    // calc_ewaldcoeff_lj (src/gromacs/ewald/ewald_utils.cpp:73-112): double
+    // beta from 10 until below rtol, then bisect with i + 60 halvings.
+    const auto ljFunction = [rc](double beta) {
+        const double x2 = (beta * rc) * (beta * rc);
+        return std::exp(-x2) * (1.0 + x2 + 0.5 * x2 * x2);
+    };
+    double beta = 5.0;
+    int i = 0;
+    do
+    {
+        i++;
+        beta *= 2.0;
+    } while (ljFunction(beta) > rtol);
+    const int n = i + 60;
+    double low = 0.0, high = beta;
+    for (int k = 0; k < n; k++)
+    {
+        beta = 0.5 * (low + high);
+        if (ljFunction(beta) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
+int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* o, nbnxm_interaction_const_t* ic)
+{
+    if (!o || !ic)
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: null pointer");
+    }
+    const double rc = o->rcoulomb;
+    const double rvdw = (o->rvdw > 0) ? o->rvdw : rc;
+    if (!(rc > 0) || !(rvdw <= rc))
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: need rcoulomb > 0 and 0 < rvdw <= rcoulomb");
+    }
+    if (o->eeltype != NBNXM_ELEC_EWALD && o->eeltype != NBNXM_ELEC_RF)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "init_interaction_const_ex: unknown Coulomb type");
+    }
+    if (o->vdwModifier < NBNXM_VDW_MOD_POTSHIFT || o->vdwModifier > NBNXM_VDW_MOD_POTSWITCH)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "init_interaction_const_ex: unknown LJ modifier");
+    }
+    if (o->vdwModifier != NBNXM_VDW_MOD_POTSHIFT && !(o->rvdw_switch >= 0 && o->rvdw_switch < rvdw))
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: switched LJ needs 0 <= rvdw_switch < rvdw");
+    }
+    // LJ-PME exists with the potential shift only (src/gromacs/nbnxm/simd_kernel.h:227-228).
+    if (o->ljPme && o->vdwModifier != NBNXM_VDW_MOD_POTSHIFT)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "init_interaction_const_ex: LJ-PME supports the potential-shift modifier only");
+    }
+    if (o->eeltype == NBNXM_ELEC_EWALD && !(o->ewaldRtol > 0 && o->ewaldRtol < 1))
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: Ewald needs 0 < ewaldRtol < 1");
+    }
+    if (o->ljPme && !(o->ewaldRtolLj > 0 && o->ewaldRtolLj < 1))
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: LJ-PME needs 0 < ewaldRtolLj < 1");
+    }
+    nbnxm_interaction_const_t c{};
+    c.rcoulomb = rc;
+    c.rvdw = rvdw;
+    c.eeltype = o->eeltype;
+    c.vdwModifier = o->vdwModifier;
+    c.ljPme = o->ljPme ? 1 : 0;
+    c.rvdw_switch = (o->vdwModifier == NBNXM_VDW_MOD_POTSHIFT) ? 0.0 : o->rvdw_switch;
+    c.epsfac = (o->epsilonR != 0) ? one4PiEps0() / o->epsilonR : 0.0;
+
+    // Lennard-Jones modifier constants (interaction_const.cpp:199-246, 284-305).
+    const double rsw = c.rvdw_switch;
+    const auto forceSwitch = [rsw, rvdw](double p, double* c2, double* c3, double* cpot) {
+        const double d = rvdw - rsw;
+        *c2 = ((p + 1) * rsw - (p + 4) * rvdw) / (std::pow(rvdw, p + 2) * d * d);
+        *c3 = -((p + 1) * rsw - (p + 3) * rvdw) / (std::pow(rvdw, p + 2) * d * d * d);
+        *cpot = -std::pow(rvdw, -p) + p * (*c2) / 3 * d * d * d + p * (*c3) / 4 * d * d * d * d;
+    };
+    switch (o->vdwModifier)
+    {
+        case NBNXM_VDW_MOD_POTSHIFT:
+            c.dispersion_shift_cpot = -1.0 / std::pow(rvdw, 6);
+            c.repulsion_shift_cpot = -1.0 / std::pow(rvdw, 12);
+            break;
+        case NBNXM_VDW_MOD_FORCESWITCH:
+            forceSwitch(6.0, &c.dispersion_shift_c2, &c.dispersion_shift_c3, &c.dispersion_shift_cpot);
+            forceSwitch(12.0, &c.repulsion_shift_c2, &c.repulsion_shift_c3, &c.repulsion_shift_cpot);
+            break;
+        default:
+        {
+            const double d = rvdw - rsw;
+            c.vdw_switch_c3 = -10.0 / (d * d * d);
+            c.vdw_switch_c4 = 15.0 / (d * d * d * d);
+            c.vdw_switch_c5 = -6.0 / (d * d * d * d * d);
+            break;
+        }
+    }
+    if (c.ljPme)
+    {
+        // initVdwEwaldParameters (interaction_const.cpp:122-148).
+        c.ewaldcoeff_lj = nbnxm_calc_ewaldcoeff_lj(rvdw, o->ewaldRtolLj);
+        const double crc2 = (c.ewaldcoeff_lj * rvdw) * (c.ewaldcoeff_lj * rvdw);
+        c.sh_lj_ewald = (std::exp(-crc2) * (1 + crc2 + 0.5 * crc2 * crc2) - 1) / std::pow(rvdw, 6);
+    }
+
+    // Coulomb: Ewald with the potential shift (interaction_const.cpp:64-120), or
+    // reaction field / plain cut-off (:320-345 with calc_rffac, rf_util.cpp:50-62).
+    if (o->eeltype == NBNXM_ELEC_EWALD)
+    {
+        c.ewaldcoeff_q = nbnxm_calc_ewaldcoeff_q(rc, o->ewaldRtol);
+        c.sh_ewald = std::erfc(c.ewaldcoeff_q * rc) / rc;
+        c.k_rf = 0.0;
+        c.c_rf = 1.0 / rc;
+    }
+    else if (o->plainCutoff)
+    {
+        c.k_rf = 0.0;
+        c.c_rf = 1.0 / rc;
+    }
+    else
+    {
+        c.k_rf = (o->epsilonRf == 0) ? 1.0 / (2 * rc * rc * rc)
+                                      : (o->epsilonRf - o->epsilonR) / (2 * o->epsilonRf + o->epsilonR) / (rc * rc * rc);
+        c.c_rf = 1.0 / rc + c.k_rf * rc * rc;
+    }
+    *ic = c;
     return NBNXM_OK;
 }
 

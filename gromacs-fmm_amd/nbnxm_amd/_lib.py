@@ -49,11 +49,35 @@ class PairlistGpu(ctypes.Structure):
                 ("nexcl", ctypes.c_int), ("excl", ctypes.c_void_p)]
 
 
+IC_FIELDS = [("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double),
+             ("ewaldcoeff_q", ctypes.c_double), ("sh_ewald", ctypes.c_double),
+             ("epsfac", ctypes.c_double), ("dispersion_shift_cpot", ctypes.c_double),
+             ("repulsion_shift_cpot", ctypes.c_double),
+             ("eeltype", ctypes.c_int), ("vdwModifier", ctypes.c_int), ("ljPme", ctypes.c_int),
+             ("rvdw_switch", ctypes.c_double), ("ewaldcoeff_lj", ctypes.c_double),
+             ("sh_lj_ewald", ctypes.c_double),
+             ("dispersion_shift_c2", ctypes.c_double), ("dispersion_shift_c3", ctypes.c_double),
+             ("repulsion_shift_c2", ctypes.c_double), ("repulsion_shift_c3", ctypes.c_double),
+             ("vdw_switch_c3", ctypes.c_double), ("vdw_switch_c4", ctypes.c_double),
+             ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double)]
+
+
 class InteractionConst(ctypes.Structure):
-    _fields_ = [("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double),
-                ("ewaldcoeff_q", ctypes.c_double), ("sh_ewald", ctypes.c_double),
-                ("epsfac", ctypes.c_double), ("dispersion_shift_cpot", ctypes.c_double),
-                ("repulsion_shift_cpot", ctypes.c_double)]
+    _fields_ = IC_FIELDS
+
+
+ELEC_EWALD = 0
+ELEC_RF = 1
+VDW_MOD_POTSHIFT = 0
+VDW_MOD_FORCESWITCH = 1
+VDW_MOD_POTSWITCH = 2
+
+
+class IcOptions(ctypes.Structure):
+    _fields_ = [("eeltype", ctypes.c_int), ("vdwModifier", ctypes.c_int), ("ljPme", ctypes.c_int),
+                ("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double), ("rvdw_switch", ctypes.c_double),
+                ("ewaldRtol", ctypes.c_double), ("ewaldRtolLj", ctypes.c_double),
+                ("epsilonR", ctypes.c_double), ("epsilonRf", ctypes.c_double), ("plainCutoff", ctypes.c_int)]
 
 
 class PairlistParams(ctypes.Structure):
@@ -89,6 +113,8 @@ SIGNATURES = [
     ("nbnxm_version", ctypes.c_char_p, []),
     ("nbnxm_calc_ewaldcoeff_q", _c_double, [_c_double, _c_double]),
     ("nbnxm_init_interaction_const", _c_int, [_c_double, _c_double, _c_double, _PTR(InteractionConst)]),
+    ("nbnxm_init_interaction_const_ex", _c_int, [_PTR(IcOptions), _PTR(InteractionConst)]),
+    ("nbnxm_calc_ewaldcoeff_lj", _c_double, [_c_double, _c_double]),
     ("nbnxm_calc_shifts", _c_int, [_vp, _vp]),
     ("nbnxm_search_create", _vp, []),
     ("nbnxm_search_free", None, [_vp]),

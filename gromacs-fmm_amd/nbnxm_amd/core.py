@@ -23,8 +23,42 @@ def interaction_const(rc=1.0, ewald_rtol=1e-5, epsilon_r=1.0):
     return ic
 
 
+def interaction_const_ex(rcoulomb, coulomb="ewald", vdw_modifier="potshift", lj_pme=False, rvdw=0.0,
+                         rvdw_switch=0.0, ewald_rtol=1e-5, ewald_rtol_lj=1e-3, epsilon_r=1.0, epsilon_rf=0.0,
+                         plain_cutoff=False):
+    """Any kernel flavour: coulomb "ewald" | "rf"; vdw_modifier "potshift" |
+    "forceswitch" | "potswitch"; lj_pme adds the geometric LJ-PME grid
+    correction; rvdw < rcoulomb gives the twin cut-off."""
+    o = _lib.IcOptions()
+    o.eeltype = {"ewald": _lib.ELEC_EWALD, "rf": _lib.ELEC_RF}[coulomb]
+    o.vdwModifier = {"potshift": _lib.VDW_MOD_POTSHIFT, "forceswitch": _lib.VDW_MOD_FORCESWITCH,
+                     "potswitch": _lib.VDW_MOD_POTSWITCH}[vdw_modifier]
+    o.ljPme = int(bool(lj_pme))
+    o.rcoulomb = rcoulomb
+    o.rvdw = rvdw
+    o.rvdw_switch = rvdw_switch
+    o.ewaldRtol = ewald_rtol
+    o.ewaldRtolLj = ewald_rtol_lj
+    o.epsilonR = epsilon_r
+    o.epsilonRf = epsilon_rf
+    o.plainCutoff = int(bool(plain_cutoff))
+    ic = InteractionConst()
+    check(_lib.load().nbnxm_init_interaction_const_ex(ctypes.byref(o), ctypes.byref(ic)), "init_interaction_const_ex")
+    return ic
+
+
+def is_default_flavour(ic):
+    """Ewald + LJ cut-off with potential shift and rvdw == rcoulomb."""
+    return (ic.eeltype == _lib.ELEC_EWALD and ic.vdwModifier == _lib.VDW_MOD_POTSHIFT and not ic.ljPme
+            and ic.rvdw == ic.rcoulomb)
+
+
 def ewald_coeff(rc, rtol):
     return _lib.load().nbnxm_calc_ewaldcoeff_q(rc, rtol)
+
+
+def ewald_coeff_lj(rc, rtol):
+    return _lib.load().nbnxm_calc_ewaldcoeff_lj(rc, rtol)
 
 
 class Pairlist:

@@ -106,18 +106,61 @@ enum nbnxm_accumulation
     NBNXM_ACCUM_FP64 = 2     /* FP64 pair math with exact erfc/exp (GPU oracle mode) */
 };
 
+/* Coulomb flavour (CoulombInteractionType of the kernel). */
+enum nbnxm_elec_type
+{
+    NBNXM_ELEC_EWALD = 0, /* Ewald real space, analytical correction (PME) */
+    NBNXM_ELEC_RF = 1     /* reaction field; with k_rf = 0 the plain cut-off */
+};
+
+/* LJ interaction modifier (InteractionModifiers of the kernel). */
+enum nbnxm_vdw_modifier
+{
+    NBNXM_VDW_MOD_POTSHIFT = 0,
+    NBNXM_VDW_MOD_FORCESWITCH = 1,
+    NBNXM_VDW_MOD_POTSWITCH = 2
+};
+
 /* Subset of interaction_const_t used by the kernel
  * (src/gromacs/mdtypes/interaction_const.cpp:106-115,277-278,309). */
 typedef struct nbnxm_interaction_const
 {
     double rcoulomb;
-    double rvdw;              /* must equal rcoulomb: no twin cut-off in this kernel */
+    double rvdw;              /* <= rcoulomb; smaller gives the twin cut-off (LJ cut-off check) */
     double ewaldcoeff_q;      /* beta, from nbnxm_calc_ewaldcoeff_q */
     double sh_ewald;          /* erfc(beta*rc)/rc with PotShift, else 0 */
     double epsfac;            /* c_one4PiEps0 / epsilon_r */
-    double dispersion_shift_cpot; /* -rvdw^-6 with PotShift */
-    double repulsion_shift_cpot;  /* -rvdw^-12 with PotShift */
+    double dispersion_shift_cpot; /* -rvdw^-6 with PotShift; the force-switch constant with ForceSwitch */
+    double repulsion_shift_cpot;  /* -rvdw^-12 with PotShift; the force-switch constant with ForceSwitch */
+    /* Kernel flavour; nbnxm_init_interaction_const sets Ewald, PotShift, no LJ-PME. */
+    int eeltype;              /* enum nbnxm_elec_type */
+    int vdwModifier;          /* enum nbnxm_vdw_modifier */
+    int ljPme;                /* nonzero: LJ-PME grid correction, geometric grid combination */
+    double rvdw_switch;
+    double ewaldcoeff_lj;     /* calc_ewaldcoeff_lj, src/gromacs/ewald/ewald_utils.cpp:84-112 */
+    double sh_lj_ewald;       /* LJ-PME potential shift (interaction_const.cpp:140-143) */
+    double dispersion_shift_c2, dispersion_shift_c3; /* force_switch_constants, interaction_const.cpp:214-230 */
+    double repulsion_shift_c2, repulsion_shift_c3;
+    double vdw_switch_c3, vdw_switch_c4, vdw_switch_c5; /* potential_switch_constants, :232-246 */
+    double k_rf;              /* reaction-field coefficient (calc_rffac, src/gromacs/mdlib/rf_util.cpp:50-62) */
+    double c_rf;              /* reaction-field potential shift */
 } nbnxm_interaction_const_t;
+
+/* Flavour choices for nbnxm_init_interaction_const_ex (t_inputrec fields). */
+typedef struct nbnxm_ic_options
+{
+    int eeltype;        /* enum nbnxm_elec_type */
+    int vdwModifier;    /* enum nbnxm_vdw_modifier */
+    int ljPme;          /* LJ-PME with the geometric grid rule */
+    double rcoulomb;
+    double rvdw;        /* 0 -> rcoulomb */
+    double rvdw_switch; /* ForceSwitch / PotSwitch */
+    double ewaldRtol;   /* Coulomb Ewald tolerance */
+    double ewaldRtolLj; /* LJ-PME tolerance */
+    double epsilonR;
+    double epsilonRf;   /* reaction field: 0 = infinity */
+    int plainCutoff;    /* with NBNXM_ELEC_RF: plain cut-off (k_rf = 0, c_rf = 1/rc) */
+} nbnxm_ic_options_t;
 
 /* Pair-list parameters (src/gromacs/nbnxm/pairlistparams.h, PairlistParams). */
 typedef struct nbnxm_pairlist_params
@@ -154,6 +197,13 @@ double nbnxm_calc_ewaldcoeff_q(double rc, double rtol);
 /* Fills ic for Ewald (analytical) + LJ cut-off with PotShift on both
  * (src/gromacs/mdtypes/interaction_const.cpp:106-115,266-278,306-314). */
 int nbnxm_init_interaction_const(double rc, double ewaldRtol, double epsilonR, nbnxm_interaction_const_t* ic);
+/* Any supported flavour: init_interaction_const
+ * (src/gromacs/mdtypes/interaction_const.cpp:249-345) for Ewald or reaction
+ * field with PotShift Coulomb, LJ with PotShift / ForceSwitch / PotSwitch, an
+ * optional LJ-PME grid correction and rvdw <= rcoulomb. */
+int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* options, nbnxm_interaction_const_t* ic);
+/* calc_ewaldcoeff_lj (src/gromacs/ewald/ewald_utils.cpp:73-112). */
+double nbnxm_calc_ewaldcoeff_lj(double rc, double rtol);
 /* Shift vectors of a rectangular box (calc_shifts, src/gromacs/pbcutil/pbc.cpp:1218-1233). */
 int nbnxm_calc_shifts(const double box[3], float* shiftVec /* 3*45 */);
 

@@ -33,10 +33,18 @@ class _Atoms(ctypes.Structure):
 
 
 class _IC(ctypes.Structure):
+    """nbnxm_interaction_const_t of include/nbnxm_amd.h."""
     _fields_ = [("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double),
                 ("ewaldcoeff_q", ctypes.c_double), ("sh_ewald", ctypes.c_double),
                 ("epsfac", ctypes.c_double), ("dispersion_shift_cpot", ctypes.c_double),
-                ("repulsion_shift_cpot", ctypes.c_double)]
+                ("repulsion_shift_cpot", ctypes.c_double),
+                ("eeltype", ctypes.c_int), ("vdwModifier", ctypes.c_int), ("ljPme", ctypes.c_int),
+                ("rvdw_switch", ctypes.c_double), ("ewaldcoeff_lj", ctypes.c_double),
+                ("sh_lj_ewald", ctypes.c_double),
+                ("dispersion_shift_c2", ctypes.c_double), ("dispersion_shift_c3", ctypes.c_double),
+                ("repulsion_shift_c2", ctypes.c_double), ("repulsion_shift_c3", ctypes.c_double),
+                ("vdw_switch_c3", ctypes.c_double), ("vdw_switch_c4", ctypes.c_double),
+                ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double)]
 
 
 _vp = ctypes.c_void_p
@@ -80,9 +88,12 @@ def _p(a):
 
 
 def _ic(ic):
+    """Copies any object with the interaction-constant attributes; missing
+    flavour fields mean Ewald + LJ potential shift."""
     c = _IC()
-    for name, _ in _IC._fields_:
-        setattr(c, name, float(getattr(ic, name)))
+    for name, ctype in _IC._fields_:
+        v = getattr(ic, name, 0)
+        setattr(c, name, int(v) if ctype is ctypes.c_int else float(v))
     return c
 
 

@@ -125,7 +125,10 @@ extern "C" {
 // nbat->ljComb) keeping (i-half, j-cluster) pairs with any atom pair within rPrune.
 cpu4xm* cpu4xm_create(const nbnxm_pairlist_gpu_t* list, const nbnxm_atomdata_t* nbat, const nbnxm_interaction_const_t* ic, double rPrune, int numThreads)
 {
-    if (!list || !nbat || !ic || nbat->vdwType != NBNXM_VDW_CUT_COMB_GEOM || !nbat->ljComb)
+    // Ewald + LJ cut-off with the potential shift and rvdw = rcoulomb only.
+    if (!list || !nbat || !ic || nbat->vdwType != NBNXM_VDW_CUT_COMB_GEOM || !nbat->ljComb
+        || ic->eeltype != NBNXM_ELEC_EWALD || ic->vdwModifier != NBNXM_VDW_MOD_POTSHIFT || ic->ljPme
+        || ic->rvdw != ic->rcoulomb)
     {
         return nullptr;
     }

@@ -112,9 +112,311 @@ static int check_inputs(const nbnxm_pairlist_gpu_t* list, const nbnxm_atomdata_t
 }
 
 /* ------------------------------------------------------------------------
+ * Pair interaction of every kernel flavour, in the order of the CUDA kernel
+ * body (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:514-665): clamp r^2,
+ * LJ with the PotShift constants, force switch (with the unmasked c6/c12 of
+ * :564-566, as the GPU kernels do), LJ-PME grid correction (:570-596),
+ * potential switch (:599-605), the separate LJ cut-off check of twin cut-offs
+ * (:607-617), then Coulomb: Ewald (:638-641, :655-660) or reaction field
+ * (:633-635, :651-654; plain cut-off is reaction field with k_rf = 0).
+ * ---------------------------------------------------------------------- */
+
+/* The FP32 parameters as the CUDA kernel holds them (NBParamGpu floats,
+ * src/gromacs/nbnxm/nbnxm_gpu_data_mgmt.cpp:93-159) and derives them
+ * (nbnxm_cuda_kernel.cuh:311-376). */
+typedef struct
+{
+    int eeltype, vdwModifier, ljPme, vdwCutoffCheck;
+    float rcoulomb_sq, rvdw_sq, epsfac, beta, beta2, beta3, sh_ewald, two_k_rf, c_rf;
+    float cpot_rep, cpot_disp, rvdw_switch;
+    float disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
+    float lje_coeff2, lje_coeff6_6, sh_lj_ewald;
+} flav32_t;
+
+static flav32_t flav32_from_ic(const nbnxm_interaction_const_t* ic)
+{
+    flav32_t p;
+    memset(&p, 0, sizeof(p));
+    p.eeltype = ic->eeltype;
+    p.vdwModifier = ic->vdwModifier;
+    p.ljPme = ic->ljPme;
+    p.vdwCutoffCheck = ic->rvdw < ic->rcoulomb;
+    const float rc = (float)ic->rcoulomb;
+    const float rvdw = (float)ic->rvdw;
+    p.rcoulomb_sq = rc * rc;
+    p.rvdw_sq = rvdw * rvdw;
+    p.epsfac = (float)ic->epsfac;
+    p.beta = (float)ic->ewaldcoeff_q;
+    p.beta2 = p.beta * p.beta;
+    p.beta3 = p.beta * p.beta * p.beta;
+    p.sh_ewald = (float)ic->sh_ewald;
+    p.two_k_rf = (float)(2.0 * ic->k_rf);
+    p.c_rf = (float)ic->c_rf;
+    p.cpot_rep = (float)ic->repulsion_shift_cpot;
+    p.cpot_disp = (float)ic->dispersion_shift_cpot;
+    p.rvdw_switch = (float)ic->rvdw_switch;
+    p.disp_c2 = (float)ic->dispersion_shift_c2;
+    p.disp_c3 = (float)ic->dispersion_shift_c3;
+    p.rep_c2 = (float)ic->repulsion_shift_c2;
+    p.rep_c3 = (float)ic->repulsion_shift_c3;
+    p.sw_c3 = (float)ic->vdw_switch_c3;
+    p.sw_c4 = (float)ic->vdw_switch_c4;
+    p.sw_c5 = (float)ic->vdw_switch_c5;
+    const float ljc = (float)ic->ewaldcoeff_lj;
+    p.lje_coeff2 = ljc * ljc;
+    p.lje_coeff6_6 = p.lje_coeff2 * p.lje_coeff2 * p.lje_coeff2 * (1.0F / 6.0F);
+    p.sh_lj_ewald = (float)ic->sh_lj_ewald;
+    return p;
+}
+
+/* One pair in FP32.  qq = qi*epsfac*qj.  lbForceOnly selects the
+ * sigma/epsilon form of :555-562 (LB combination, no energies, no switch).
+ * Returns F/r; adds the pair's energies to *eLJ / *eEl when computeEnergy. */
+static float pair_f32(const flav32_t* p,
+                      int computeEnergy,
+                      float r2,
+                      float int_bit,
+                      float qq,
+                      float c6,
+                      float c12,
+                      int lbForceOnly,
+                      float sigma,
+                      float epsilon,
+                      float c6grid,
+                      float* eLJ,
+                      float* eEl)
+{
+    const float oneSixth = 1.0F / 6.0F;
+    const float oneTwelveth = 1.0F / 12.0F;
+    r2 = fmaxf(r2, NBNXM_MIN_DISTANCE_SQUARED);
+    const float inv_r = 1.0F / sqrtf(r2);
+    const float inv_r2 = inv_r * inv_r;
+    float F_invr;
+    float E_lj_p = 0.0F;
+    float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    inv_r6 *= int_bit;
+    if (lbForceOnly)
+    {
+        const float sig_r = sigma * inv_r;
+        const float sig_r2 = sig_r * sig_r;
+        float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+        sig_r6 *= int_bit;
+        F_invr = epsilon * sig_r6 * (sig_r6 - 1.0F) * inv_r2;
+    }
+    else
+    {
+        F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+    }
+    if (computeEnergy || p->vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    {
+        E_lj_p = int_bit * (c12 * (inv_r6 * inv_r6 + p->cpot_rep) * oneTwelveth - c6 * (inv_r6 + p->cpot_disp) * oneSixth);
+    }
+    if (p->vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
+    {
+        /* calculate_force_switch_F[_E] (nbnxm_cuda_kernel_utils.cuh:89-145) */
+        const float r = r2 * inv_r;
+        float r_switch = r - p->rvdw_switch;
+        r_switch = r_switch >= 0.0F ? r_switch : 0.0F;
+        F_invr += -c6 * (p->disp_c2 + p->disp_c3 * r_switch) * r_switch * r_switch * inv_r
+                  + c12 * (p->rep_c2 + p->rep_c3 * r_switch) * r_switch * r_switch * inv_r;
+        if (computeEnergy)
+        {
+            const float disp_F2 = p->disp_c2 / 3;
+            const float disp_F3 = p->disp_c3 / 4;
+            const float repu_F2 = p->rep_c2 / 3;
+            const float repu_F3 = p->rep_c3 / 4;
+            E_lj_p += c6 * (disp_F2 + disp_F3 * r_switch) * r_switch * r_switch * r_switch
+                      - c12 * (repu_F2 + repu_F3 * r_switch) * r_switch * r_switch * r_switch;
+        }
+    }
+    if (p->ljPme)
+    {
+        /* calculate_lj_ewald_comb_geom_F[_E] (nbnxm_cuda_kernel_utils.cuh:233-288) */
+        const float inv_r6_nm = inv_r2 * inv_r2 * inv_r2;
+        const float cr2 = p->lje_coeff2 * r2;
+        const float expmcr2 = expf(-cr2);
+        const float poly = 1.0F + cr2 + 0.5F * cr2 * cr2;
+        F_invr += c6grid * (inv_r6_nm - expmcr2 * (inv_r6_nm * poly + p->lje_coeff6_6)) * inv_r2;
+        if (computeEnergy)
+        {
+            const float sh_mask = p->sh_lj_ewald * int_bit;
+            E_lj_p += oneSixth * c6grid * (inv_r6_nm * (1.0F - expmcr2 * poly) + sh_mask);
+        }
+    }
+    if (p->vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    {
+        /* calculate_potential_switch_F[_E] (nbnxm_cuda_kernel_utils.cuh:147-209) */
+        const float switch_F2 = 3 * p->sw_c3;
+        const float switch_F3 = 4 * p->sw_c4;
+        const float switch_F4 = 5 * p->sw_c5;
+        const float r = r2 * inv_r;
+        float r_switch = r - p->rvdw_switch;
+        if (computeEnergy)
+        {
+            r_switch = r_switch >= 0.0F ? r_switch : 0.0F;
+        }
+        if (computeEnergy || r_switch > 0.0F)
+        {
+            const float sw = 1.0F + (p->sw_c3 + (p->sw_c4 + p->sw_c5 * r_switch) * r_switch) * r_switch * r_switch * r_switch;
+            const float dsw = (switch_F2 + (switch_F3 + switch_F4 * r_switch) * r_switch) * r_switch * r_switch;
+            F_invr = F_invr * sw - inv_r * E_lj_p * dsw;
+            if (computeEnergy)
+            {
+                E_lj_p *= sw;
+            }
+        }
+    }
+    if (p->vdwCutoffCheck)
+    {
+        const float vdw_in_range = (r2 < p->rvdw_sq) ? 1.0F : 0.0F;
+        F_invr *= vdw_in_range;
+        E_lj_p *= vdw_in_range;
+    }
+    if (computeEnergy)
+    {
+        *eLJ += E_lj_p;
+    }
+    if (p->eeltype == NBNXM_ELEC_RF)
+    {
+        F_invr += qq * (int_bit * inv_r2 * inv_r - p->two_k_rf);
+        if (computeEnergy)
+        {
+            *eEl += qq * (int_bit * inv_r + 0.5F * p->two_k_rf * r2 - p->c_rf);
+        }
+    }
+    else
+    {
+        F_invr += qq * (int_bit * inv_r2 * inv_r + oracle_pmecorrF(p->beta2 * r2) * p->beta3);
+        if (computeEnergy)
+        {
+            *eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * p->beta)) - int_bit * p->sh_ewald);
+        }
+    }
+    return F_invr;
+}
+
+/* The same pair in FP64 with exact erf/exp: the Ewald force correction is
+ * 2 beta/sqrt(pi) exp(-beta^2 r^2)/r^2 - erf(beta r)/r^3, the exact function
+ * pmecorrF approximates. */
+static double pair_f64(const nbnxm_interaction_const_t* ic,
+                       double r2,
+                       double int_bit,
+                       double qq,
+                       double c6,
+                       double c12,
+                       double c6grid,
+                       double* eLJ,
+                       double* eEl)
+{
+    if (r2 < NBNXM_MIN_DISTANCE_SQUARED)
+    {
+        r2 = NBNXM_MIN_DISTANCE_SQUARED;
+    }
+    const double r = sqrt(r2);
+    const double inv_r = 1.0 / r;
+    const double inv_r2 = inv_r * inv_r;
+    const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
+    double F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+    double E_lj_p = int_bit
+                    * (c12 * (inv_r6 * inv_r6 + ic->repulsion_shift_cpot) / 12.0
+                       - c6 * (inv_r6 + ic->dispersion_shift_cpot) / 6.0);
+    if (ic->vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
+    {
+        double rs = r - ic->rvdw_switch;
+        rs = rs > 0.0 ? rs : 0.0;
+        F_invr += (-c6 * (ic->dispersion_shift_c2 + ic->dispersion_shift_c3 * rs)
+                   + c12 * (ic->repulsion_shift_c2 + ic->repulsion_shift_c3 * rs))
+                  * rs * rs * inv_r;
+        E_lj_p += (c6 * (ic->dispersion_shift_c2 / 3.0 + ic->dispersion_shift_c3 / 4.0 * rs)
+                   - c12 * (ic->repulsion_shift_c2 / 3.0 + ic->repulsion_shift_c3 / 4.0 * rs))
+                  * rs * rs * rs;
+    }
+    if (ic->ljPme)
+    {
+        const double b2 = ic->ewaldcoeff_lj * ic->ewaldcoeff_lj;
+        const double b6_6 = b2 * b2 * b2 / 6.0;
+        const double inv_r6_nm = inv_r2 * inv_r2 * inv_r2;
+        const double cr2 = b2 * r2;
+        const double e = exp(-cr2);
+        const double poly = 1.0 + cr2 + 0.5 * cr2 * cr2;
+        F_invr += c6grid * (inv_r6_nm - e * (inv_r6_nm * poly + b6_6)) * inv_r2;
+        E_lj_p += c6grid / 6.0 * (inv_r6_nm * (1.0 - e * poly) + ic->sh_lj_ewald * int_bit);
+    }
+    if (ic->vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    {
+        double rs = r - ic->rvdw_switch;
+        rs = rs > 0.0 ? rs : 0.0;
+        const double sw = 1.0 + (ic->vdw_switch_c3 + (ic->vdw_switch_c4 + ic->vdw_switch_c5 * rs) * rs) * rs * rs * rs;
+        const double dsw = (3.0 * ic->vdw_switch_c3 + (4.0 * ic->vdw_switch_c4 + 5.0 * ic->vdw_switch_c5 * rs) * rs) * rs * rs;
+        F_invr = F_invr * sw - inv_r * E_lj_p * dsw;
+        E_lj_p *= sw;
+    }
+    if (ic->rvdw < ic->rcoulomb && !(r2 < ic->rvdw * ic->rvdw))
+    {
+        F_invr = 0.0;
+        E_lj_p = 0.0;
+    }
+    *eLJ += E_lj_p;
+    if (ic->eeltype == NBNXM_ELEC_RF)
+    {
+        F_invr += qq * (int_bit * inv_r2 * inv_r - 2.0 * ic->k_rf);
+        *eEl += qq * (int_bit * inv_r + ic->k_rf * r2 - ic->c_rf);
+    }
+    else
+    {
+        const double br = ic->ewaldcoeff_q * r;
+        const double erfbr = erf(br);
+        F_invr += qq * (int_bit * inv_r2 * inv_r + 2.0 * ic->ewaldcoeff_q * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
+        *eEl += qq * (inv_r * (int_bit - erfbr) - int_bit * ic->sh_ewald);
+    }
+    return F_invr;
+}
+
+/* Self energies on the diagonal entry (nbnxm_cuda_kernel.cuh:381-418):
+ * Coulomb -epsfac sum q^2 times beta/sqrt(pi) (Ewald) or c_rf/2 (reaction
+ * field); LJ-PME + sum 6C6_ii/12 * beta_lj^6/6. */
+static double self_coulomb_factor(const nbnxm_interaction_const_t* ic)
+{
+    return ic->eeltype == NBNXM_ELEC_RF ? 0.5 * ic->c_rf : ic->ewaldcoeff_q * c_oneSqrtPi;
+}
+
+static int check_flavour(const nbnxm_atomdata_t* nbat, const nbnxm_interaction_const_t* ic)
+{
+    if (ic->eeltype != NBNXM_ELEC_EWALD && ic->eeltype != NBNXM_ELEC_RF)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    if (ic->vdwModifier < NBNXM_VDW_MOD_POTSHIFT || ic->vdwModifier > NBNXM_VDW_MOD_POTSWITCH)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    /* LJ-PME reads the type table for the grid C6 (NULL nbat: brute force, table given). */
+    if (ic->ljPme && nbat && (!nbat->type || !nbat->nbfp))
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    return NBNXM_OK;
+}
+
+/* Per-type sqrt(6C6_ii) of the LJ-PME geometric grid rule (atomdata.cpp:299-309). */
+static float* grid_c6_f32(const nbnxm_atomdata_t* nbat)
+{
+    const int nt = nbat->numTypes;
+    float* g = (float*)malloc(sizeof(float) * (size_t)(nt > 0 ? nt : 1));
+    if (g)
+    {
+        for (int t = 0; t < nt; t++)
+        {
+            g[t] = sqrtf(nbat->nbfp[2 * (t * nt + t)]);
+        }
+    }
+    return g;
+}
+
+/* ------------------------------------------------------------------------
  * FP32 restatement of the CUDA kernel body
- * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:325-718, EL_EWALD_ANA with
- * LJ cut-off; LJ parameters as :514-533).
+ * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:325-718; LJ parameters as
+ * :514-533), every flavour through pair_f32.
  * ---------------------------------------------------------------------- */
 int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
                       const nbnxm_atomdata_t* nbat,
@@ -126,22 +428,23 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
                       double* Vc)
 {
     int rc = check_inputs(list, nbat);
-    if (rc != NBNXM_OK || !ic || !f || !fshift)
+    if (rc != NBNXM_OK || !ic || !f || !fshift || check_flavour(nbat, ic) != NBNXM_OK)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const flav32_t p = flav32_from_ic(ic);
+    float* gridC6 = p.ljPme ? grid_c6_f32(nbat) : NULL;
+    if (p.ljPme && !gridC6)
     {
         return NBNXM_ERR_INVALID;
     }
     const float* xq = nbat->xq;
-    const float epsfac = (float)ic->epsfac;
-    const float rcoulomb_sq = (float)(ic->rcoulomb * ic->rcoulomb);
-    const float beta = (float)ic->ewaldcoeff_q;
-    const float beta2 = beta * beta;
-    const float beta3 = beta * beta * beta;
-    const float ewald_shift = (float)ic->sh_ewald;
-    const float cpot_rep = (float)ic->repulsion_shift_cpot;
-    const float cpot_disp = (float)ic->dispersion_shift_cpot;
-    const float oneSixth = 1.0F / 6.0F;
-    const float oneTwelveth = 1.0F / 12.0F;
+    const float epsfac = p.epsfac;
+    const float selfFactor = (p.eeltype == NBNXM_ELEC_RF) ? 0.5F * p.c_rf : p.beta * (float)c_oneSqrtPi;
     const int ntypes = nbat->numTypes;
+    /* The sigma/epsilon force of :555-562 exists in the LB cut-off kernel only. */
+    const int lbForceOnlyKernel = !computeEnergy && nbat->vdwType == NBNXM_VDW_CUT_COMB_LB
+                                  && p.vdwModifier == NBNXM_VDW_MOD_POTSHIFT && !p.ljPme;
     double vvdwTot = 0.0;
     double vcTot = 0.0;
 
@@ -155,14 +458,23 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
         if (computeEnergy && s.shift == NBNXM_CENTRAL_SHIFT && s.cjPackedEnd > s.cjPackedBegin
             && list->cjPacked[s.cjPackedBegin].cj[0] == s.sci * NCPS)
         {
-            /* Self energy on the diagonal entry (:381-418). */
             float sumq2 = 0.0F;
+            float sumc6 = 0.0F;
             for (int a = 0; a < NCPS * CL; a++)
             {
-                const float qi = xq[4 * (s.sci * NCPS * CL + a) + 3] * epsfac;
+                const int ai = s.sci * NCPS * CL + a;
+                const float qi = xq[4 * ai + 3] * epsfac;
                 sumq2 += qi * qi;
+                if (p.ljPme)
+                {
+                    sumc6 += nbat->nbfp[2 * (nbat->type[ai] * (ntypes + 1))];
+                }
             }
-            E_el += -(sumq2 / epsfac) * beta * (float)c_oneSqrtPi;
+            E_el += -(sumq2 / epsfac) * selfFactor;
+            if (p.ljPme)
+            {
+                E_lj += sumc6 * 0.5F * (1.0F / 6.0F) * p.lje_coeff6_6;
+            }
         }
 
         for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
@@ -199,16 +511,15 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
                             const float dx = xi - xq[4 * aj + 0];
                             const float dy = yi - xq[4 * aj + 1];
                             const float dz = zi - xq[4 * aj + 2];
-                            float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             const int nonSelf = !(s.shift == NBNXM_CENTRAL_SHIFT && tj <= ti);
-                            if (!(r2 < rcoulomb_sq && (nonSelf || ci != cj)))
+                            if (!(r2 < p.rcoulomb_sq && (nonSelf || ci != cj)))
                             {
                                 continue;
                             }
                             const float int_bit = ((ex->pair[tj * CL + ti] >> bit) & 1U) ? 1.0F : 0.0F;
                             const float qj = xq[4 * aj + 3];
                             float c6, c12;
-                            int lbForceOnly = 0;
                             float sigma = 0.0F, epsilon = 0.0F;
                             if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
                             {
@@ -225,43 +536,15 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
                             {
                                 sigma = nbat->ljComb[2 * ai] + nbat->ljComb[2 * aj];
                                 epsilon = nbat->ljComb[2 * ai + 1] * nbat->ljComb[2 * aj + 1];
-                                /* convert_sigma_epsilon_to_c6_c12 (nbnxm_cuda_kernel_utils.cuh) */
+                                /* convert_sigma_epsilon_to_c6_c12 (nbnxm_cuda_kernel_utils.cuh:76-86) */
                                 const float sigma2 = sigma * sigma;
                                 const float sigma6 = sigma2 * sigma2 * sigma2;
                                 c6 = epsilon * sigma6;
                                 c12 = c6 * sigma6;
-                                lbForceOnly = !computeEnergy;
                             }
-                            r2 = fmaxf(r2, NBNXM_MIN_DISTANCE_SQUARED);
-                            const float inv_r = 1.0F / sqrtf(r2);
-                            const float inv_r2 = inv_r * inv_r;
-                            float F_invr;
-                            float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                            inv_r6 *= int_bit;
-                            if (lbForceOnly)
-                            {
-                                /* LB force-only branch (:555-562) */
-                                const float sig_r = sigma * inv_r;
-                                const float sig_r2 = sig_r * sig_r;
-                                float sig_r6 = sig_r2 * sig_r2 * sig_r2;
-                                sig_r6 *= int_bit;
-                                F_invr = epsilon * sig_r6 * (sig_r6 - 1.0F) * inv_r2;
-                            }
-                            else
-                            {
-                                F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-                            }
-                            if (computeEnergy)
-                            {
-                                E_lj += int_bit
-                                        * (c12 * (inv_r6 * inv_r6 + cpot_rep) * oneTwelveth
-                                           - c6 * (inv_r6 + cpot_disp) * oneSixth);
-                            }
-                            F_invr += qi * qj * (int_bit * inv_r2 * inv_r + oracle_pmecorrF(beta2 * r2) * beta3);
-                            if (computeEnergy)
-                            {
-                                E_el += qi * qj * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewald_shift);
-                            }
+                            const float c6grid = p.ljPme ? gridC6[nbat->type[ai]] * gridC6[nbat->type[aj]] : 0.0F;
+                            const float F_invr = pair_f32(&p, computeEnergy, r2, int_bit, qi * qj, c6, c12,
+                                                          lbForceOnlyKernel, sigma, epsilon, c6grid, &E_lj, &E_el);
                             const float tx = dx * F_invr;
                             const float ty = dy * F_invr;
                             const float tz = dz * F_invr;
@@ -288,6 +571,7 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
         vvdwTot += E_lj;
         vcTot += E_el;
     }
+    free(gridC6);
     if (computeEnergy)
     {
         if (Vvdw)
@@ -302,21 +586,6 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
     return NBNXM_OK;
 }
 
-/* Ewald real-space pair terms in FP64 with exact special functions.
- * fOverR gets qq*(int_bit/r^3 + [2 beta/sqrt(pi) exp(-beta^2 r^2)/r^2 - erf(beta r)/r^3]),
- * the exact counterpart of :639-640; vOut qq*((int_bit - erf(beta r))/r - int_bit*sh_ewald) (:656-659). */
-static void ewald_pair_f64(double qq, double r2, double int_bit, double beta, double sh_ewald, double* fOverR, double* vOut)
-{
-    const double r = sqrt(r2);
-    const double inv_r = 1.0 / r;
-    const double inv_r2 = inv_r * inv_r;
-    const double br = beta * r;
-    const double erfbr = erf(br);
-    const double corr = 2.0 * beta * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r;
-    *fOverR = qq * (int_bit * inv_r2 * inv_r + corr);
-    *vOut = qq * (inv_r * (int_bit - erfbr) - int_bit * sh_ewald);
-}
-
 int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                       const nbnxm_atomdata_t* nbat,
                       const nbnxm_interaction_const_t* ic,
@@ -327,14 +596,13 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                       double* Vc)
 {
     int rc = check_inputs(list, nbat);
-    if (rc != NBNXM_OK || !ic || !f || !fshift)
+    if (rc != NBNXM_OK || !ic || !f || !fshift || check_flavour(nbat, ic) != NBNXM_OK)
     {
         return NBNXM_ERR_INVALID;
     }
     const float* xq = nbat->xq;
     const double epsfac = ic->epsfac;
     const double rcoulomb_sq = ic->rcoulomb * ic->rcoulomb;
-    const double beta = ic->ewaldcoeff_q;
     const int ntypes = nbat->numTypes;
     double vvdwTot = 0.0;
     double vcTot = 0.0;
@@ -347,13 +615,23 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
         if (computeEnergy && s.shift == NBNXM_CENTRAL_SHIFT && s.cjPackedEnd > s.cjPackedBegin
             && list->cjPacked[s.cjPackedBegin].cj[0] == s.sci * NCPS)
         {
-            double sumq2 = 0.0;
+            double sumq2 = 0.0, sumc6 = 0.0;
             for (int a = 0; a < NCPS * CL; a++)
             {
-                const double q = xq[4 * (s.sci * NCPS * CL + a) + 3];
+                const int ai = s.sci * NCPS * CL + a;
+                const double q = xq[4 * ai + 3];
                 sumq2 += q * q;
+                if (ic->ljPme)
+                {
+                    sumc6 += nbat->nbfp[2 * (nbat->type[ai] * (ntypes + 1))];
+                }
             }
-            vcTot += -epsfac * sumq2 * beta * c_oneSqrtPi;
+            vcTot += -epsfac * sumq2 * self_coulomb_factor(ic);
+            if (ic->ljPme)
+            {
+                const double b2 = ic->ewaldcoeff_lj * ic->ewaldcoeff_lj;
+                vvdwTot += sumc6 / 12.0 * (b2 * b2 * b2 / 6.0);
+            }
         }
 
         for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
@@ -386,7 +664,7 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                             const double dx = xi - (double)xq[4 * aj + 0];
                             const double dy = yi - (double)xq[4 * aj + 1];
                             const double dz = zi - (double)xq[4 * aj + 2];
-                            double r2 = dx * dx + dy * dy + dz * dz;
+                            const double r2 = dx * dx + dy * dy + dz * dz;
                             const int nonSelf = !(s.shift == NBNXM_CENTRAL_SHIFT && tj <= ti);
                             if (!(r2 < rcoulomb_sq && (nonSelf || ci != cj)))
                             {
@@ -414,22 +692,18 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                                 c6 = epsilon * sigma6;
                                 c12 = c6 * sigma6;
                             }
-                            if (r2 < NBNXM_MIN_DISTANCE_SQUARED)
+                            double c6grid = 0.0;
+                            if (ic->ljPme)
                             {
-                                r2 = NBNXM_MIN_DISTANCE_SQUARED;
+                                c6grid = sqrt((double)nbat->nbfp[2 * (nbat->type[ai] * (ntypes + 1))]
+                                              * (double)nbat->nbfp[2 * (nbat->type[aj] * (ntypes + 1))]);
                             }
-                            const double inv_r2 = 1.0 / r2;
-                            const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
-                            double F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-                            double fc, vc;
-                            ewald_pair_f64(qi * qj, r2, int_bit, beta, ic->sh_ewald, &fc, &vc);
-                            F_invr += fc;
+                            double elj = 0.0, eel = 0.0;
+                            const double F_invr = pair_f64(ic, r2, int_bit, qi * qj, c6, c12, c6grid, &elj, &eel);
                             if (computeEnergy)
                             {
-                                vvdwTot += int_bit
-                                           * (c12 * (inv_r6 * inv_r6 + ic->repulsion_shift_cpot) / 12.0
-                                              - c6 * (inv_r6 + ic->dispersion_shift_cpot) / 6.0);
-                                vcTot += vc;
+                                vvdwTot += elj;
+                                vcTot += eel;
                             }
                             const double tx = dx * F_invr;
                             const double ty = dy * F_invr;
@@ -501,7 +775,7 @@ static int bruteforce_f64(int n,
                           long long* numPairsInRange,
                           double* virial)
 {
-    if (n < 0 || !x || !q || !type || !nbfp || !box || !ic || !f)
+    if (n < 0 || !x || !q || !type || !nbfp || !box || !ic || !f || check_flavour(NULL, ic) != NBNXM_OK)
     {
         return NBNXM_ERR_INVALID;
     }
@@ -514,7 +788,6 @@ static int bruteforce_f64(int n,
         }
     }
     const double rc2 = rc * rc;
-    const double beta = ic->ewaldcoeff_q;
     const double epsfac = ic->epsfac;
     double vvdw = 0.0, vc = 0.0;
     long long npairs = 0;
@@ -538,16 +811,18 @@ static int bruteforce_f64(int n,
         }
     }
 
-    double sumq2 = 0.0;
+    double sumq2 = 0.0, sumc6 = 0.0;
     for (int i = 0; i < n; i++)
     {
         sumq2 += q[i] * q[i];
+        sumc6 += nbfp[2 * (numTypes * type[i] + type[i])];
         for (int j = i + 1; j < n; j++)
         {
             const int excluded = is_excluded(i, j, exclRanges, exclElements);
             const double qq = epsfac * q[i] * q[j];
             const double c6 = nbfp[2 * (numTypes * type[i] + type[j])];
             const double c12 = nbfp[2 * (numTypes * type[i] + type[j]) + 1];
+            const double c6grid = ic->ljPme ? sqrt(nbfp[2 * (numTypes * type[i] + type[i])] * nbfp[2 * (numTypes * type[j] + type[j])]) : 0.0;
             for (int sz = -1; sz <= 1; sz++)
             {
                 for (int sy = -1; sy <= 1; sy++)
@@ -557,7 +832,7 @@ static int bruteforce_f64(int n,
                         const double dx = xw[3 * i + 0] - xw[3 * j + 0] + sx * box[0];
                         const double dy = xw[3 * i + 1] - xw[3 * j + 1] + sy * box[1];
                         const double dz = xw[3 * i + 2] - xw[3 * j + 2] + sz * box[2];
-                        double r2 = dx * dx + dy * dy + dz * dz;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 >= rc2)
                         {
                             continue;
@@ -567,20 +842,7 @@ static int bruteforce_f64(int n,
                         {
                             npairs++;
                         }
-                        if (r2 < NBNXM_MIN_DISTANCE_SQUARED)
-                        {
-                            r2 = NBNXM_MIN_DISTANCE_SQUARED;
-                        }
-                        const double inv_r2 = 1.0 / r2;
-                        const double inv_r6 = int_bit * inv_r2 * inv_r2 * inv_r2;
-                        double fr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-                        double fcoul, vcoul;
-                        ewald_pair_f64(qq, r2, int_bit, beta, ic->sh_ewald, &fcoul, &vcoul);
-                        fr += fcoul;
-                        vvdw += int_bit
-                                * (c12 * (inv_r6 * inv_r6 + ic->repulsion_shift_cpot) / 12.0
-                                   - c6 * (inv_r6 + ic->dispersion_shift_cpot) / 6.0);
-                        vc += vcoul;
+                        const double fr = pair_f64(ic, r2, int_bit, qq, c6, c12, c6grid, &vvdw, &vc);
                         f[3 * i + 0] += fr * dx;
                         f[3 * i + 1] += fr * dy;
                         f[3 * i + 2] += fr * dz;
@@ -604,7 +866,12 @@ static int bruteforce_f64(int n,
             }
         }
     }
-    vc += -epsfac * beta * c_oneSqrtPi * sumq2;
+    vc += -epsfac * self_coulomb_factor(ic) * sumq2;
+    if (ic->ljPme)
+    {
+        const double b2 = ic->ewaldcoeff_lj * ic->ewaldcoeff_lj;
+        vvdw += sumc6 / 12.0 * (b2 * b2 * b2 / 6.0);
+    }
     free(xw);
     if (Vvdw)
     {

@@ -9,12 +9,17 @@
  * Pinning status (what fixes each function independently of its own formula):
  *   oracle_calc_ewaldcoeff_q   closed form erfc(beta*rc) == rtol; beta(1.0,1e-5)
  *   oracle_pmecorrF            closed form 2exp(-z^2)/(sqrt(pi)z^2) - erf(z)/z^3, |err|<1e-6
- *   oracle_kernel_f64          refdata XML golden forces/energies (243-atom box),
- *                              brute force all-image sum, Newton's third law
+ *   oracle_kernel_f64          refdata XML golden forces/energies (243-atom box) of
+ *                              all 18 Coulomb x LJ flavours, brute force all-image
+ *                              sum per flavour, Newton's third law
  *   oracle_kernel_f32          same golden vectors at the reference FP32 tolerance;
- *                              agreement with oracle_kernel_f64
- *   oracle_bruteforce_f64      exact pair sums on 2-3 atom systems (hand values),
- *                              translation invariance, golden XML
+ *                              agreement with oracle_kernel_f64 per flavour
+ *   oracle_bruteforce_f64      exact pair sums on 2-3 atom systems (hand values):
+ *                              Ewald, reaction field (+ excluded pair), LJ-PME as
+ *                              the real-space kernel -C6 g(beta r)/r^6, force and
+ *                              potential switch continuity and dV/dr = -F;
+ *                              translation invariance; all 18 golden XML cases to
+ *                              1e-5 (forces) and 1e-8 (LJ energy)
  *   oracle_prune               double-precision any-pair-within-r predicate away
  *                              from the threshold; masks subset of input masks
  *   oracle_sci_sort            counts non-increasing, permutation of input
@@ -34,7 +39,13 @@ extern "C" {
 double oracle_calc_ewaldcoeff_q(double rc, double rtol);
 float oracle_pmecorrF(float z2);
 
-/* Restatement of the CUDA kernel nbnxn_kernel_ElecEw_VdwLJ*_{F,VF}_cuda in FP32
+/* Every kernel flavour follows ic: Ewald or reaction field (plain cut-off is
+ * reaction field with k_rf = 0); LJ with potential shift, force switch or
+ * potential switch; optional LJ-PME grid correction with the geometric grid
+ * rule (needs nbat->type and the nbfp table); rvdw < rcoulomb adds the LJ
+ * cut-off check of the twin-cut-off kernels.
+ *
+ * Restatement of the CUDA kernels nbnxn_kernel_Elec*_Vdw*_{F,VF}_cuda in FP32
  * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:145-741), loop structure of
  * nbnxn_kernel_gpu_ref (src/gromacs/nbnxm/kernels_reference/kernel_gpu_ref.cpp:57-354).
  * Forces are accumulated (f += ...) in grid order; fshift[45*3] accumulated. */
@@ -59,7 +70,8 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
 
 /* All-image O(N^2) FP64 sum in original atom order, independent of any list.
  * Box must be rectangular with every edge > rc.  Excluded pairs get the Ewald
- * exclusion correction in every image within rc (the list semantics of
+ * or reaction-field exclusion correction (and the LJ-PME grid correction, with
+ * grid C6 = sqrt(nbfp_ii nbfp_jj)) in every image within rc (the list semantics of
  * src/gromacs/nbnxm/pairlist.cpp:1948-2046), self energy included
  * (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:381-418). */
 int oracle_bruteforce_f64(int n,

@@ -6,10 +6,14 @@ Run once in a container that has the reference checkout:
 It extracts data only:
   - the 243-atom water box of src/gromacs/nbnxm/tests/spc27_coords.h
     (coordinates and box),
-  - the double-precision reference forces and energies of
-    src/gromacs/nbnxm/tests/refdata/NbnxmKernelTest_CoulombEwald_VdwCutComb{Geom,LB,None}.xml,
+  - the double-precision reference forces and energies of the 18 files
+    src/gromacs/nbnxm/tests/refdata/NbnxmKernelTest_Coulomb{Ewald,EwaldTwin,ReactionField}_
+    Vdw{CutCombGeom,CutCombLB,CutCombNone,ForceSwitch,PotSwitch,EwaldCombGeom}.xml,
 and records the test options of src/gromacs/nbnxm/tests/kernel_test.cpp that
-produced them (cut-off 0.9 nm, Ewald rtol 1e-6, PotShift, three atom types).
+produced them (cut-off 0.9 nm, Ewald rtol 1e-6, PotShift Coulomb, three atom
+types; kernel_test.cpp:402-447 for the per-case options: twin cut-off
+rvdw = 0.7 nm, rvdw_switch = rvdw - 0.2 nm, LJ-PME rtol 1e-4 with the
+geometric grid rule, reaction field with epsilon_rf = 0).
 """
 import json
 import os
@@ -18,7 +22,9 @@ import sys
 import xml.etree.ElementTree as ET
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CASES = ["CoulombEwald_VdwCutCombGeom", "CoulombEwald_VdwCutCombLB", "CoulombEwald_VdwCutCombNone"]
+COULOMB = ["Ewald", "EwaldTwin", "ReactionField"]
+VDW = ["CutCombGeom", "CutCombLB", "CutCombNone", "ForceSwitch", "PotSwitch", "EwaldCombGeom"]
+CASES = [f"Coulomb{c}_Vdw{v}" for c in COULOMB for v in VDW]
 
 
 def parse_coords(path):
@@ -60,8 +66,11 @@ def main():
             "charge_O": -0.8476, "charge_H": 0.4238,
             "sigma_O": 0.316557, "epsilon_O": 0.650194,
             "sigma_H": 0.04, "epsilon_H": 0.192464,
+            "ewald_rtol_lj": 1e-4, "epsilon_rf": 0.0, "rvdw_twin": 0.7, "switch_width": 0.2,
             "note": "kernel_test.cpp:104-129,150-271: first third of atoms uncharged, "
-                    "hydrogens of the last third carry LJ, water exclusions",
+                    "hydrogens of the last third carry LJ, water exclusions; "
+                    "kernel_test.cpp:402-447: per-case interaction options; "
+                    "kernel_test.cpp:577-611: LB-combined parameters for every case but CutCombGeom",
         },
         "box": box,
         "coordinates": coords,

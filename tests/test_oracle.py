@@ -7,14 +7,7 @@ from scipy.special import erf
 
 import nbnxm_amd as nb
 import oracle
-from helpers import rel_rms, wrapped_float_system
-
-GOLDEN_CASES = [
-    ("CoulombEwald_VdwCutCombNone", "none", nb.VDW_CUT_TYPE_TABLE),
-    ("CoulombEwald_VdwCutCombGeom", "geom", nb.VDW_CUT_COMB_GEOM),
-    ("CoulombEwald_VdwCutCombLB", "lb", nb.VDW_CUT_COMB_LB),
-]
-REF_FORCE_TOL = 0.0119  # kernel_test.cpp:663-698 in FP32
+from helpers import GOLDEN_ALL, golden_case, rel_rms, wrapped_float_system
 
 
 def test_ewald_coeff_closed_form():
@@ -79,11 +72,11 @@ def test_bruteforce_translation_invariance():
     assert eel2 == pytest.approx(eel1, rel=1e-10)
 
 
-@pytest.mark.parametrize("case,rule,vdw", GOLDEN_CASES)
-def test_golden_bruteforce(golden, case, rule, vdw):
-    """The FP64 all-image sum reproduces the reference's double-precision refdata."""
-    system = nb.systems.spc27_system(golden, rule)
-    ic = nb.interaction_const(0.9, 1e-6)
+@pytest.mark.parametrize("case", GOLDEN_ALL)
+def test_golden_bruteforce(golden, case):
+    """The FP64 all-image sum reproduces the reference's double-precision refdata
+    for every Coulomb / LJ flavour."""
+    system, ic, _, _ = golden_case(golden, case)
     f, elj, eel, _ = oracle.bruteforce(system, ic)
     ref = np.array(golden["cases"][case]["forces"])
     assert np.abs(f - ref).max() < 1e-5
@@ -91,20 +84,25 @@ def test_golden_bruteforce(golden, case, rule, vdw):
     assert eel == pytest.approx(golden["cases"][case]["coulomb_energy"], abs=1e-5)
 
 
-@pytest.mark.parametrize("case,rule,vdw", GOLDEN_CASES)
+@pytest.mark.parametrize("case", GOLDEN_ALL)
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_golden_list_kernel(golden, case, rule, vdw, precision):
-    """The list-based restatements match the refdata within the reference test tolerance."""
-    system = nb.systems.spc27_system(golden, rule)
-    ic = nb.interaction_const(0.9, 1e-6)
+def test_golden_list_kernel(golden, case, precision):
+    """The list-based restatements match the refdata within the reference test
+    tolerance (kernel_test.cpp:663-698), energies included."""
+    system, ic, vdw, tol = golden_case(golden, case)
     search, plist, atoms = nb.build(system, 0.9, vdw)
     f, fs, elj, eel = oracle.kernel(plist, atoms, ic, True, precision)
     ref = np.array(golden["cases"][case]["forces"])
     fo = atoms.to_original_order(f, system.n)
-    tol = 3e-3 if precision == "f64" else REF_FORCE_TOL
+    if precision == "f64":
+        tol = 3e-3
     assert np.abs(fo - ref).max() < tol
     assert abs(elj - golden["cases"][case]["vdw_energy"]) < tol
     assert abs(eel - golden["cases"][case]["coulomb_energy"]) < 10 * tol
+    # The force-only pass gives the same forces within 10 ulp at magnitude 1000
+    # (kernel_test.cpp:720-729; the LB force-only form rounds differently).
+    f2, _, _, _ = oracle.kernel(plist, atoms, ic, False, precision)
+    assert np.abs(atoms.to_original_order(f2, system.n) - fo).max() < (1e-9 if precision == "f64" else 1000 * 10 * 2.0 ** -23)
 
 
 @pytest.mark.parametrize("n_mol,max_pairs", [(216, 0), (216, 16), (500, 0), (700, 24)])
@@ -219,3 +217,111 @@ def test_cpu4xm_baseline_matches_fp64():
     assert vc == pytest.approx(eel64, rel=1e-4)
     f2, _, _ = k.compute(False)
     assert rel_rms(f2[real], f[real]) < 1e-6
+
+
+FLAVOURS = {
+    "rf": dict(coulomb="rf"),
+    "cutoff": dict(coulomb="rf", plain_cutoff=True),
+    "twin": dict(rvdw=0.8),
+    "forceswitch": dict(vdw_modifier="forceswitch", rvdw_switch=0.8),
+    "potswitch": dict(vdw_modifier="potswitch", rvdw_switch=0.75),
+    "ljpme": dict(lj_pme=True, ewald_rtol_lj=1e-3),
+    "rf_twin_potswitch": dict(coulomb="rf", rvdw=0.9, vdw_modifier="potswitch", rvdw_switch=0.7),
+    "ljpme_twin": dict(lj_pme=True, rvdw=0.85),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FLAVOURS))
+def test_flavour_list_kernel_equals_bruteforce(name):
+    """List semantics of every flavour (excluded pairs, Newton masks, twin cut-off
+    check) against the list-free FP64 sum on identical FP32-rounded inputs."""
+    s = wrapped_float_system(nb.systems.water_box(216, None, seed=5))
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5, **FLAVOURS[name])
+    search, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_TYPE_TABLE, 16)
+    f, fs, elj, eel = oracle.kernel(plist, atoms, ic, True, "f64")
+    fb, eljb, eelb, npairs = oracle.bruteforce(s, ic)
+    assert rel_rms(atoms.to_original_order(f, s.n), fb) < 1e-6
+    assert eel == pytest.approx(eelb, rel=1e-7)
+    assert elj == pytest.approx(eljb, rel=1e-6)
+    f32, _, elj32, eel32 = oracle.kernel(plist, atoms, ic, True, "f32")
+    real = atoms.atom_indices >= 0
+    assert rel_rms(f32[real], f[real]) < 5e-6
+    # Energies accumulate in FP32 per entry, as the CUDA kernel does.
+    assert eel32 == pytest.approx(eel, rel=1e-4)
+    assert elj32 == pytest.approx(elj, rel=1e-4)
+
+
+def _two_atoms(r, q=(1.0, -1.0), c6=1e-3, c12=1e-6, excluded=False):
+    ranges = np.array([0, 2, 4], dtype=np.int32) if excluded else np.array([0, 1, 2], dtype=np.int32)
+    elements = np.array([0, 1, 0, 1], dtype=np.int32) if excluded else np.array([0, 1], dtype=np.int32)
+    return nb.systems.System(box=np.array([3.0, 3.0, 3.0]), x=np.array([[0.5, 0.5, 0.5], [0.5 + r, 0.5, 0.5]]),
+                             q=np.array(q), types=np.array([0, 0], dtype=np.int32),
+                             c6c12=np.array([[[c6, c12]]]), excl_ranges=ranges, excl_elements=elements)
+
+
+def test_reaction_field_closed_form():
+    """Two ions: V = qq (1/r + k_rf r^2 - c_rf), F/r = qq (1/r^3 - 2 k_rf), self
+    energy -c_rf/2 epsfac sum q^2 (rf_util.cpp:50-62 constants, eps_rf = inf)."""
+    rc, r = 1.0, 0.4
+    ic = nb.interaction_const_ex(rc, coulomb="rf", epsilon_rf=0.0)
+    assert ic.k_rf == pytest.approx(0.5 / rc ** 3) and ic.c_rf == pytest.approx(1.5 / rc)
+    s = _two_atoms(r, c6=0.0, c12=0.0)
+    f, elj, eel, _ = oracle.bruteforce(s, ic)
+    qq = -ic.epsfac
+    assert eel == pytest.approx(qq * (1 / r + ic.k_rf * r * r - ic.c_rf) - 0.5 * ic.c_rf * ic.epsfac * 2, rel=1e-12)
+    assert f[0, 0] == pytest.approx(-qq * (1 / r ** 3 - 2 * ic.k_rf) * r, rel=1e-12)
+    # Excluded: only the reaction-field correction qq (k_rf r^2 - c_rf), force -2 k_rf qq r.
+    f, _, eel, _ = oracle.bruteforce(_two_atoms(r, c6=0.0, c12=0.0, excluded=True), ic)
+    assert eel == pytest.approx(qq * (ic.k_rf * r * r - ic.c_rf) - ic.c_rf * ic.epsfac, rel=1e-12)
+    assert f[0, 0] == pytest.approx(2 * ic.k_rf * qq * r, rel=1e-12)
+    # Finite epsilon_rf: k_rf = (eps_rf - eps_r) / (2 eps_rf + eps_r) / rc^3.
+    ic2 = nb.interaction_const_ex(1.2, coulomb="rf", epsilon_rf=78.0)
+    assert ic2.k_rf == pytest.approx(77.0 / 157.0 / 1.2 ** 3) and ic2.c_rf == pytest.approx(1 / 1.2 + ic2.k_rf * 1.44)
+
+
+def test_lj_switch_continuity():
+    """Force switch: force and energy vanish at rvdw, force equals plain LJ
+    below rvdw_switch; potential switch: energy and force vanish at rvdw and equal
+    the unshifted LJ below rvdw_switch (interaction_const.cpp:199-246)."""
+    c6, c12 = 2e-3, 3e-6
+    rc, rsw = 1.0, 0.7
+    for modifier in ("forceswitch", "potswitch"):
+        ic = nb.interaction_const_ex(rc, vdw_modifier=modifier, rvdw_switch=rsw)
+        ic.epsfac = 0.0  # LJ only
+        for r in (0.5, 0.69):
+            f, elj, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=c12), ic)
+            assert -f[0, 0] == pytest.approx((12 * c12 / r ** 13 - 6 * c6 / r ** 7), rel=1e-10)
+            if modifier == "potswitch":
+                assert elj == pytest.approx(c12 / r ** 12 - c6 / r ** 6, rel=1e-10)
+        r = rc - 1e-7
+        f, elj, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=c12), ic)
+        assert abs(elj) < 1e-12 and abs(f[0, 0]) < 1e-9
+        # Energy is the integral of the force: central difference in the switch region.
+        h, r = 1e-5, 0.85
+        e1 = oracle.bruteforce(_two_atoms(r - h, q=(0.0, 0.0), c6=c6, c12=c12), ic)[1]
+        e2 = oracle.bruteforce(_two_atoms(r + h, q=(0.0, 0.0), c6=c6, c12=c12), ic)[1]
+        f, _, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=c12), ic)
+        assert -f[0, 0] == pytest.approx(-(e2 - e1) / (2 * h), rel=1e-6)
+
+
+def test_lj_pme_closed_form():
+    """With equal table and grid C6 the dispersion becomes the real-space Ewald
+    kernel -C6 g(beta r)/r^6, g(x) = exp(-x^2)(1 + x^2 + x^4/2) (ewald_utils.cpp:73-82),
+    plus constants; calc_ewaldcoeff_lj solves g(beta rc) = rtol."""
+    rc, rtol = 1.0, 1e-3
+    beta = nb.ewald_coeff_lj(rc, rtol)
+    g = lambda x: math.exp(-x * x) * (1 + x * x + x ** 4 / 2)
+    assert g(beta * rc) == pytest.approx(rtol, rel=1e-12)
+    ic = nb.interaction_const_ex(rc, lj_pme=True, ewald_rtol_lj=rtol)
+    assert ic.ewaldcoeff_lj == beta
+    ic.epsfac = 0.0
+    c6 = 2e-3
+    self_e = 2 * c6 * beta ** 6 / 12.0   # + sum C6_ii beta^6 / 12 (nbnxm_cuda_kernel.cuh:395-409)
+    for r in (0.3, 0.6, 0.95):
+        _, elj, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=0.0), ic)
+        expect = -c6 * g(beta * r) / r ** 6 - c6 * (-1 / rc ** 6) + c6 * ic.sh_lj_ewald
+        assert elj - self_e == pytest.approx(expect, rel=1e-10)
+    # An excluded pair keeps only the grid correction C6 (1 - g(beta r))/r^6.
+    r = 0.3
+    _, elj, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=0.0, excluded=True), ic)
+    assert elj - self_e == pytest.approx(c6 * (1 - g(beta * r)) / r ** 6, rel=1e-10)
