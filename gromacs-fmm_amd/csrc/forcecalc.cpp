@@ -156,9 +156,9 @@ extern "C" int nbnxm_force_calculator_create(int numParticles,
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "force_calculator_create: this calculator runs on the GPU only (useGpu must be set)");
     }
-    if (options->coulombType != NBNXM_COULOMB_PME)
+    if (options->coulombType < NBNXM_COULOMB_PME || options->coulombType > NBNXM_COULOMB_REACTION_FIELD)
     {
-        return fail(NBNXM_ERR_UNSUPPORTED, "force_calculator_create: only the PME (Ewald real-space) Coulomb type is implemented");
+        return fail(NBNXM_ERR_UNSUPPORTED, "force_calculator_create: unknown Coulomb type");
     }
     if (options->useTabulatedEwaldCorr)
     {
@@ -221,13 +221,25 @@ extern "C" int nbnxm_force_calculator_create(int numParticles,
         }
     }
     c->options = *options;
-    const double rtol = options->ewaldRtol > 0 ? options->ewaldRtol : 1e-5;
-    const double epsR = options->epsilonR > 0 ? options->epsilonR : 1.0;
-    int rc = nbnxm_init_interaction_const(options->pairlistCutoff, rtol, epsR, &c->ic);
+    // createInteractionConst (api/nblib/nbnxmsetuphelpers.cpp:219-276): LJ cut-off
+    // with the potential shift at rvdw = rcoulomb = pairlistCutoff; PME without
+    // the Coulomb potential shift ("we ignore the potential shift", sh_ewald = 0);
+    // Cutoff and ReactionField through calc_rffac with epsilon_rf = 1, which
+    // gives k_rf = 0 and c_rf = 1/rc for both.
+    nbnxm_ic_options_t io{};
+    io.eeltype = (options->coulombType == NBNXM_COULOMB_PME) ? NBNXM_ELEC_EWALD : NBNXM_ELEC_RF;
+    io.vdwModifier = NBNXM_VDW_MOD_POTSHIFT;
+    io.rcoulomb = options->pairlistCutoff;
+    io.rvdw = options->pairlistCutoff;
+    io.ewaldRtol = options->ewaldRtol > 0 ? options->ewaldRtol : 1e-5;
+    io.epsilonR = options->epsilonR > 0 ? options->epsilonR : 1.0;
+    io.epsilonRf = 1.0;
+    int rc = nbnxm_init_interaction_const_ex(&io, &c->ic);
     if (rc != NBNXM_OK)
     {
         return rc;
     }
+    c->ic.sh_ewald = 0.0;
     c->rlist = options->rlist > 0 ? options->rlist : options->pairlistCutoff;
     rc = nbnxm_detect_comb_rule(numTypes, c->nbfp.data(), 1e-5);
     if (rc < 0)

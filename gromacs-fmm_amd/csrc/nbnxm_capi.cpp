@@ -77,6 +77,48 @@ int validateAtomdata(const nbnxm_atomdata_t* nbat)
     return NBNXM_OK;
 }
 
+// The flavours the kernels implement (nbnxm_cuda.cu:171-254 selects among the
+// same set): Ewald or reaction field, LJ PotShift / ForceSwitch / PotSwitch,
+// geometric LJ-PME with PotShift, rvdw <= rcoulomb.
+int validateFlavour(const nbnxm_interaction_const_t* ic)
+{
+    if (ic->eeltype != NBNXM_ELEC_EWALD && ic->eeltype != NBNXM_ELEC_RF)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: unknown Coulomb type");
+    }
+    if (ic->vdwModifier < NBNXM_VDW_MOD_POTSHIFT || ic->vdwModifier > NBNXM_VDW_MOD_POTSWITCH)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: unknown LJ modifier");
+    }
+    if (ic->ljPme && ic->vdwModifier != NBNXM_VDW_MOD_POTSHIFT)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: LJ-PME supports the potential-shift modifier only");
+    }
+    if (!(ic->rcoulomb > 0) || !(ic->rvdw > 0) || ic->rvdw > ic->rcoulomb)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: need 0 < rvdw <= rcoulomb");
+    }
+    return NBNXM_OK;
+}
+
+// LJ-PME reads per-slot types and the type table for the grid C6 of every
+// combination kernel.
+int validateLjPmeAtomdata(const nbnxm_atomdata_t* nbat)
+{
+    if (nbat->numTypes < 1 || !nbat->nbfp || (nbat->numAtoms > 0 && !nbat->type))
+    {
+        return fail(NBNXM_ERR_INVALID, "atomdata: LJ-PME needs numTypes, type and nbfp");
+    }
+    for (int i = 0; i < nbat->numAtoms; i++)
+    {
+        if (nbat->type[i] < 0 || nbat->type[i] >= nbat->numTypes)
+        {
+            return fail(NBNXM_ERR_INVALID, "atomdata: type of slot " + std::to_string(i) + " out of range");
+        }
+    }
+    return NBNXM_OK;
+}
+
 int validatePairlist(const nbnxm_pairlist_gpu_t* list, int numAtoms)
 {
     if (!list || list->na_ci != NBNXM_CLUSTER_SIZE || list->nsci < 0 || list->ncjPacked < 0 || list->nexcl < 1)
@@ -142,6 +184,9 @@ struct nbnxm_gpu
     DeviceBuffer<float4> paramsA;
     DeviceBuffer<float2> paramsB;
     DeviceBuffer<double> paramsD;
+    NbFlavourParams flavour;       // used unless the flavour is the tuned default
+    DeviceBuffer<float> gridC6;    // LJ-PME: sqrt(6 C6_ii) per type
+    DeviceBuffer<double> gridC6D;
 
     int nsci = 0;
     int ncjPacked = 0;
@@ -565,9 +610,10 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
     {
         return fail(NBNXM_ERR_INVALID, "gpu_init: unknown accumulation mode");
     }
-    if (std::fabs(ic->rvdw - ic->rcoulomb) > 1e-12 * ic->rcoulomb)
+    int frc = validateFlavour(ic);
+    if (frc != NBNXM_OK)
     {
-        return fail(NBNXM_ERR_UNSUPPORTED, "gpu_init: rvdw must equal rcoulomb (no twin cut-off kernel)");
+        return frc;
     }
     if (!(listParams->rlistOuter >= ic->rcoulomb) || !(listParams->rlistInner >= ic->rcoulomb)
         || listParams->rlistInner > listParams->rlistOuter || listParams->numRollingPruningParts < 1)
@@ -591,6 +637,7 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hFshift), 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipHostMallocDefault));
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hVirial), 9 * sizeof(double), hipHostMallocDefault));
     nb->ic = *ic;
+    nb->flavour = flavourParams(*ic);
     nb->listParams = *listParams;
     nb->accumulation = accumulation;
     rc = uploadParams(nb.get());
@@ -629,6 +676,14 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
     {
         return rc;
     }
+    if (nb->ic.ljPme)
+    {
+        rc = validateLjPmeAtomdata(nbat);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+    }
     const int n = nbat->numAtoms;
     nb->numAtoms = n;
     nb->vdwType = nbat->vdwType;
@@ -642,7 +697,7 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
     {
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->xq.ptr, nbat->xq, sizeof(float4) * n, hipMemcpyHostToDevice, nb->stream));
     }
-    if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    if (nbat->vdwType == NBNXM_VDW_CUT_TYPE_TABLE || nb->ic.ljPme)
     {
         NBNXM_HIP_CHECK(nb->types.resize(n));
         NBNXM_HIP_CHECK(nb->nbfp.resize(static_cast<size_t>(nbat->numTypes) * nbat->numTypes));
@@ -652,13 +707,34 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
         }
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->nbfp.ptr, nbat->nbfp, sizeof(float2) * nbat->numTypes * nbat->numTypes, hipMemcpyHostToDevice, nb->stream));
     }
-    else
+    if (nbat->vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
     {
         NBNXM_HIP_CHECK(nb->ljComb.resize(n));
         if (n > 0)
         {
             NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ljComb.ptr, nbat->ljComb, sizeof(float2) * n, hipMemcpyHostToDevice, nb->stream));
         }
+    }
+    if (nb->ic.ljPme)
+    {
+        // Geometric grid rule: sqrt of the diagonal 6 C6 (atomdata.cpp:299-309).
+        const int nt = nbat->numTypes;
+        std::vector<float> g(nt);
+        std::vector<double> gd(nt);
+        for (int t = 0; t < nt; t++)
+        {
+            const float c6 = nbat->nbfp[2 * (static_cast<size_t>(t) * nt + t)];
+            g[t] = std::sqrt(c6);
+            gd[t] = std::sqrt(static_cast<double>(c6));
+        }
+        NBNXM_HIP_CHECK(nb->gridC6.resize(nt));
+        NBNXM_HIP_CHECK(nb->gridC6D.resize(nt));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->gridC6.ptr, g.data(), sizeof(float) * nt, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->gridC6D.ptr, gd.data(), sizeof(double) * nt, hipMemcpyHostToDevice, nb->stream));
+        // The copies read host vectors that go out of scope: finish them here.
+        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+        nb->flavour.gridC6 = nb->gridC6.ptr;
+        nb->flavour.gridC6D = nb->gridC6D.ptr;
     }
     NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
@@ -908,7 +984,14 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     a.fFixed = nb->fFixed.ptr;
     a.entryPartials = nb->partials.ptr;
     a.computeFshift = computeVirial ? 1 : 0;
-    NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
+    if (isDefaultFlavour(nb->ic))
+    {
+        NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
+    }
+    else
+    {
+        NBNXM_HIP_CHECK(launchNonbondedFlavour(a, nb->flavour, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->stream));
+    }
     nb->lastComputeEnergy = computeEnergy != 0;
     nb->lastComputeVirial = computeVirial != 0;
     nb->forcesConsumed = false;
@@ -1124,7 +1207,7 @@ int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb)
 
 int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb)
 {
-    return nb ? 1 : NBNXM_ERR_INVALID;
+    return nb ? (nb->ic.eeltype == NBNXM_ELEC_EWALD ? 1 : 0) : NBNXM_ERR_INVALID;
 }
 
 int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int* imaskOuter)

@@ -1,0 +1,512 @@
+// gfx950 kernel for every interaction flavour other than the tuned default
+// (Ewald analytical + LJ cut-off with potential shift, rvdw = rcoulomb):
+// reaction field / plain cut-off Coulomb, the twin cut-off LJ check, LJ force
+// switch, LJ potential switch and the LJ-PME grid correction with the
+// geometric grid rule.  These are the kernels the reference generates from one
+// body with preprocessor flavours (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh,
+// selected in nbnxm_cuda.cu:171-254); here one body takes the flavour as
+// wave-uniform kernel arguments, so a flavour costs scalar branches, not a
+// separate code object.
+//
+// Layout: one wave per pair-list entry, lane l = i-atom (l & 7) x j-atom (l >> 3)
+// of each 8x8 cluster pair, i-cluster data in registers for the whole entry,
+// j-forces reduced over the 8 i-lanes, i-forces over the 8 j-lanes at the end.
+// Real = float is the FP32 kernel (the reference's arithmetic); Real = double
+// is the FP64 kernel with exact erf/exp.  Forces go to int64 fixed point or,
+// for FP32 with float accumulation, to float atomics.
+#include "device_common.h"
+#include "nbnxm_kernels.h"
+
+#include <type_traits>
+
+namespace nbnxm_amd
+{
+
+namespace
+{
+
+using namespace dev;
+
+// Flavour constants in the kernel's arithmetic type, derived the way the
+// reference kernel derives them from its FP32 parameter block
+// (nbnxm_gpu_data_mgmt.cpp:93-159, nbnxm_cuda_kernel.cuh:311-376).
+template<typename Real>
+struct FlavourConsts
+{
+    Real rcoulomb_sq, rvdw_sq, epsfac, beta, beta2, beta3, sh_ewald, two_k_rf, c_rf;
+    Real cpot_rep, cpot_disp, rvdw_switch;
+    Real disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
+    Real lje_coeff2, lje_coeff6_6, sh_lj_ewald;
+
+    __device__ explicit FlavourConsts(const NbFlavourParams& p)
+    {
+        const Real rc = static_cast<Real>(p.rcoulomb);
+        const Real rvdw = static_cast<Real>(p.rvdw);
+        rcoulomb_sq = rc * rc;
+        rvdw_sq = rvdw * rvdw;
+        epsfac = static_cast<Real>(p.epsfac);
+        beta = static_cast<Real>(p.beta);
+        beta2 = beta * beta;
+        beta3 = beta * beta * beta;
+        sh_ewald = static_cast<Real>(p.sh_ewald);
+        two_k_rf = static_cast<Real>(2.0 * p.k_rf);
+        c_rf = static_cast<Real>(p.c_rf);
+        cpot_rep = static_cast<Real>(p.cpot_rep);
+        cpot_disp = static_cast<Real>(p.cpot_disp);
+        rvdw_switch = static_cast<Real>(p.rvdw_switch);
+        disp_c2 = static_cast<Real>(p.disp_c2);
+        disp_c3 = static_cast<Real>(p.disp_c3);
+        rep_c2 = static_cast<Real>(p.rep_c2);
+        rep_c3 = static_cast<Real>(p.rep_c3);
+        sw_c3 = static_cast<Real>(p.sw_c3);
+        sw_c4 = static_cast<Real>(p.sw_c4);
+        sw_c5 = static_cast<Real>(p.sw_c5);
+        const Real ljc = static_cast<Real>(p.ewaldcoeff_lj);
+        lje_coeff2 = ljc * ljc;
+        lje_coeff6_6 = lje_coeff2 * lje_coeff2 * lje_coeff2 * (Real(1) / Real(6));
+        sh_lj_ewald = static_cast<Real>(p.sh_lj_ewald);
+    }
+};
+
+template<typename Real>
+__device__ __forceinline__ Real rsqrtR(Real x)
+{
+    if constexpr (std::is_same_v<Real, float>)
+    {
+        return __builtin_amdgcn_rsqf(x);
+    }
+    else
+    {
+        return 1.0 / sqrt(x);
+    }
+}
+
+template<typename Real>
+__device__ __forceinline__ Real expR(Real x)
+{
+    if constexpr (std::is_same_v<Real, float>)
+    {
+        return __expf(x);
+    }
+    else
+    {
+        return exp(x);
+    }
+}
+
+// One pair, in the order of the reference kernel body (nbnxm_cuda_kernel.cuh:514-665):
+// clamp r^2, LJ with the PotShift constants, force switch (unmasked c6/c12 as
+// the reference GPU kernels use them), LJ-PME grid part, potential switch, LJ
+// cut-off check of twin cut-offs, Coulomb.  Returns F/r; adds energies.
+template<typename Real, bool computeEnergy>
+__device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
+                                          const FlavourConsts<Real>& k,
+                                          Real r2,
+                                          Real int_bit,
+                                          Real qq,
+                                          Real c6,
+                                          Real c12,
+                                          Real c6grid,
+                                          Real& eLJ,
+                                          Real& eEl)
+{
+    constexpr Real oneSixth = Real(1) / Real(6);
+    constexpr Real oneTwelveth = Real(1) / Real(12);
+    r2 = r2 > Real(NBNXM_MIN_DISTANCE_SQUARED) ? r2 : Real(NBNXM_MIN_DISTANCE_SQUARED);
+    const Real inv_r = rsqrtR(r2);
+    const Real inv_r2 = inv_r * inv_r;
+    const Real inv_r6_nm = inv_r2 * inv_r2 * inv_r2;
+    const Real inv_r6 = inv_r6_nm * int_bit;
+    Real F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+    Real E_lj_p = int_bit * (c12 * (inv_r6 * inv_r6 + k.cpot_rep) * oneTwelveth - c6 * (inv_r6 + k.cpot_disp) * oneSixth);
+    if (flav.vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
+    {
+        const Real r = r2 * inv_r;
+        Real rs = r - k.rvdw_switch;
+        rs = rs >= Real(0) ? rs : Real(0);
+        F_invr += (-c6 * (k.disp_c2 + k.disp_c3 * rs) + c12 * (k.rep_c2 + k.rep_c3 * rs)) * rs * rs * inv_r;
+        if constexpr (computeEnergy)
+        {
+            E_lj_p += (c6 * (k.disp_c2 / Real(3) + k.disp_c3 / Real(4) * rs) - c12 * (k.rep_c2 / Real(3) + k.rep_c3 / Real(4) * rs))
+                      * rs * rs * rs;
+        }
+    }
+    if (flav.ljPme)
+    {
+        const Real cr2 = k.lje_coeff2 * r2;
+        const Real expmcr2 = expR(-cr2);
+        const Real poly = Real(1) + cr2 + Real(0.5) * cr2 * cr2;
+        F_invr += c6grid * (inv_r6_nm - expmcr2 * (inv_r6_nm * poly + k.lje_coeff6_6)) * inv_r2;
+        if constexpr (computeEnergy)
+        {
+            E_lj_p += oneSixth * c6grid * (inv_r6_nm * (Real(1) - expmcr2 * poly) + k.sh_lj_ewald * int_bit);
+        }
+    }
+    if (flav.vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    {
+        const Real r = r2 * inv_r;
+        Real rs = r - k.rvdw_switch;
+        rs = rs >= Real(0) ? rs : Real(0);
+        const Real sw = Real(1) + (k.sw_c3 + (k.sw_c4 + k.sw_c5 * rs) * rs) * rs * rs * rs;
+        const Real dsw = (Real(3) * k.sw_c3 + (Real(4) * k.sw_c4 + Real(5) * k.sw_c5 * rs) * rs) * rs * rs;
+        F_invr = F_invr * sw - inv_r * E_lj_p * dsw;
+        E_lj_p *= sw;
+    }
+    if (flav.vdwCutoffCheck && !(r2 < k.rvdw_sq))
+    {
+        F_invr = Real(0);
+        E_lj_p = Real(0);
+    }
+    if constexpr (computeEnergy)
+    {
+        eLJ += E_lj_p;
+    }
+    if (flav.eeltype == NBNXM_ELEC_RF)
+    {
+        F_invr += qq * (int_bit * inv_r2 * inv_r - k.two_k_rf);
+        if constexpr (computeEnergy)
+        {
+            eEl += qq * (int_bit * inv_r + Real(0.5) * k.two_k_rf * r2 - k.c_rf);
+        }
+    }
+    else if constexpr (std::is_same_v<Real, float>)
+    {
+        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(k.beta2 * r2) * k.beta3);
+        if constexpr (computeEnergy)
+        {
+            eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+        }
+    }
+    else
+    {
+        const Real br = k.beta * r2 * inv_r;
+        const Real erfbr = erf(br);
+        F_invr += qq * (int_bit * inv_r2 * inv_r + Real(2) * k.beta * Real(c_oneSqrtPi) * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
+        if constexpr (computeEnergy)
+        {
+            eEl += qq * (inv_r * (int_bit - erfbr) - int_bit * k.sh_ewald);
+        }
+    }
+    return F_invr;
+}
+
+// Per-type grid factor sqrt(6 C6_ii) of the geometric LJ-PME rule.
+template<typename Real>
+__device__ __forceinline__ Real gridC6(const NbFlavourParams& flav, int type)
+{
+    if constexpr (std::is_same_v<Real, float>)
+    {
+        return flav.gridC6[type];
+    }
+    else
+    {
+        return flav.gridC6D[type];
+    }
+}
+
+template<typename Real>
+__device__ __forceinline__ Real sumOverTi(Real v)
+{
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
+template<typename Real>
+__device__ __forceinline__ Real sumOverTj(Real v)
+{
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+template<typename Real, int accumulation>
+__device__ __forceinline__ void addForce(const NbKernelArgs& a, int index, Real v)
+{
+    if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+    {
+        atomicAdd(a.fFloat + index, static_cast<float>(v));
+    }
+    else if constexpr (std::is_same_v<Real, float>)
+    {
+        atomicAdd(a.fFixed + index, toFixed(v));
+    }
+    else
+    {
+        atomicAdd(a.fFixed + index, toFixedD(v));
+    }
+}
+
+template<typename Real, bool computeEnergy, int accumulation>
+__global__ __launch_bounds__(256) void nbnxmFlavourKernel(const NbKernelArgs a, const NbFlavourParams flav, int vdwType, int wavesPerBlock)
+{
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7;
+    const int tj = lane >> 3;
+    const int work = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    if (work >= a.nsci)
+    {
+        return;
+    }
+    const FlavourConsts<Real> k(flav);
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
+    const nbnxm_sci_t s = a.sci[entry];
+    const bool centralShift = (s.shift % NBNXM_NUM_SHIFTS) == NBNXM_CENTRAL_SHIFT;
+    const float4 sh = a.shiftVec[s.shift];
+    const bool useTable = (vdwType == NBNXM_VDW_CUT_TYPE_TABLE);
+    const bool useGeom = (vdwType == NBNXM_VDW_CUT_COMB_GEOM);
+    const bool needTypes = useTable || flav.ljPme;
+
+    // i-cluster atoms of this lane, shifted, charges scaled by epsfac.
+    Real xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
+    int typei[c_ncps];
+    float2 combi[c_ncps];
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        const float4 v = a.xq[ai];
+        if constexpr (std::is_same_v<Real, float>)
+        {
+            xi[c] = v.x + sh.x;
+            yi[c] = v.y + sh.y;
+            zi[c] = v.z + sh.z;
+        }
+        else
+        {
+            xi[c] = static_cast<Real>(v.x) + sh.x;
+            yi[c] = static_cast<Real>(v.y) + sh.y;
+            zi[c] = static_cast<Real>(v.z) + sh.z;
+        }
+        qi[c] = static_cast<Real>(v.w) * k.epsfac;
+        typei[c] = needTypes ? a.atomTypes[ai] : 0;
+        combi[c] = useTable ? make_float2(0.f, 0.f) : a.ljComb[ai];
+    }
+
+    Real eLJ = Real(0), eEl = Real(0);
+    double eLJAcc = 0.0, eElAcc = 0.0;
+    if constexpr (computeEnergy)
+    {
+        // Self terms on the diagonal entry (nbnxm_cuda_kernel.cuh:381-418), on the tj == 0 lanes.
+        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && a.cjPacked[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
+        {
+            double q2 = 0.0, c6self = 0.0;
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const double q = a.xq[(s.sci * c_ncps + c) * c_cl + ti].w;
+                q2 += q * q;
+                if (flav.ljPme)
+                {
+                    c6self += a.nbfp[typei[c] * (a.numTypes + 1)].x;
+                }
+            }
+            const double selfFactor = (flav.eeltype == NBNXM_ELEC_RF) ? 0.5 * flav.c_rf : flav.beta * c_oneSqrtPi;
+            eElAcc -= flav.epsfac * q2 * selfFactor;
+            if (flav.ljPme)
+            {
+                const double b2 = flav.ewaldcoeff_lj * flav.ewaldcoeff_lj;
+                eLJAcc += c6self / 12.0 * (b2 * b2 * b2 / 6.0);
+            }
+        }
+    }
+    const bool nonSelfLane = !(centralShift && tj <= ti);
+    Real fci[c_ncps][3];
+    for (int c = 0; c < c_ncps; c++)
+    {
+        fci[c][0] = fci[c][1] = fci[c][2] = Real(0);
+    }
+
+    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+    {
+        const unsigned int imask = a.cjPacked[jp].imei[0].imask;
+        if (imask == 0)
+        {
+            continue;
+        }
+        const unsigned int wexcl = a.excl[a.cjPacked[jp].imei[0].excl_ind].pair[lane];
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if (!(imask & (0xFFu << (jm * c_ncps))))
+            {
+                continue;
+            }
+            const int cj = a.cjPacked[jp].cj[jm];
+            const int aj = cj * c_cl + tj;
+            const float4 xqj = a.xq[aj];
+            const int typej = needTypes ? a.atomTypes[aj] : 0;
+            const float2 combj = useTable ? make_float2(0.f, 0.f) : a.ljComb[aj];
+            const Real qj = static_cast<Real>(xqj.w);
+            const Real c6gridJ = flav.ljPme ? gridC6<Real>(flav, typej) : Real(0);
+            Real fcj[3] = { Real(0), Real(0), Real(0) };
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const int bit = jm * c_ncps + c;
+                if (!(imask & (1u << bit)))
+                {
+                    continue;
+                }
+                const Real dx = xi[c] - static_cast<Real>(xqj.x);
+                const Real dy = yi[c] - static_cast<Real>(xqj.y);
+                const Real dz = zi[c] - static_cast<Real>(xqj.z);
+                Real r2;
+                if constexpr (std::is_same_v<Real, float>)
+                {
+                    r2 = norm2Fixed(dx, dy, dz);
+                }
+                else
+                {
+                    r2 = dx * dx + dy * dy + dz * dz;
+                }
+                const bool notDiag = (s.sci * c_ncps + c) != cj;
+                if (r2 < k.rcoulomb_sq && (nonSelfLane || notDiag))
+                {
+                    const Real int_bit = ((wexcl >> bit) & 1u) ? Real(1) : Real(0);
+                    Real c6, c12;
+                    if (useTable)
+                    {
+                        const float2 p = a.nbfp[a.numTypes * typei[c] + typej];
+                        c6 = p.x;
+                        c12 = p.y;
+                    }
+                    else if (useGeom)
+                    {
+                        c6 = static_cast<Real>(combi[c].x) * combj.x;
+                        c12 = static_cast<Real>(combi[c].y) * combj.y;
+                    }
+                    else
+                    {
+                        // convert_sigma_epsilon_to_c6_c12 (nbnxm_cuda_kernel_utils.cuh:76-86).
+                        const Real sigma = static_cast<Real>(combi[c].x) + combj.x;
+                        const Real epsilon = static_cast<Real>(combi[c].y) * combj.y;
+                        const Real sigma2 = sigma * sigma;
+                        const Real sigma6 = sigma2 * sigma2 * sigma2;
+                        c6 = epsilon * sigma6;
+                        c12 = c6 * sigma6;
+                    }
+                    const Real c6grid = flav.ljPme ? gridC6<Real>(flav, typei[c]) * c6gridJ : Real(0);
+                    const Real F_invr = pairForce<Real, computeEnergy>(flav, k, r2, int_bit, qi[c] * qj, c6, c12, c6grid, eLJ, eEl);
+                    fcj[0] -= dx * F_invr;
+                    fcj[1] -= dy * F_invr;
+                    fcj[2] -= dz * F_invr;
+                    fci[c][0] += dx * F_invr;
+                    fci[c][1] += dy * F_invr;
+                    fci[c][2] += dz * F_invr;
+                }
+            }
+            const Real fj0 = sumOverTi(fcj[0]);
+            const Real fj1 = sumOverTi(fcj[1]);
+            const Real fj2 = sumOverTi(fcj[2]);
+            if (ti < 3)
+            {
+                addForce<Real, accumulation>(a, 3 * aj + ti, ti == 0 ? fj0 : (ti == 1 ? fj1 : fj2));
+            }
+            if constexpr (computeEnergy)
+            {
+                // FP64 per j-cluster keeps the long energy sums accurate.
+                eLJAcc += static_cast<double>(eLJ);
+                eElAcc += static_cast<double>(eEl);
+                eLJ = Real(0);
+                eEl = Real(0);
+            }
+        }
+    }
+    double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const Real f0 = sumOverTj(fci[c][0]);
+        const Real f1 = sumOverTj(fci[c][1]);
+        const Real f2 = sumOverTj(fci[c][2]);
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        if (tj < 3)
+        {
+            addForce<Real, accumulation>(a, 3 * ai + tj, tj == 0 ? f0 : (tj == 1 ? f1 : f2));
+        }
+        fsX += f0;
+        fsY += f1;
+        fsZ += f2;
+    }
+    if (computeEnergy || a.computeFshift)
+    {
+        const double keep = (tj == 0) ? 1.0 : 0.0;
+        const double v0 = reduceWaveD(eLJAcc);
+        const double v1 = reduceWaveD(eElAcc);
+        const bool doShift = a.computeFshift && !centralShift;
+        const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
+        const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
+        const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
+        if (lane < 8)
+        {
+            const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
+            a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+        }
+    }
+}
+
+template<typename Real, bool E, int A>
+hipError_t launchFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int vdwType, hipStream_t stream)
+{
+    constexpr int wpb = 4;
+    const int blocks = (a.nsci + wpb - 1) / wpb;
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((nbnxmFlavourKernel<Real, E, A>), dim3(blocks), dim3(64 * wpb), 0, stream, a, flav, vdwType, wpb);
+    return hipGetLastError();
+}
+
+template<bool E>
+hipError_t launchFlavourAcc(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, hipStream_t stream)
+{
+    switch (accumulation)
+    {
+        case NBNXM_ACCUM_FLOAT: return launchFlavour<float, E, NBNXM_ACCUM_FLOAT>(a, flav, vdwType, stream);
+        case NBNXM_ACCUM_FIXED64: return launchFlavour<float, E, NBNXM_ACCUM_FIXED64>(a, flav, vdwType, stream);
+        case NBNXM_ACCUM_FP64: return launchFlavour<double, E, NBNXM_ACCUM_FP64>(a, flav, vdwType, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
+{
+    NbFlavourParams p;
+    p.eeltype = ic.eeltype;
+    p.vdwModifier = ic.vdwModifier;
+    p.ljPme = ic.ljPme;
+    p.vdwCutoffCheck = ic.rvdw < ic.rcoulomb ? 1 : 0;
+    p.rcoulomb = ic.rcoulomb;
+    p.rvdw = ic.rvdw;
+    p.epsfac = ic.epsfac;
+    p.beta = ic.ewaldcoeff_q;
+    p.sh_ewald = ic.sh_ewald;
+    p.k_rf = ic.k_rf;
+    p.c_rf = ic.c_rf;
+    p.cpot_rep = ic.repulsion_shift_cpot;
+    p.cpot_disp = ic.dispersion_shift_cpot;
+    p.rvdw_switch = ic.rvdw_switch;
+    p.disp_c2 = ic.dispersion_shift_c2;
+    p.disp_c3 = ic.dispersion_shift_c3;
+    p.rep_c2 = ic.repulsion_shift_c2;
+    p.rep_c3 = ic.repulsion_shift_c3;
+    p.sw_c3 = ic.vdw_switch_c3;
+    p.sw_c4 = ic.vdw_switch_c4;
+    p.sw_c5 = ic.vdw_switch_c5;
+    p.ewaldcoeff_lj = ic.ewaldcoeff_lj;
+    p.sh_lj_ewald = ic.sh_lj_ewald;
+    return p;
+}
+
+hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream)
+{
+    if (vdwType < NBNXM_VDW_CUT_TYPE_TABLE || vdwType > NBNXM_VDW_CUT_COMB_LB || (flav.ljPme && (!flav.gridC6 || !flav.gridC6D || !a.atomTypes || !a.nbfp)))
+    {
+        return hipErrorInvalidValue;
+    }
+    return computeEnergy ? launchFlavourAcc<true>(a, flav, accumulation, vdwType, stream)
+                         : launchFlavourAcc<false>(a, flav, accumulation, vdwType, stream);
+}
+
+} // namespace nbnxm_amd

@@ -46,6 +46,35 @@ enum class KernelFlavor : int
     VF = 1
 };
 
+// Flavour of the generic kernel: the interaction constants of one launch
+// (nbnxm_interaction_const_t, wave-uniform), plus the per-type LJ-PME grid
+// factors sqrt(6 C6_ii) on the device in FP32 and FP64.
+struct NbFlavourParams
+{
+    int eeltype = NBNXM_ELEC_EWALD;
+    int vdwModifier = NBNXM_VDW_MOD_POTSHIFT;
+    int ljPme = 0;
+    int vdwCutoffCheck = 0; // rvdw < rcoulomb
+    double rcoulomb = 0, rvdw = 0, epsfac = 0, beta = 0, sh_ewald = 0, k_rf = 0, c_rf = 0;
+    double cpot_rep = 0, cpot_disp = 0, rvdw_switch = 0;
+    double disp_c2 = 0, disp_c3 = 0, rep_c2 = 0, rep_c3 = 0, sw_c3 = 0, sw_c4 = 0, sw_c5 = 0;
+    double ewaldcoeff_lj = 0, sh_lj_ewald = 0;
+    const float* gridC6 = nullptr;
+    const double* gridC6D = nullptr;
+};
+
+// True for the flavour of the tuned kernels: Ewald + LJ potential shift, no
+// LJ-PME, rvdw == rcoulomb.
+inline bool isDefaultFlavour(const nbnxm_interaction_const_t& ic)
+{
+    return ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT && !ic.ljPme && ic.rvdw == ic.rcoulomb;
+}
+
+NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic);
+
+// Every flavour (one wave per entry, FP32 or, with NBNXM_ACCUM_FP64, FP64).
+hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream);
+
 // Launch tuning chosen per size bucket by the autotuner.
 struct NbLaunchConfig
 {

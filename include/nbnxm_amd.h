@@ -248,7 +248,12 @@ int nbnxm_search_fill_ljcomb(const nbnxm_search_t* s, const int* type, const flo
 typedef struct nbnxm_gpu nbnxm_gpu_t;
 
 /* gpu_init (src/gromacs/nbnxm/nbnxm_gpu_data_mgmt.cpp:442): parameters, atom
- * data (as gpu_init_atomdata) and shift vectors when nbat->shiftVec is set. */
+ * data (as gpu_init_atomdata) and shift vectors when nbat->shiftVec is set.
+ * The flavour comes from ic: Ewald + LJ potential shift with rvdw = rcoulomb
+ * runs the tuned kernels; every other flavour (reaction field, plain cut-off,
+ * twin cut-off, force / potential switch, LJ-PME) runs the generic flavour
+ * kernel (nbnxm_cuda.cu:171-254 selects among the same set).  LJ-PME needs
+ * nbat->type and nbat->nbfp for every vdwType (grid C6 = sqrt(6 C6_ii 6 C6_jj)). */
 int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
                    const nbnxm_pairlist_params_t* listParams,
                    const nbnxm_atomdata_t* nbat,
@@ -300,7 +305,8 @@ int nbnxm_gpu_x_to_nbat_x(nbnxm_gpu_t* nb, const float* d_x);
 int nbnxm_gpu_add_nbat_f_to_f(nbnxm_gpu_t* nb, float* d_f, const float* d_fToAdd, int accumulate);
 /* gpu_min_ci_balanced (gpu_data_mgmt.h:119). */
 int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb);
-/* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): always 1 here. */
+/* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): 1 for Ewald (always the
+ * analytical correction here), 0 for reaction field / plain cut-off. */
 int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb);
 /* Downloads the per-group masks (cjPacked imask and the rlistOuter mask,
  * gpu_plist::imask); no counterpart in the reference, used by tests. */
@@ -412,9 +418,9 @@ int nbnxm_detect_comb_rule(int numTypes, const double* nbfp, double tol);
 /* nblib::CoulombType (api/nblib/include/nblib/kerneloptions.h:73-79). */
 enum nbnxm_coulomb_type
 {
-    NBNXM_COULOMB_PME = 0,           /* Ewald real space, analytical correction (supported) */
-    NBNXM_COULOMB_CUTOFF = 1,        /* not supported by this kernel */
-    NBNXM_COULOMB_REACTION_FIELD = 2 /* not supported by this kernel */
+    NBNXM_COULOMB_PME = 0,           /* Ewald real space, analytical correction, no potential shift */
+    NBNXM_COULOMB_CUTOFF = 1,        /* plain cut-off: k_rf = 0, c_rf = 1/rc */
+    NBNXM_COULOMB_REACTION_FIELD = 2 /* reaction field with epsilon_rf = 1: the same constants */
 };
 
 /* Energy terms of the output, in NonBondedEnergyTerms order
@@ -425,13 +431,16 @@ enum nbnxm_coulomb_type
 /* nblib::NBKernelOptions (api/nblib/include/nblib/kerneloptions.h:84-102),
  * the fields that apply to the GPU kernel.  The nblib defaults are
  * pairlistCutoff 1.0, coulombType PME, analytical Ewald, Ewald rtol 1e-5,
- * epsilon_r 1 (api/nblib/nbnxmsetuphelpers.cpp:219-272). */
+ * epsilon_r 1; the interaction constants follow createInteractionConst
+ * (api/nblib/nbnxmsetuphelpers.cpp:219-276): LJ potential shift, PME without
+ * the Coulomb potential shift, cut-off and reaction field with
+ * epsilon_rf = 1. */
 typedef struct nbnxm_kernel_options
 {
     int useGpu;                /* must be nonzero: the kernel runs on the GPU only */
     int numOpenMPThreads;      /* host threads for the search (0 = hardware) */
     double pairlistCutoff;     /* rc = rvdw = rcoulomb; also the list radius unless rlist > 0 */
-    int coulombType;           /* enum nbnxm_coulomb_type: PME only */
+    int coulombType;           /* enum nbnxm_coulomb_type */
     int useTabulatedEwaldCorr; /* must be 0: the kernel is analytical */
     double ewaldRtol;          /* 0 -> 1e-5 */
     double epsilonR;           /* 0 -> 1 */

@@ -32,7 +32,12 @@ def water9k_oracle(water9k):
     real = atoms.atom_indices >= 0
     xq = atoms.xq.reshape(-1, 4)[:, :3].astype(np.float64)
     vir = oracle.single_sum_virial(xq[real], f[real], atoms.shift_vec, fs)
-    return dict(f=atoms.to_original_order(f, water9k.n), elj=elj, eel=eel, vir=vir, atoms=atoms, plist=plist, ic=ic)
+    # nblib's calculator has no Coulomb potential shift (nbnxmsetuphelpers.cpp:262-270).
+    ic_nblib = nb.interaction_const(1.0, 1e-5)
+    ic_nblib.sh_ewald = 0.0
+    _, _, _, eel_nblib = oracle.kernel(plist, atoms, ic_nblib, True, "f64")
+    return dict(f=atoms.to_original_order(f, water9k.n), elj=elj, eel=eel, eel_nblib=eel_nblib, vir=vir,
+                atoms=atoms, plist=plist, ic=ic)
 
 
 def test_calculator_water9k(water9k, water9k_oracle):
@@ -45,7 +50,7 @@ def test_calculator_water9k(water9k, water9k_oracle):
     ref = water9k_oracle
     assert rel_rms(f, ref["f"]) <= 5e-6
     assert en[1] == pytest.approx(ref["elj"], rel=1e-5)
-    assert en[0] == pytest.approx(ref["eel"], rel=1e-5)
+    assert en[0] == pytest.approx(ref["eel_nblib"], rel=1e-5)
     assert en[2:].tolist() == [0.0, 0.0, 0.0]
     assert np.abs(vir - ref["vir"]).max() <= 1e-5 * np.abs(ref["vir"]).max()
     # Forces are added to the output buffer (atomdata_add_nbat_f_to_f); the
@@ -68,7 +73,7 @@ def test_calculator_fp64_mode(water9k, water9k_oracle):
     ref = water9k_oracle
     # FP64 on both sides; the kernel reads FP32 coordinates and parameters like the oracle.
     assert rel_rms(f, ref["f"]) <= 1e-9
-    assert en[0] == pytest.approx(ref["eel"], rel=1e-9)
+    assert en[0] == pytest.approx(ref["eel_nblib"], rel=1e-9)
     assert np.abs(vir - ref["vir"]).max() <= 1e-8 * np.abs(ref["vir"]).max()
 
 
@@ -88,7 +93,29 @@ def test_calculator_golden(golden, rule):
     ref = np.array(golden["cases"][case]["forces"])
     assert np.abs(f - ref).max() < REF_FORCE_TOL
     assert abs(en[1] - golden["cases"][case]["vdw_energy"]) < REF_FORCE_TOL
+    # The refdata carry the Ewald potential shift, which nblib leaves out: the
+    # difference is sh_ewald * sum of qq over the pairs within the cut-off.
     assert abs(en[0] - golden["cases"][case]["coulomb_energy"]) < 10 * REF_FORCE_TOL
+
+
+@pytest.mark.parametrize("coulomb", ["cutoff", "rf"])
+def test_calculator_cutoff_and_rf(water9k, coulomb):
+    """nblib's Cutoff and ReactionField types: both run the reaction-field
+    kernel with k_rf = 0, c_rf = 1/rc (calc_rffac with epsilon_rf = 1)."""
+    announce(f"calculator water9k {coulomb}")
+    ct = {"cutoff": nb._lib.COULOMB_CUTOFF, "rf": nb._lib.COULOMB_REACTION_FIELD}[coulomb]
+    opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1, coulombType=ct)
+    calc = nb.ForceCalculator(water9k.types, nb_params(water9k), water9k.q, water9k.excl_ranges,
+                              water9k.excl_elements, opt)
+    x = calc.update_pairlist(water9k.x.copy(), water9k.box)
+    f, vir, en = calc.compute(x, water9k.box, virial=True, energies=True)
+    ic = nb.interaction_const_ex(1.0, coulomb="rf", epsilon_rf=1.0)
+    assert ic.k_rf == 0.0 and ic.c_rf == pytest.approx(1.0)
+    search, plist, atoms = nb.build(water9k, 1.1, nb.VDW_CUT_COMB_GEOM)
+    f64, fs64, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    assert rel_rms(f, atoms.to_original_order(f64, water9k.n)) <= 5e-6
+    assert en[0] == pytest.approx(eel64, rel=2e-5)
+    assert en[1] == pytest.approx(elj64, rel=1e-5)
 
 
 def test_gpu_task_virial(water9k, water9k_oracle):

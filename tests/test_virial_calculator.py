@@ -129,8 +129,8 @@ def test_force_calculator_argument_checks():
     params = np.stack([6.0 * w.c6c12[..., 0], 12.0 * w.c6c12[..., 1]], axis=-1)
     with pytest.raises(nb.NbnxmError, match="GPU only"):
         nb.ForceCalculator(w.types, params, w.q, w.excl_ranges, w.excl_elements, _options(useGpu=False))
-    with pytest.raises(nb.NbnxmError, match="PME"):
-        nb.ForceCalculator(w.types, params, w.q, options=_options(coulombType=_lib.COULOMB_REACTION_FIELD))
+    with pytest.raises(nb.NbnxmError, match="Coulomb type"):
+        nb.ForceCalculator(w.types, params, w.q, options=_options(coulombType=3))
     with pytest.raises(nb.NbnxmError, match="analytical"):
         nb.ForceCalculator(w.types, params, w.q, options=_options(useTabulatedEwaldCorr=True))
     bad = w.types.copy()
