@@ -160,10 +160,6 @@ extern "C" int nbnxm_force_calculator_create(int numParticles,
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "force_calculator_create: unknown Coulomb type");
     }
-    if (options->useTabulatedEwaldCorr)
-    {
-        return fail(NBNXM_ERR_UNSUPPORTED, "force_calculator_create: only the analytical Ewald correction is implemented");
-    }
     if (!(options->pairlistCutoff > 0) || (options->rlist != 0 && options->rlist < options->pairlistCutoff))
     {
         return fail(NBNXM_ERR_INVALID, "force_calculator_create: pairlistCutoff must be > 0 and rlist 0 or >= pairlistCutoff");
@@ -234,6 +230,7 @@ extern "C" int nbnxm_force_calculator_create(int numParticles,
     io.ewaldRtol = options->ewaldRtol > 0 ? options->ewaldRtol : 1e-5;
     io.epsilonR = options->epsilonR > 0 ? options->epsilonR : 1.0;
     io.epsilonRf = 1.0;
+    io.tabulatedEwald = (options->coulombType == NBNXM_COULOMB_PME && options->useTabulatedEwaldCorr) ? 1 : 0;
     int rc = nbnxm_init_interaction_const_ex(&io, &c->ic);
     if (rc != NBNXM_OK)
     {

@@ -94,6 +94,10 @@ int validateFlavour(const nbnxm_interaction_const_t* ic)
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: LJ-PME supports the potential-shift modifier only");
     }
+    if (ic->tabulatedEwald && (ic->eeltype != NBNXM_ELEC_EWALD || !(ic->ewaldcoeff_q > 0)))
+    {
+        return fail(NBNXM_ERR_INVALID, "interaction constants: the tabulated correction needs Ewald with beta > 0");
+    }
     if (!(ic->rcoulomb > 0) || !(ic->rvdw > 0) || ic->rvdw > ic->rcoulomb)
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: need 0 < rvdw <= rcoulomb");
@@ -187,6 +191,8 @@ struct nbnxm_gpu
     NbFlavourParams flavour;       // used unless the flavour is the tuned default
     DeviceBuffer<float> gridC6;    // LJ-PME: sqrt(6 C6_ii) per type
     DeviceBuffer<double> gridC6D;
+    DeviceBuffer<float> ewaldTable; // tabulated Ewald: F(r), one padding entry
+    DeviceBuffer<double> ewaldTableD;
 
     int nsci = 0;
     int ncjPacked = 0;
@@ -363,6 +369,84 @@ double nbnxm_calc_ewaldcoeff_lj(double rc, double rtol)
     return beta;
 }
 
+int nbnxm_ewald_correction_table(double beta, double rc, float* tableF, double* tableFD, int capacity, double* scale)
+{
+    if (!(beta > 0) || !(rc > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "ewald_correction_table: need beta > 0 and rc > 0");
+    }
+    // Scale: the larger of the FP32 force-accuracy and energy-accuracy
+    // spacings for a quadratic spline of erf(x)/x (max third derivative
+    // 1.0522), energy tolerance 0.1 erfc(beta rc), in the mixed-precision
+    // build's FP32 arithmetic for beta and the result.
+    const float betaF = static_cast<float>(beta);
+    const double floatEps = static_cast<double>(1.19209290e-07F);
+    const double thirdDerivMax = 1.0522;
+    const float erfcRc = std::erfc(betaF * static_cast<float>(rc));
+    const double funcTol = std::max(0.1 * erfcRc, floatEps);
+    const double scDeriv = std::sqrt(thirdDerivMax / (6 * 4 * floatEps * betaF)) * betaF;
+    const double scFunc = std::cbrt(thirdDerivMax / (6 * 12 * std::sqrt(3.0) * funcTol)) * betaF;
+    const float scaleF = static_cast<float>(std::max(scDeriv, scFunc));
+    const int numPoints = static_cast<int>(static_cast<float>(rc) * scaleF) + 2;
+    if (scale)
+    {
+        *scale = scaleF;
+    }
+    if (!tableF && !tableFD)
+    {
+        return numPoints;
+    }
+    if (capacity < numPoints)
+    {
+        return fail(NBNXM_ERR_INVALID, "ewald_correction_table: capacity below the number of points");
+    }
+    // Quadratic splines through v at the interval ends whose derivative matches
+    // the secant plus half the mid-point second difference; the force at each
+    // point is the average spline derivative of its two intervals.  erf(br)/r
+    // is finite on the whole range, so the out-of-range continuation of the
+    // general generator never applies.
+    const double dx = 1.0 / static_cast<double>(scaleF);
+    const auto v = [betaF](double r) {
+        return r == 0 ? static_cast<double>(betaF) * 2 / std::sqrt(M_PI) : std::erf(static_cast<double>(betaF) * r) / r;
+    };
+    std::vector<double> f(numPoints, 0.0);
+    double dc = 0;
+    for (int i = numPoints - 1; i >= 1; i--)
+    {
+        const double xr0 = i * dx;
+        const double vr0 = v(xr0);
+        const double vr1 = v((i - 1) * dx);
+        double a2dx = (vr0 + vr1 - 2 * v(xr0 - 0.5 * dx)) / (0.25 * dx);
+        dc = (vr0 - vr1) / dx + 0.5 * a2dx;
+        if (i == numPoints - 1)
+        {
+            f[i] = -dc;
+        }
+        else
+        {
+            f[i] += -0.5 * dc;
+        }
+        const double a0 = vr0;
+        const double a1 = dc;
+        a2dx = (a1 * dx + vr1 - a0) * 2 / dx;
+        dc = a1 - a2dx;
+        f[i - 1] = -0.5 * dc;
+    }
+    f[0] *= 2;
+    for (int i = 0; i < numPoints; i++)
+    {
+        if (tableF)
+        {
+            tableF[i] = static_cast<float>(f[i]);
+        }
+        if (tableFD)
+        {
+            tableFD[i] = f[i];
+        }
+    }
+    return numPoints;
+}
+
 int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* o, nbnxm_interaction_const_t* ic)
 {
     if (!o || !ic)
@@ -446,6 +530,11 @@ int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* o, nbnxm_interacti
 
     // Coulomb: Ewald with the potential shift (interaction_const.cpp:64-120), or
     // reaction field / plain cut-off (:320-345 with calc_rffac, rf_util.cpp:50-62).
+    if (o->tabulatedEwald && o->eeltype != NBNXM_ELEC_EWALD)
+    {
+        return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: tabulated correction needs Ewald");
+    }
+    c.tabulatedEwald = o->tabulatedEwald ? 1 : 0;
     if (o->eeltype == NBNXM_ELEC_EWALD)
     {
         c.ewaldcoeff_q = nbnxm_calc_ewaldcoeff_q(rc, o->ewaldRtol);
@@ -639,6 +728,31 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
     nb->ic = *ic;
     nb->flavour = flavourParams(*ic);
     nb->listParams = *listParams;
+    if (ic->tabulatedEwald)
+    {
+        // init_ewald_coulomb_force_table (nbnxm_gpu_data_mgmt.cpp:83-95); the
+        // extra last entry repeats the end point so that r a rounding step
+        // above rc still reads inside the table.
+        double scale = 0;
+        const int np = nbnxm_ewald_correction_table(ic->ewaldcoeff_q, ic->rcoulomb, nullptr, nullptr, 0, &scale);
+        if (np < 2)
+        {
+            return np < 0 ? np : fail(NBNXM_ERR_INVALID, "gpu_init: Ewald table too short");
+        }
+        std::vector<float> t(np + 1);
+        std::vector<double> td(np + 1);
+        nbnxm_ewald_correction_table(ic->ewaldcoeff_q, ic->rcoulomb, t.data(), td.data(), np, &scale);
+        t[np] = t[np - 1];
+        td[np] = td[np - 1];
+        NBNXM_HIP_CHECK(nb->ewaldTable.resize(np + 1));
+        NBNXM_HIP_CHECK(nb->ewaldTableD.resize(np + 1));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTable.ptr, t.data(), sizeof(float) * (np + 1), hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTableD.ptr, td.data(), sizeof(double) * (np + 1), hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+        nb->flavour.ewaldTableScale = scale;
+        nb->flavour.ewaldTable = nb->ewaldTable.ptr;
+        nb->flavour.ewaldTableD = nb->ewaldTableD.ptr;
+    }
     nb->accumulation = accumulation;
     rc = uploadParams(nb.get());
     if (rc != NBNXM_OK)

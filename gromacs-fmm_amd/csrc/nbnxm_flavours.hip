@@ -1,8 +1,8 @@
 // gfx950 kernel for every interaction flavour other than the tuned default
 // (Ewald analytical + LJ cut-off with potential shift, rvdw = rcoulomb):
-// reaction field / plain cut-off Coulomb, the twin cut-off LJ check, LJ force
-// switch, LJ potential switch and the LJ-PME grid correction with the
-// geometric grid rule.  These are the kernels the reference generates from one
+// tabulated Ewald, reaction field / plain cut-off Coulomb, the twin cut-off LJ
+// check, LJ force switch, LJ potential switch and the LJ-PME grid correction
+// with the geometric grid rule.  These are the kernels the reference generates from one
 // body with preprocessor flavours (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh,
 // selected in nbnxm_cuda.cu:171-254); here one body takes the flavour as
 // wave-uniform kernel arguments, so a flavour costs scalar branches, not a
@@ -37,6 +37,7 @@ struct FlavourConsts
     Real cpot_rep, cpot_disp, rvdw_switch;
     Real disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
     Real lje_coeff2, lje_coeff6_6, sh_lj_ewald;
+    Real tableScale;
 
     __device__ explicit FlavourConsts(const NbFlavourParams& p)
     {
@@ -65,6 +66,7 @@ struct FlavourConsts
         lje_coeff2 = ljc * ljc;
         lje_coeff6_6 = lje_coeff2 * lje_coeff2 * lje_coeff2 * (Real(1) / Real(6));
         sh_lj_ewald = static_cast<Real>(p.sh_lj_ewald);
+        tableScale = static_cast<Real>(p.ewaldTableScale);
     }
 };
 
@@ -167,6 +169,39 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
         if constexpr (computeEnergy)
         {
             eEl += qq * (int_bit * inv_r + Real(0.5) * k.two_k_rf * r2 - k.c_rf);
+        }
+    }
+    else if (flav.tabulatedEwald)
+    {
+        // interpolate_coulomb_force_r (nbnxm_cuda_kernel_utils.cuh:345-385): linear
+        // interpolation of the F(r) table, lerp as two fused multiply-adds.
+        const Real normalized = k.tableScale * (r2 * inv_r);
+        const int index = static_cast<int>(normalized);
+        const Real fraction = normalized - static_cast<Real>(index);
+        Real corr;
+        if constexpr (std::is_same_v<Real, float>)
+        {
+            const float d0 = flav.ewaldTable[index];
+            const float d1 = flav.ewaldTable[index + 1];
+            corr = __builtin_fmaf(fraction, d1, __builtin_fmaf(-fraction, d0, d0));
+        }
+        else
+        {
+            const double d0 = flav.ewaldTableD[index];
+            const double d1 = flav.ewaldTableD[index + 1];
+            corr = fma(fraction, d1, fma(-fraction, d0, d0));
+        }
+        F_invr += qq * (int_bit * inv_r2 - corr) * inv_r;
+        if constexpr (computeEnergy)
+        {
+            if constexpr (std::is_same_v<Real, float>)
+            {
+                eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+            }
+            else
+            {
+                eEl += qq * (inv_r * (int_bit - erf(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+            }
         }
     }
     else if constexpr (std::is_same_v<Real, float>)
@@ -496,12 +531,14 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
     p.sw_c5 = ic.vdw_switch_c5;
     p.ewaldcoeff_lj = ic.ewaldcoeff_lj;
     p.sh_lj_ewald = ic.sh_lj_ewald;
+    p.tabulatedEwald = ic.tabulatedEwald;
     return p;
 }
 
 hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream)
 {
-    if (vdwType < NBNXM_VDW_CUT_TYPE_TABLE || vdwType > NBNXM_VDW_CUT_COMB_LB || (flav.ljPme && (!flav.gridC6 || !flav.gridC6D || !a.atomTypes || !a.nbfp)))
+    if (vdwType < NBNXM_VDW_CUT_TYPE_TABLE || vdwType > NBNXM_VDW_CUT_COMB_LB || (flav.ljPme && (!flav.gridC6 || !flav.gridC6D || !a.atomTypes || !a.nbfp))
+        || (flav.tabulatedEwald && (!flav.ewaldTable || !flav.ewaldTableD || !(flav.ewaldTableScale > 0))))
     {
         return hipErrorInvalidValue;
     }

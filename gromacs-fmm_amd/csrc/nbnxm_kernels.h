@@ -61,13 +61,19 @@ struct NbFlavourParams
     double ewaldcoeff_lj = 0, sh_lj_ewald = 0;
     const float* gridC6 = nullptr;
     const double* gridC6D = nullptr;
+    // Tabulated Ewald correction: F(r) table with one padding entry.
+    int tabulatedEwald = 0;
+    double ewaldTableScale = 0;
+    const float* ewaldTable = nullptr;
+    const double* ewaldTableD = nullptr;
 };
 
-// True for the flavour of the tuned kernels: Ewald + LJ potential shift, no
-// LJ-PME, rvdw == rcoulomb.
+// True for the flavour of the tuned kernels: analytical Ewald + LJ potential
+// shift, no LJ-PME, rvdw == rcoulomb.
 inline bool isDefaultFlavour(const nbnxm_interaction_const_t& ic)
 {
-    return ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT && !ic.ljPme && ic.rvdw == ic.rcoulomb;
+    return ic.eeltype == NBNXM_ELEC_EWALD && !ic.tabulatedEwald && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT && !ic.ljPme
+           && ic.rvdw == ic.rcoulomb;
 }
 
 NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic);

@@ -59,7 +59,8 @@ IC_FIELDS = [("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double),
              ("dispersion_shift_c2", ctypes.c_double), ("dispersion_shift_c3", ctypes.c_double),
              ("repulsion_shift_c2", ctypes.c_double), ("repulsion_shift_c3", ctypes.c_double),
              ("vdw_switch_c3", ctypes.c_double), ("vdw_switch_c4", ctypes.c_double),
-             ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double)]
+             ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double),
+             ("tabulatedEwald", ctypes.c_int)]
 
 
 class InteractionConst(ctypes.Structure):
@@ -77,7 +78,8 @@ class IcOptions(ctypes.Structure):
     _fields_ = [("eeltype", ctypes.c_int), ("vdwModifier", ctypes.c_int), ("ljPme", ctypes.c_int),
                 ("rcoulomb", ctypes.c_double), ("rvdw", ctypes.c_double), ("rvdw_switch", ctypes.c_double),
                 ("ewaldRtol", ctypes.c_double), ("ewaldRtolLj", ctypes.c_double),
-                ("epsilonR", ctypes.c_double), ("epsilonRf", ctypes.c_double), ("plainCutoff", ctypes.c_int)]
+                ("epsilonR", ctypes.c_double), ("epsilonRf", ctypes.c_double), ("plainCutoff", ctypes.c_int),
+                ("tabulatedEwald", ctypes.c_int)]
 
 
 class PairlistParams(ctypes.Structure):
@@ -115,6 +117,7 @@ SIGNATURES = [
     ("nbnxm_init_interaction_const", _c_int, [_c_double, _c_double, _c_double, _PTR(InteractionConst)]),
     ("nbnxm_init_interaction_const_ex", _c_int, [_PTR(IcOptions), _PTR(InteractionConst)]),
     ("nbnxm_calc_ewaldcoeff_lj", _c_double, [_c_double, _c_double]),
+    ("nbnxm_ewald_correction_table", _c_int, [_c_double, _c_double, _vp, _vp, _c_int, _PTR(ctypes.c_double)]),
     ("nbnxm_calc_shifts", _c_int, [_vp, _vp]),
     ("nbnxm_search_create", _vp, []),
     ("nbnxm_search_free", None, [_vp]),

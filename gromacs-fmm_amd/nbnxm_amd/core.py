@@ -25,10 +25,11 @@ def interaction_const(rc=1.0, ewald_rtol=1e-5, epsilon_r=1.0):
 
 def interaction_const_ex(rcoulomb, coulomb="ewald", vdw_modifier="potshift", lj_pme=False, rvdw=0.0,
                          rvdw_switch=0.0, ewald_rtol=1e-5, ewald_rtol_lj=1e-3, epsilon_r=1.0, epsilon_rf=0.0,
-                         plain_cutoff=False):
+                         plain_cutoff=False, tabulated=False):
     """Any kernel flavour: coulomb "ewald" | "rf"; vdw_modifier "potshift" |
     "forceswitch" | "potswitch"; lj_pme adds the geometric LJ-PME grid
-    correction; rvdw < rcoulomb gives the twin cut-off."""
+    correction; rvdw < rcoulomb gives the twin cut-off; tabulated selects the
+    tabulated Ewald correction."""
     o = _lib.IcOptions()
     o.eeltype = {"ewald": _lib.ELEC_EWALD, "rf": _lib.ELEC_RF}[coulomb]
     o.vdwModifier = {"potshift": _lib.VDW_MOD_POTSHIFT, "forceswitch": _lib.VDW_MOD_FORCESWITCH,
@@ -42,6 +43,7 @@ def interaction_const_ex(rcoulomb, coulomb="ewald", vdw_modifier="potshift", lj_
     o.epsilonR = epsilon_r
     o.epsilonRf = epsilon_rf
     o.plainCutoff = int(bool(plain_cutoff))
+    o.tabulatedEwald = int(bool(tabulated))
     ic = InteractionConst()
     check(_lib.load().nbnxm_init_interaction_const_ex(ctypes.byref(o), ctypes.byref(ic)), "init_interaction_const_ex")
     return ic
@@ -49,8 +51,23 @@ def interaction_const_ex(rcoulomb, coulomb="ewald", vdw_modifier="potshift", lj_
 
 def is_default_flavour(ic):
     """Ewald + LJ cut-off with potential shift and rvdw == rcoulomb."""
-    return (ic.eeltype == _lib.ELEC_EWALD and ic.vdwModifier == _lib.VDW_MOD_POTSHIFT and not ic.ljPme
-            and ic.rvdw == ic.rcoulomb)
+    return (ic.eeltype == _lib.ELEC_EWALD and not ic.tabulatedEwald and ic.vdwModifier == _lib.VDW_MOD_POTSHIFT
+            and not ic.ljPme and ic.rvdw == ic.rcoulomb)
+
+
+def ewald_correction_table(beta, rc):
+    """(scale, F table FP32, F table FP64) of the tabulated Ewald correction."""
+    lib = _lib.load()
+    scale = ctypes.c_double(0.0)
+    n = lib.nbnxm_ewald_correction_table(beta, rc, None, None, 0, ctypes.byref(scale))
+    if n < 0:
+        check(n, "ewald_correction_table")
+    t = np.zeros(n, dtype=np.float32)
+    td = np.zeros(n, dtype=np.float64)
+    rc2 = lib.nbnxm_ewald_correction_table(beta, rc, ptr(t), ptr(td), n, ctypes.byref(scale))
+    if rc2 != n:
+        check(rc2 if rc2 < 0 else -1, "ewald_correction_table")
+    return scale.value, t, td
 
 
 def ewald_coeff(rc, rtol):

@@ -144,6 +144,10 @@ typedef struct nbnxm_interaction_const
     double vdw_switch_c3, vdw_switch_c4, vdw_switch_c5; /* potential_switch_constants, :232-246 */
     double k_rf;              /* reaction-field coefficient (calc_rffac, src/gromacs/mdlib/rf_util.cpp:50-62) */
     double c_rf;              /* reaction-field potential shift */
+    /* Nonzero: the Ewald correction force comes from the quadratic-spline table
+     * of init_interaction_const_tables (src/gromacs/mdlib/forcerec.cpp:586-640),
+     * ElecType::EwaldTab of nbnxm_gpu_data_mgmt.cpp:128-175; zero: analytical. */
+    int tabulatedEwald;
 } nbnxm_interaction_const_t;
 
 /* Flavour choices for nbnxm_init_interaction_const_ex (t_inputrec fields). */
@@ -160,6 +164,7 @@ typedef struct nbnxm_ic_options
     double epsilonR;
     double epsilonRf;   /* reaction field: 0 = infinity */
     int plainCutoff;    /* with NBNXM_ELEC_RF: plain cut-off (k_rf = 0, c_rf = 1/rc) */
+    int tabulatedEwald; /* with NBNXM_ELEC_EWALD: tabulated instead of analytical correction */
 } nbnxm_ic_options_t;
 
 /* Pair-list parameters (src/gromacs/nbnxm/pairlistparams.h, PairlistParams). */
@@ -204,6 +209,15 @@ int nbnxm_init_interaction_const(double rc, double ewaldRtol, double epsilonR, n
 int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* options, nbnxm_interaction_const_t* ic);
 /* calc_ewaldcoeff_lj (src/gromacs/ewald/ewald_utils.cpp:73-112). */
 double nbnxm_calc_ewaldcoeff_lj(double rc, double rtol);
+/* Coulomb Ewald correction force table of the tabulated kernel: the scale of
+ * ewald_spline3_table_scale (src/gromacs/tables/forcetable.cpp:296-365, FP32
+ * accuracy targets) and tableF of generateEwaldCorrectionTables
+ * (:139-276) for v(r) = erf(beta r)/r over [0, rc], int(rc*scale) + 2
+ * points.  Returns the number of points (tableF may be NULL to query it;
+ * capacity is the room in tableF or tableFD), or a negative error code.  The
+ * table values are computed in FP64 and stored in FP32 (tableF) and / or FP64
+ * (tableFD); *scale gets the FP32-rounded scale the kernel uses. */
+int nbnxm_ewald_correction_table(double beta, double rc, float* tableF, double* tableFD, int capacity, double* scale);
 /* Shift vectors of a rectangular box (calc_shifts, src/gromacs/pbcutil/pbc.cpp:1218-1233). */
 int nbnxm_calc_shifts(const double box[3], float* shiftVec /* 3*45 */);
 
@@ -441,7 +455,7 @@ typedef struct nbnxm_kernel_options
     int numOpenMPThreads;      /* host threads for the search (0 = hardware) */
     double pairlistCutoff;     /* rc = rvdw = rcoulomb; also the list radius unless rlist > 0 */
     int coulombType;           /* enum nbnxm_coulomb_type */
-    int useTabulatedEwaldCorr; /* must be 0: the kernel is analytical */
+    int useTabulatedEwaldCorr; /* PME: tabulated instead of analytical Ewald correction */
     double ewaldRtol;          /* 0 -> 1e-5 */
     double epsilonR;           /* 0 -> 1 */
     double rlist;              /* 0 -> pairlistCutoff (nblib); larger gives a buffered list */

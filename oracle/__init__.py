@@ -44,7 +44,8 @@ class _IC(ctypes.Structure):
                 ("dispersion_shift_c2", ctypes.c_double), ("dispersion_shift_c3", ctypes.c_double),
                 ("repulsion_shift_c2", ctypes.c_double), ("repulsion_shift_c3", ctypes.c_double),
                 ("vdw_switch_c3", ctypes.c_double), ("vdw_switch_c4", ctypes.c_double),
-                ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double)]
+                ("vdw_switch_c5", ctypes.c_double), ("k_rf", ctypes.c_double), ("c_rf", ctypes.c_double),
+                ("tabulatedEwald", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -60,6 +61,8 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     lib.oracle_calc_ewaldcoeff_q.restype = ctypes.c_double
     lib.oracle_calc_ewaldcoeff_q.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.oracle_ewald_table.restype = ctypes.c_int
+    lib.oracle_ewald_table.argtypes = [ctypes.c_double, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp]
     lib.oracle_pmecorrF.restype = ctypes.c_float
     lib.oracle_pmecorrF.argtypes = [ctypes.c_float]
     for name in ("oracle_kernel_f32", "oracle_kernel_f64"):
@@ -125,6 +128,18 @@ def _atoms(a):
 
 def ewald_coeff(rc, rtol):
     return load().oracle_calc_ewaldcoeff_q(rc, rtol)
+
+
+def ewald_table(beta, rc):
+    """(scale, F table FP32, F table FP64) of the tabulated Ewald correction."""
+    lib = load()
+    scale = ctypes.c_double(0.0)
+    n = lib.oracle_ewald_table(beta, rc, None, None, 0, ctypes.byref(scale))
+    t = np.zeros(n, dtype=np.float32)
+    td = np.zeros(n, dtype=np.float64)
+    if lib.oracle_ewald_table(beta, rc, _p(t), _p(td), n, ctypes.byref(scale)) != n:
+        raise RuntimeError("oracle ewald table failed")
+    return scale.value, t, td
 
 
 def pmecorrF(z2):

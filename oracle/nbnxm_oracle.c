@@ -20,6 +20,7 @@
 
 /* c_one4PiEps0 is not needed here: epsfac arrives in the interaction constants. */
 static const double c_oneSqrtPi = 0.56418958354775628695; /* 1/sqrt(pi) */
+static const double c_pi = 3.14159265358979323846;
 
 /* Restates calc_ewaldcoeff_q (src/gromacs/ewald/ewald_utils.cpp:43-71):
  * double beta until erfc(beta*rc) <= rtol, then bisect for i+60 steps. */
@@ -90,6 +91,125 @@ float oracle_pmecorrF(float z2)
     return polyFN0 * polyFD0;
 }
 
+/* Restates ewald_spline3_table_scale for Coulomb (src/gromacs/tables/forcetable.cpp:296-340
+ * with spline3_table_scale :280-294, mixed-precision FP32 for beta, the
+ * erfc argument and the result) and generateEwaldCorrectionTables (:139-276,
+ * the F column) for v(r) = erf(beta r)/r, the point count of
+ * init_ewald_f_table (src/gromacs/mdlib/forcerec.cpp:586-624).  Returns the
+ * number of points; tables may be NULL to query it. */
+int oracle_ewald_table(double beta, double rc, float* tableF, double* tableFD, int capacity, double* scaleOut)
+{
+    const float betaF = (float)beta;
+    const double derivTol = (double)1.19209290e-07F;
+    const double xd3 = 1.0522;
+    const float erfcRc = erfcf(betaF * (float)rc);
+    double funcTol = 0.1 * erfcRc;
+    if (funcTol < derivTol)
+    {
+        funcTol = derivTol;
+    }
+    const double scDeriv = sqrt(xd3 / (6 * 4 * derivTol * betaF)) * betaF;
+    const double scFunc = cbrt(xd3 / (6 * 12 * sqrt(3.0) * funcTol)) * betaF;
+    const float scale = (float)(scDeriv > scFunc ? scDeriv : scFunc);
+    const int n = (int)((float)rc * scale) + 2;
+    if (scaleOut)
+    {
+        *scaleOut = scale;
+    }
+    if (!tableF && !tableFD)
+    {
+        return n;
+    }
+    if (capacity < n)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const double dx = 1.0 / (double)scale;
+    double* f = (double*)calloc((size_t)n, sizeof(double));
+    if (!f)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    double dc = 0.0;
+    for (int i = n - 1; i >= 1; i--)
+    {
+        const double x0 = i * dx;
+        const double x1 = (i - 1) * dx;
+        const double xm = x0 - 0.5 * dx;
+        const double v0 = erf(betaF * x0) / x0;
+        const double v1 = (i - 1 == 0) ? betaF * 2 / sqrt(c_pi) : erf(betaF * x1) / x1;
+        const double vm = erf(betaF * xm) / xm;
+        /* mean second derivative times dx over [x1, x0]; spline slope at x0 */
+        double a2dx = (v0 + v1 - 2 * vm) / (0.25 * dx);
+        dc = (v0 - v1) / dx + 0.5 * a2dx;
+        if (i == n - 1)
+        {
+            f[i] = -dc;
+        }
+        else
+        {
+            f[i] += -0.5 * dc;
+        }
+        /* spline a0 + a1 (x - x0) + a2/2 (x - x0)^2 through v0, v1 with slope dc
+         * at x0; its slope at x1 starts the next interval */
+        a2dx = (dc * dx + v1 - v0) * 2 / dx;
+        dc = dc - a2dx;
+        f[i - 1] = -0.5 * dc;
+    }
+    f[0] *= 2;
+    for (int i = 0; i < n; i++)
+    {
+        if (tableF)
+        {
+            tableF[i] = (float)f[i];
+        }
+        if (tableFD)
+        {
+            tableFD[i] = f[i];
+        }
+    }
+    free(f);
+    return n;
+}
+
+/* The tables of one evaluation, with one padding entry repeating the end point. */
+typedef struct
+{
+    int n;
+    double scale;
+    float* f32;
+    double* f64;
+} ewald_table_t;
+
+static int ewald_table_make(const nbnxm_interaction_const_t* ic, ewald_table_t* t)
+{
+    memset(t, 0, sizeof(*t));
+    if (!ic->tabulatedEwald)
+    {
+        return NBNXM_OK;
+    }
+    const int n = oracle_ewald_table(ic->ewaldcoeff_q, ic->rcoulomb, NULL, NULL, 0, &t->scale);
+    t->f32 = (float*)malloc(sizeof(float) * (size_t)(n + 1));
+    t->f64 = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    if (!t->f32 || !t->f64 || oracle_ewald_table(ic->ewaldcoeff_q, ic->rcoulomb, t->f32, t->f64, n, &t->scale) != n)
+    {
+        free(t->f32);
+        free(t->f64);
+        return NBNXM_ERR_INVALID;
+    }
+    t->f32[n] = t->f32[n - 1];
+    t->f64[n] = t->f64[n - 1];
+    t->n = n;
+    return NBNXM_OK;
+}
+
+static void ewald_table_free(ewald_table_t* t)
+{
+    free(t->f32);
+    free(t->f64);
+    memset(t, 0, sizeof(*t));
+}
+
 static int check_inputs(const nbnxm_pairlist_gpu_t* list, const nbnxm_atomdata_t* nbat)
 {
     if (!list || !nbat || !nbat->xq || !nbat->shiftVec)
@@ -131,12 +251,18 @@ typedef struct
     float cpot_rep, cpot_disp, rvdw_switch;
     float disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
     float lje_coeff2, lje_coeff6_6, sh_lj_ewald;
+    int tabulated;
+    float tabScale;
+    const float* tab;
 } flav32_t;
 
-static flav32_t flav32_from_ic(const nbnxm_interaction_const_t* ic)
+static flav32_t flav32_from_ic(const nbnxm_interaction_const_t* ic, const ewald_table_t* tab)
 {
     flav32_t p;
     memset(&p, 0, sizeof(p));
+    p.tabulated = ic->tabulatedEwald;
+    p.tabScale = (float)tab->scale;
+    p.tab = tab->f32;
     p.eeltype = ic->eeltype;
     p.vdwModifier = ic->vdwModifier;
     p.ljPme = ic->ljPme;
@@ -286,7 +412,19 @@ static float pair_f32(const flav32_t* p,
     }
     else
     {
-        F_invr += qq * (int_bit * inv_r2 * inv_r + oracle_pmecorrF(p->beta2 * r2) * p->beta3);
+        if (p->tabulated)
+        {
+            /* interpolate_coulomb_force_r (nbnxm_cuda_kernel_utils.cuh:345-385, :640-645) */
+            const float normalized = p->tabScale * (r2 * inv_r);
+            const int index = (int)normalized;
+            const float fraction = normalized - (float)index;
+            const float corr = fmaf(fraction, p->tab[index + 1], fmaf(-fraction, p->tab[index], p->tab[index]));
+            F_invr += qq * (int_bit * inv_r2 - corr) * inv_r;
+        }
+        else
+        {
+            F_invr += qq * (int_bit * inv_r2 * inv_r + oracle_pmecorrF(p->beta2 * r2) * p->beta3);
+        }
         if (computeEnergy)
         {
             *eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * p->beta)) - int_bit * p->sh_ewald);
@@ -299,6 +437,7 @@ static float pair_f32(const flav32_t* p,
  * 2 beta/sqrt(pi) exp(-beta^2 r^2)/r^2 - erf(beta r)/r^3, the exact function
  * pmecorrF approximates. */
 static double pair_f64(const nbnxm_interaction_const_t* ic,
+                       const ewald_table_t* tab,
                        double r2,
                        double int_bit,
                        double qq,
@@ -366,7 +505,18 @@ static double pair_f64(const nbnxm_interaction_const_t* ic,
     {
         const double br = ic->ewaldcoeff_q * r;
         const double erfbr = erf(br);
-        F_invr += qq * (int_bit * inv_r2 * inv_r + 2.0 * ic->ewaldcoeff_q * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
+        if (ic->tabulatedEwald)
+        {
+            const double normalized = tab->scale * r;
+            const int index = (int)normalized;
+            const double fraction = normalized - index;
+            const double corr = fma(fraction, tab->f64[index + 1], fma(-fraction, tab->f64[index], tab->f64[index]));
+            F_invr += qq * (int_bit * inv_r2 - corr) * inv_r;
+        }
+        else
+        {
+            F_invr += qq * (int_bit * inv_r2 * inv_r + 2.0 * ic->ewaldcoeff_q * c_oneSqrtPi * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
+        }
         *eEl += qq * (inv_r * (int_bit - erfbr) - int_bit * ic->sh_ewald);
     }
     return F_invr;
@@ -387,6 +537,10 @@ static int check_flavour(const nbnxm_atomdata_t* nbat, const nbnxm_interaction_c
         return NBNXM_ERR_INVALID;
     }
     if (ic->vdwModifier < NBNXM_VDW_MOD_POTSHIFT || ic->vdwModifier > NBNXM_VDW_MOD_POTSWITCH)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    if (ic->tabulatedEwald && (ic->eeltype != NBNXM_ELEC_EWALD || !(ic->ewaldcoeff_q > 0)))
     {
         return NBNXM_ERR_INVALID;
     }
@@ -432,10 +586,16 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
     {
         return NBNXM_ERR_INVALID;
     }
-    const flav32_t p = flav32_from_ic(ic);
+    ewald_table_t tab;
+    if (ewald_table_make(ic, &tab) != NBNXM_OK)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    const flav32_t p = flav32_from_ic(ic, &tab);
     float* gridC6 = p.ljPme ? grid_c6_f32(nbat) : NULL;
     if (p.ljPme && !gridC6)
     {
+        ewald_table_free(&tab);
         return NBNXM_ERR_INVALID;
     }
     const float* xq = nbat->xq;
@@ -572,6 +732,7 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
         vcTot += E_el;
     }
     free(gridC6);
+    ewald_table_free(&tab);
     if (computeEnergy)
     {
         if (Vvdw)
@@ -597,6 +758,11 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
 {
     int rc = check_inputs(list, nbat);
     if (rc != NBNXM_OK || !ic || !f || !fshift || check_flavour(nbat, ic) != NBNXM_OK)
+    {
+        return NBNXM_ERR_INVALID;
+    }
+    ewald_table_t tab;
+    if (ewald_table_make(ic, &tab) != NBNXM_OK)
     {
         return NBNXM_ERR_INVALID;
     }
@@ -699,7 +865,7 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                                               * (double)nbat->nbfp[2 * (nbat->type[aj] * (ntypes + 1))]);
                             }
                             double elj = 0.0, eel = 0.0;
-                            const double F_invr = pair_f64(ic, r2, int_bit, qi * qj, c6, c12, c6grid, &elj, &eel);
+                            const double F_invr = pair_f64(ic, &tab, r2, int_bit, qi * qj, c6, c12, c6grid, &elj, &eel);
                             if (computeEnergy)
                             {
                                 vvdwTot += elj;
@@ -729,6 +895,7 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
             }
         }
     }
+    ewald_table_free(&tab);
     if (computeEnergy)
     {
         if (Vvdw)
@@ -791,11 +958,17 @@ static int bruteforce_f64(int n,
     const double epsfac = ic->epsfac;
     double vvdw = 0.0, vc = 0.0;
     long long npairs = 0;
+    ewald_table_t tab;
+    if (ewald_table_make(ic, &tab) != NBNXM_OK)
+    {
+        return NBNXM_ERR_INVALID;
+    }
 
     /* Wrap into the unit cell so every separation is within (-L, L). */
     double* xw = (double*)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
     if (!xw)
     {
+        ewald_table_free(&tab);
         return NBNXM_ERR_INVALID;
     }
     for (int i = 0; i < n; i++)
@@ -842,7 +1015,7 @@ static int bruteforce_f64(int n,
                         {
                             npairs++;
                         }
-                        const double fr = pair_f64(ic, r2, int_bit, qq, c6, c12, c6grid, &vvdw, &vc);
+                        const double fr = pair_f64(ic, &tab, r2, int_bit, qq, c6, c12, c6grid, &vvdw, &vc);
                         f[3 * i + 0] += fr * dx;
                         f[3 * i + 1] += fr * dy;
                         f[3 * i + 2] += fr * dz;
@@ -873,6 +1046,7 @@ static int bruteforce_f64(int n,
         vvdw += sumc6 / 12.0 * (b2 * b2 * b2 / 6.0);
     }
     free(xw);
+    ewald_table_free(&tab);
     if (Vvdw)
     {
         *Vvdw += vvdw;

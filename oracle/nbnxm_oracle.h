@@ -8,6 +8,9 @@
  *
  * Pinning status (what fixes each function independently of its own formula):
  *   oracle_calc_ewaldcoeff_q   closed form erfc(beta*rc) == rtol; beta(1.0,1e-5)
+ *   oracle_ewald_table         interpolated table against the exact derivative
+ *                              -d/dr erf(beta r)/r (FP32 spline error bound); the
+ *                              12 Ewald golden cases with the tabulated kernel
  *   oracle_pmecorrF            closed form 2exp(-z^2)/(sqrt(pi)z^2) - erf(z)/z^3, |err|<1e-6
  *   oracle_kernel_f64          refdata XML golden forces/energies (243-atom box) of
  *                              all 18 Coulomb x LJ flavours, brute force all-image
@@ -37,6 +40,11 @@ extern "C" {
 #endif
 
 double oracle_calc_ewaldcoeff_q(double rc, double rtol);
+
+/* Ewald correction force table of the tabulated kernel (forcetable.cpp:139-365,
+ * forcerec.cpp:586-624): returns the point count, fills tableF / tableFD when
+ * given (capacity entries), *scale = the FP32 table scale. */
+int oracle_ewald_table(double beta, double rc, float* tableF, double* tableFD, int capacity, double* scale);
 float oracle_pmecorrF(float z2);
 
 /* Every kernel flavour follows ic: Ewald or reaction field (plain cut-off is

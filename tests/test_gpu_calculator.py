@@ -98,6 +98,20 @@ def test_calculator_golden(golden, rule):
     assert abs(en[0] - golden["cases"][case]["coulomb_energy"]) < 10 * REF_FORCE_TOL
 
 
+def test_calculator_tabulated_ewald(water9k, water9k_oracle):
+    """useTabulatedEwaldCorr: the tabulated correction of the same Ewald
+    interaction agrees with the analytical FP64 oracle to the table accuracy."""
+    announce("calculator water9k tabulated Ewald")
+    opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1, useTabulatedEwaldCorr=True)
+    calc = nb.ForceCalculator(water9k.types, nb_params(water9k), water9k.q, water9k.excl_ranges,
+                              water9k.excl_elements, opt)
+    x = calc.update_pairlist(water9k.x.copy(), water9k.box)
+    f, vir, en = calc.compute(x, water9k.box, virial=True, energies=True)
+    ref = water9k_oracle
+    assert rel_rms(f, ref["f"]) <= 5e-6
+    assert en[0] == pytest.approx(ref["eel_nblib"], rel=1e-5)
+
+
 @pytest.mark.parametrize("coulomb", ["cutoff", "rf"])
 def test_calculator_cutoff_and_rf(water9k, coulomb):
     """nblib's Cutoff and ReactionField types: both run the reaction-field

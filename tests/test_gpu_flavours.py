@@ -58,7 +58,33 @@ def test_golden_flavours(golden, case, accumulation):
     assert np.abs(f2 - f).max() < F_VS_VF_TOL
 
 
+TABULATED_CASES = [c for c in GOLDEN_ALL if not c.startswith("CoulombReactionField")]
+
+
+@pytest.mark.parametrize("case", TABULATED_CASES)
+@pytest.mark.parametrize("accumulation", ACCUMULATIONS)
+def test_golden_tabulated_ewald(golden, case, accumulation):
+    """Tabulated Ewald kernels against the shared refdata (kernel_test.cpp:478-489)
+    and the oracle's own table interpolation."""
+    announce(f"tabulated golden {case} accumulation={accumulation}")
+    system, ic, vdw, tol = golden_case(golden, case)
+    ic.tabulatedEwald = 1
+    search, plist, atoms = nb.build(system, 0.9, vdw)
+    task, f, elj, eel, fshift = _run(ic, atoms, plist, 0.9, accumulation)
+    ref = np.array(golden["cases"][case]["forces"])
+    assert np.abs(atoms.to_original_order(f, system.n) - ref).max() < tol
+    assert abs(elj - golden["cases"][case]["vdw_energy"]) < tol
+    assert abs(eel - golden["cases"][case]["coulomb_energy"]) < 10 * tol
+    precision = "f64" if accumulation == nb.ACCUM_FP64 else "f32"
+    fo, _, eljo, eelo = oracle.kernel(plist, atoms, ic, True, precision)
+    real = atoms.atom_indices >= 0
+    assert rel_rms(f[real], fo[real]) < (1e-10 if precision == "f64" else 2e-6)
+    assert task._lib.nbnxm_gpu_is_kernel_ewald_analytical(task._h) == 1
+
+
 FLAVOURS = {
+    "tabulated": dict(tabulated=True),
+    "tabulated_twin_forceswitch": dict(tabulated=True, rvdw=0.9, vdw_modifier="forceswitch", rvdw_switch=0.7),
     "rf": dict(coulomb="rf"),
     "rf_epsrf78": dict(coulomb="rf", epsilon_rf=78.0),
     "cutoff": dict(coulomb="rf", plain_cutoff=True),

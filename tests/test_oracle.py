@@ -325,3 +325,47 @@ def test_lj_pme_closed_form():
     r = 0.3
     _, elj, _, _ = oracle.bruteforce(_two_atoms(r, q=(0.0, 0.0), c6=c6, c12=0.0, excluded=True), ic)
     assert elj - self_e == pytest.approx(c6 * (1 - g(beta * r)) / r ** 6, rel=1e-10)
+
+
+def test_ewald_table_against_exact():
+    """The tabulated Ewald correction interpolates -d/dr erf(beta r)/r with the
+    FP32 spline error eps * beta^3 of its spacing (forcetable.cpp:178-181,
+    :280-294), and the library builds the same table."""
+    for rc, rtol in [(1.0, 1e-5), (0.9, 1e-6), (1.2, 1e-4)]:
+        beta = oracle.ewald_coeff(rc, rtol)
+        scale, t32, t64 = oracle.ewald_table(beta, rc)
+        assert len(t32) == int(np.float32(rc) * np.float32(scale)) + 2
+        b = float(np.float32(beta))
+        r = np.linspace(0.02, rc, 5001)
+        x = scale * r
+        i = x.astype(int)
+        fr = x - i
+        interp = t64[i] + fr * (t64[i + 1] - t64[i])
+        exact = erf(b * r) / r ** 2 - 2 * b / math.sqrt(math.pi) * np.exp(-(b * r) ** 2) / r
+        eps = 2.0 ** -23
+        assert np.abs(interp - exact).max() < 2 * eps * b ** 3
+        s2, p32, p64 = nb.ewald_correction_table(beta, rc)
+        assert s2 == scale
+        assert np.array_equal(p32, t32)
+        assert np.allclose(p64, t64, rtol=1e-14, atol=1e-14)
+
+
+TABULATED_CASES = [c for c in GOLDEN_ALL if not c.startswith("CoulombReactionField")]
+
+
+@pytest.mark.parametrize("case", TABULATED_CASES)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_golden_tabulated_ewald(golden, case, precision):
+    """The tabulated Ewald kernels share the analytical cases' refdata
+    (kernel_test.cpp:478-489); the FP32 tolerance is the same (:663-698)."""
+    system, ic, vdw, tol = golden_case(golden, case)
+    ic.tabulatedEwald = 1
+    search, plist, atoms = nb.build(system, 0.9, vdw)
+    f, fs, elj, eel = oracle.kernel(plist, atoms, ic, True, precision)
+    ref = np.array(golden["cases"][case]["forces"])
+    fo = atoms.to_original_order(f, system.n)
+    assert np.abs(fo - ref).max() < tol
+    assert abs(elj - golden["cases"][case]["vdw_energy"]) < tol
+    assert abs(eel - golden["cases"][case]["coulomb_energy"]) < 10 * tol
+    fb, eljb, eelb, _ = oracle.bruteforce(system, ic)
+    assert np.abs(fb - ref).max() < tol

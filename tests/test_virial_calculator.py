@@ -131,8 +131,6 @@ def test_force_calculator_argument_checks():
         nb.ForceCalculator(w.types, params, w.q, w.excl_ranges, w.excl_elements, _options(useGpu=False))
     with pytest.raises(nb.NbnxmError, match="Coulomb type"):
         nb.ForceCalculator(w.types, params, w.q, options=_options(coulombType=3))
-    with pytest.raises(nb.NbnxmError, match="analytical"):
-        nb.ForceCalculator(w.types, params, w.q, options=_options(useTabulatedEwaldCorr=True))
     bad = w.types.copy()
     bad[0] = 7
     with pytest.raises(nb.NbnxmError, match="type id"):
