@@ -23,6 +23,11 @@ import numpy as np  # noqa: E402
 
 METRIC = "Gpairs/s, batched fixed-point FP32 vs plain launch, at stated err vs FP64"
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
+# HBM bytes per batched F+E launch of this workload and schedule: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy) from two rocprofv3 --pmc
+# runs, profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
+# inside the timed run, so the measured figure is carried here.
+TRAFFIC_BYTES_PER_LAUNCH = 1.886e9
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_VF = 107       # Ewald + LJ force+energy (src/gromacs/gmxlib/nrnb.cpp:90-103)
 RC, RLIST_OUTER, RLIST_INNER, EWALD_RTOL, SIGMA = 1.0, 1.1, 1.04, 1e-5, 0.002
@@ -245,7 +250,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "note": "compute roofline of the FP32 vector pipe (the kernel issues VALU, not MFMA); "
                     "algorithmic flops = useful pairs x 107 (Ewald+LJ F+E weight)",
         },
