@@ -1102,8 +1102,14 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     {
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
     }
+    else if (nb->accumulation != NBNXM_ACCUM_FP64 && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE))
+    {
+        // Other FP32 flavours: the tuned list walk with the generic pair interaction.
+        NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream, &nb->flavour));
+    }
     else
     {
+        // FP64 mode, and LJ-PME with a combination-rule kernel: the standalone flavour kernel.
         NBNXM_HIP_CHECK(launchNonbondedFlavour(a, nb->flavour, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->stream));
     }
     nb->lastComputeEnergy = computeEnergy != 0;

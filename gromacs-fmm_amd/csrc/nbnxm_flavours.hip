@@ -15,6 +15,7 @@
 // is the FP64 kernel with exact erf/exp.  Forces go to int64 fixed point or,
 // for FP32 with float accumulation, to float atomics.
 #include "device_common.h"
+#include "flavour_math.h"
 #include "nbnxm_kernels.h"
 
 #include <type_traits>
@@ -26,218 +27,6 @@ namespace
 {
 
 using namespace dev;
-
-// Flavour constants in the kernel's arithmetic type, derived the way the
-// reference kernel derives them from its FP32 parameter block
-// (nbnxm_gpu_data_mgmt.cpp:93-159, nbnxm_cuda_kernel.cuh:311-376).
-template<typename Real>
-struct FlavourConsts
-{
-    Real rcoulomb_sq, rvdw_sq, epsfac, beta, beta2, beta3, sh_ewald, two_k_rf, c_rf;
-    Real cpot_rep, cpot_disp, rvdw_switch;
-    Real disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
-    Real lje_coeff2, lje_coeff6_6, sh_lj_ewald;
-    Real tableScale;
-
-    __device__ explicit FlavourConsts(const NbFlavourParams& p)
-    {
-        const Real rc = static_cast<Real>(p.rcoulomb);
-        const Real rvdw = static_cast<Real>(p.rvdw);
-        rcoulomb_sq = rc * rc;
-        rvdw_sq = rvdw * rvdw;
-        epsfac = static_cast<Real>(p.epsfac);
-        beta = static_cast<Real>(p.beta);
-        beta2 = beta * beta;
-        beta3 = beta * beta * beta;
-        sh_ewald = static_cast<Real>(p.sh_ewald);
-        two_k_rf = static_cast<Real>(2.0 * p.k_rf);
-        c_rf = static_cast<Real>(p.c_rf);
-        cpot_rep = static_cast<Real>(p.cpot_rep);
-        cpot_disp = static_cast<Real>(p.cpot_disp);
-        rvdw_switch = static_cast<Real>(p.rvdw_switch);
-        disp_c2 = static_cast<Real>(p.disp_c2);
-        disp_c3 = static_cast<Real>(p.disp_c3);
-        rep_c2 = static_cast<Real>(p.rep_c2);
-        rep_c3 = static_cast<Real>(p.rep_c3);
-        sw_c3 = static_cast<Real>(p.sw_c3);
-        sw_c4 = static_cast<Real>(p.sw_c4);
-        sw_c5 = static_cast<Real>(p.sw_c5);
-        const Real ljc = static_cast<Real>(p.ewaldcoeff_lj);
-        lje_coeff2 = ljc * ljc;
-        lje_coeff6_6 = lje_coeff2 * lje_coeff2 * lje_coeff2 * (Real(1) / Real(6));
-        sh_lj_ewald = static_cast<Real>(p.sh_lj_ewald);
-        tableScale = static_cast<Real>(p.ewaldTableScale);
-    }
-};
-
-template<typename Real>
-__device__ __forceinline__ Real rsqrtR(Real x)
-{
-    if constexpr (std::is_same_v<Real, float>)
-    {
-        return __builtin_amdgcn_rsqf(x);
-    }
-    else
-    {
-        return 1.0 / sqrt(x);
-    }
-}
-
-template<typename Real>
-__device__ __forceinline__ Real expR(Real x)
-{
-    if constexpr (std::is_same_v<Real, float>)
-    {
-        return __expf(x);
-    }
-    else
-    {
-        return exp(x);
-    }
-}
-
-// One pair, in the order of the reference kernel body (nbnxm_cuda_kernel.cuh:514-665):
-// clamp r^2, LJ with the PotShift constants, force switch (unmasked c6/c12 as
-// the reference GPU kernels use them), LJ-PME grid part, potential switch, LJ
-// cut-off check of twin cut-offs, Coulomb.  Returns F/r; adds energies.
-template<typename Real, bool computeEnergy>
-__device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
-                                          const FlavourConsts<Real>& k,
-                                          Real r2,
-                                          Real int_bit,
-                                          Real qq,
-                                          Real c6,
-                                          Real c12,
-                                          Real c6grid,
-                                          Real& eLJ,
-                                          Real& eEl)
-{
-    constexpr Real oneSixth = Real(1) / Real(6);
-    constexpr Real oneTwelveth = Real(1) / Real(12);
-    r2 = r2 > Real(NBNXM_MIN_DISTANCE_SQUARED) ? r2 : Real(NBNXM_MIN_DISTANCE_SQUARED);
-    const Real inv_r = rsqrtR(r2);
-    const Real inv_r2 = inv_r * inv_r;
-    const Real inv_r6_nm = inv_r2 * inv_r2 * inv_r2;
-    const Real inv_r6 = inv_r6_nm * int_bit;
-    Real F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-    Real E_lj_p = int_bit * (c12 * (inv_r6 * inv_r6 + k.cpot_rep) * oneTwelveth - c6 * (inv_r6 + k.cpot_disp) * oneSixth);
-    if (flav.vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
-    {
-        const Real r = r2 * inv_r;
-        Real rs = r - k.rvdw_switch;
-        rs = rs >= Real(0) ? rs : Real(0);
-        F_invr += (-c6 * (k.disp_c2 + k.disp_c3 * rs) + c12 * (k.rep_c2 + k.rep_c3 * rs)) * rs * rs * inv_r;
-        if constexpr (computeEnergy)
-        {
-            E_lj_p += (c6 * (k.disp_c2 / Real(3) + k.disp_c3 / Real(4) * rs) - c12 * (k.rep_c2 / Real(3) + k.rep_c3 / Real(4) * rs))
-                      * rs * rs * rs;
-        }
-    }
-    if (flav.ljPme)
-    {
-        const Real cr2 = k.lje_coeff2 * r2;
-        const Real expmcr2 = expR(-cr2);
-        const Real poly = Real(1) + cr2 + Real(0.5) * cr2 * cr2;
-        F_invr += c6grid * (inv_r6_nm - expmcr2 * (inv_r6_nm * poly + k.lje_coeff6_6)) * inv_r2;
-        if constexpr (computeEnergy)
-        {
-            E_lj_p += oneSixth * c6grid * (inv_r6_nm * (Real(1) - expmcr2 * poly) + k.sh_lj_ewald * int_bit);
-        }
-    }
-    if (flav.vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
-    {
-        const Real r = r2 * inv_r;
-        Real rs = r - k.rvdw_switch;
-        rs = rs >= Real(0) ? rs : Real(0);
-        const Real sw = Real(1) + (k.sw_c3 + (k.sw_c4 + k.sw_c5 * rs) * rs) * rs * rs * rs;
-        const Real dsw = (Real(3) * k.sw_c3 + (Real(4) * k.sw_c4 + Real(5) * k.sw_c5 * rs) * rs) * rs * rs;
-        F_invr = F_invr * sw - inv_r * E_lj_p * dsw;
-        E_lj_p *= sw;
-    }
-    if (flav.vdwCutoffCheck && !(r2 < k.rvdw_sq))
-    {
-        F_invr = Real(0);
-        E_lj_p = Real(0);
-    }
-    if constexpr (computeEnergy)
-    {
-        eLJ += E_lj_p;
-    }
-    if (flav.eeltype == NBNXM_ELEC_RF)
-    {
-        F_invr += qq * (int_bit * inv_r2 * inv_r - k.two_k_rf);
-        if constexpr (computeEnergy)
-        {
-            eEl += qq * (int_bit * inv_r + Real(0.5) * k.two_k_rf * r2 - k.c_rf);
-        }
-    }
-    else if (flav.tabulatedEwald)
-    {
-        // interpolate_coulomb_force_r (nbnxm_cuda_kernel_utils.cuh:345-385): linear
-        // interpolation of the F(r) table, lerp as two fused multiply-adds.
-        const Real normalized = k.tableScale * (r2 * inv_r);
-        const int index = static_cast<int>(normalized);
-        const Real fraction = normalized - static_cast<Real>(index);
-        Real corr;
-        if constexpr (std::is_same_v<Real, float>)
-        {
-            const float d0 = flav.ewaldTable[index];
-            const float d1 = flav.ewaldTable[index + 1];
-            corr = __builtin_fmaf(fraction, d1, __builtin_fmaf(-fraction, d0, d0));
-        }
-        else
-        {
-            const double d0 = flav.ewaldTableD[index];
-            const double d1 = flav.ewaldTableD[index + 1];
-            corr = fma(fraction, d1, fma(-fraction, d0, d0));
-        }
-        F_invr += qq * (int_bit * inv_r2 - corr) * inv_r;
-        if constexpr (computeEnergy)
-        {
-            if constexpr (std::is_same_v<Real, float>)
-            {
-                eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
-            }
-            else
-            {
-                eEl += qq * (inv_r * (int_bit - erf(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
-            }
-        }
-    }
-    else if constexpr (std::is_same_v<Real, float>)
-    {
-        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(k.beta2 * r2) * k.beta3);
-        if constexpr (computeEnergy)
-        {
-            eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
-        }
-    }
-    else
-    {
-        const Real br = k.beta * r2 * inv_r;
-        const Real erfbr = erf(br);
-        F_invr += qq * (int_bit * inv_r2 * inv_r + Real(2) * k.beta * Real(c_oneSqrtPi) * exp(-br * br) * inv_r2 - erfbr * inv_r2 * inv_r);
-        if constexpr (computeEnergy)
-        {
-            eEl += qq * (inv_r * (int_bit - erfbr) - int_bit * k.sh_ewald);
-        }
-    }
-    return F_invr;
-}
-
-// Per-type grid factor sqrt(6 C6_ii) of the geometric LJ-PME rule.
-template<typename Real>
-__device__ __forceinline__ Real gridC6(const NbFlavourParams& flav, int type)
-{
-    if constexpr (std::is_same_v<Real, float>)
-    {
-        return flav.gridC6[type];
-    }
-    else
-    {
-        return flav.gridC6D[type];
-    }
-}
 
 template<typename Real>
 __device__ __forceinline__ Real sumOverTi(Real v)

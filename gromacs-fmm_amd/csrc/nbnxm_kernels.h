@@ -87,7 +87,16 @@ struct NbLaunchConfig
     int wavesPerBlock = 4;
 };
 
-hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream);
+// flav == nullptr: the tuned flavour (per-system parameters from sysParams*);
+// otherwise the generic FP32 instantiation with the flavour of flav (single
+// parameter set, FLOAT or FIXED64 accumulation, LJ-PME with the type table).
+hipError_t launchNonbonded(const NbKernelArgs& a,
+                           int accumulation,
+                           int vdwType,
+                           bool computeEnergy,
+                           const NbLaunchConfig& cfg,
+                           hipStream_t stream,
+                           const NbFlavourParams* flav = nullptr);
 
 struct PruneArgs
 {

@@ -16,6 +16,7 @@
 //    per-entry slot, then summed per system in a fixed order by a second kernel.
 #include "nbnxm_kernels.h"
 #include "device_common.h"
+#include "flavour_math.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -108,10 +109,14 @@ __device__ __forceinline__ float erfcxPolyT(float t)
     return p;
 }
 
-// Force-only kernels fit 4 waves per SIMD (128 VGPRs); the energy kernels need
-// more registers and run at the occupancy their allocation allows.
-template<bool computeEnergy, int vdwType, int accumulation>
-__global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKernel(const NbKernelArgs a, int wavesPerBlock)
+// No occupancy floor: capping the force-only kernel at 128 VGPRs (4 waves per
+// SIMD) spills, and measured 25 % slower than its 3-wave allocation.
+// generic = false is the tuned flavour (analytical Ewald, LJ potential shift,
+// rvdw = rcoulomb); generic = true runs every other FP32 flavour through the
+// same list walk with the pair interaction of flavour_math.h, the flavour
+// taken from flav (wave-uniform, so its branches are scalar).
+template<bool computeEnergy, int vdwType, int accumulation, bool generic>
+__global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
 {
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7;
@@ -141,6 +146,7 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
     const float cpotRep = pB.x;
     const float cpotDisp = pB.y;
     const float4 sh = loadF4(asConst(a.shiftVec), s.shift);
+    [[maybe_unused]] const FlavourConsts<float> fk(flav);
 
     // i-atom coordinates and charges of all 8 clusters in registers; their LJ
     // parameters (type row offset or combination parameters) in per-wave LDS.
@@ -169,6 +175,17 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
         }
     }
 
+    // LJ-PME grid factors of the i-atoms (generic flavour with the type table).
+    [[maybe_unused]] float gci[c_ncps];
+    if constexpr (generic)
+    {
+#pragma unroll
+        for (int c = 0; c < c_ncps; c++)
+        {
+            gci[c] = flav.ljPme ? flav.gridC6[a.atomTypes[(s.sci * c_ncps + c) * c_cl + ti]] : 0.0f;
+        }
+    }
+
     float fciX[c_ncps], fciY[c_ncps], fciZ[c_ncps];
 #pragma unroll
     for (int c = 0; c < c_ncps; c++)
@@ -187,7 +204,31 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
         // added once per i-atom by the tj == 0 lanes.
         if (centralShift && s.cjPackedEnd > s.cjPackedBegin && cjList[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
         {
-            if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+            if constexpr (generic)
+            {
+                // Coulomb: -epsfac sum q^2 times beta/sqrt(pi) (Ewald) or c_rf/2
+                // (reaction field); LJ-PME: + sum 6C6_ii/12 beta_lj^6/6.
+                double q2 = 0.0, c6self = 0.0;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+                    const double q = loadF4(xqc, ai).w;
+                    q2 += q * q;
+                    if (flav.ljPme)
+                    {
+                        c6self += a.nbfp[a.atomTypes[ai] * (a.numTypes + 1)].x;
+                    }
+                }
+                const double selfFactor = (flav.eeltype == NBNXM_ELEC_RF) ? 0.5 * flav.c_rf : flav.beta * c_oneSqrtPi;
+                eElAcc -= static_cast<EAcc>(flav.epsfac * q2 * selfFactor);
+                if (flav.ljPme)
+                {
+                    const double b2 = flav.ewaldcoeff_lj * flav.ewaldcoeff_lj;
+                    eLJAcc += static_cast<EAcc>(c6self / 12.0 * (b2 * b2 * b2 / 6.0));
+                }
+            }
+            else if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
             {
                 float q2 = 0.0f;
 #pragma unroll
@@ -250,6 +291,7 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
     auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
         constexpr bool hasExcl = decltype(withExcl)::value;
         float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
+        [[maybe_unused]] float eLJg = 0.0f;
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)
         {
@@ -262,6 +304,11 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
             const float2 ljStaged = stageLj[jm * c_cl + tj];
             const int typej = __float_as_int(ljStaged.x);
             const float2 ljcj = ljStaged;
+            [[maybe_unused]] float gcj = 0.0f;
+            if constexpr (generic && vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+            {
+                gcj = flav.ljPme ? flav.gridC6[typej] : 0.0f;
+            }
             float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
             unsigned long long anyInRange = 0ull;
 #pragma unroll
@@ -325,7 +372,12 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                     const float inv_r2 = inv_r * inv_r;
                     const float qq = qi[c] * xqj.w;
                     float F_invr;
-                    if (hasExcl && exclHere)
+                    if constexpr (generic)
+                    {
+                        const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
+                        F_invr = pairForce<float, computeEnergy>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG);
+                    }
+                    else if (hasExcl && exclHere)
                     {
                         // The reference form with the interaction bit (:548-659).
                         const float int_bit = static_cast<float>((wexcl >> bit) & 1u);
@@ -406,7 +458,12 @@ __global__ __launch_bounds__(256, computeEnergy ? 1 : 4) void nbnxmNonbondedKern
                 pendMask |= 1u << jm;
             }
         }
-        if constexpr (computeEnergy)
+        if constexpr (computeEnergy && generic)
+        {
+            eLJAcc += static_cast<EAcc>(eLJg);
+            eElAcc += static_cast<EAcc>(eElG);
+        }
+        else if constexpr (computeEnergy)
         {
             const float eLJ = (eLJ12 + cpotRep * eSumC12 - 2.0f * cpotDisp * eSumC6) * (1.0f / 12.0f);
             eLJAcc += static_cast<EAcc>(eLJ);
@@ -1185,7 +1242,7 @@ __global__ void fillI32Kernel(int* p, size_t n, int value)
 }
 
 template<bool E, int V, int A>
-hipError_t launchNb(const NbKernelArgs& a, const NbLaunchConfig& cfg, hipStream_t stream)
+hipError_t launchNb(const NbKernelArgs& a, const NbFlavourParams* flav, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     const int wpb = cfg.wavesPerBlock;
     const int blocks = (a.nsci + wpb - 1) / wpb;
@@ -1197,47 +1254,61 @@ hipError_t launchNb(const NbKernelArgs& a, const NbLaunchConfig& cfg, hipStream_
     {
         hipLaunchKernelGGL((nbnxmNonbondedKernelFp64<E, V>), dim3(blocks), dim3(64 * wpb), 0, stream, a, wpb);
     }
+    else if (flav)
+    {
+        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, true>), dim3(blocks), dim3(64 * wpb), 0, stream, a, *flav, wpb);
+    }
     else
     {
-        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A>), dim3(blocks), dim3(64 * wpb), 0, stream, a, wpb);
+        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, false>), dim3(blocks), dim3(64 * wpb), 0, stream, a, NbFlavourParams{}, wpb);
     }
     return hipGetLastError();
 }
 
 template<bool E, int V>
-hipError_t launchNbAcc(const NbKernelArgs& a, int accumulation, const NbLaunchConfig& cfg, hipStream_t stream)
+hipError_t launchNbAcc(const NbKernelArgs& a, const NbFlavourParams* flav, int accumulation, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     switch (accumulation)
     {
-        case NBNXM_ACCUM_FLOAT: return launchNb<E, V, NBNXM_ACCUM_FLOAT>(a, cfg, stream);
-        case NBNXM_ACCUM_FIXED64: return launchNb<E, V, NBNXM_ACCUM_FIXED64>(a, cfg, stream);
-        case NBNXM_ACCUM_FP64: return launchNb<E, V, NBNXM_ACCUM_FP64>(a, cfg, stream);
+        case NBNXM_ACCUM_FLOAT: return launchNb<E, V, NBNXM_ACCUM_FLOAT>(a, flav, cfg, stream);
+        case NBNXM_ACCUM_FIXED64: return launchNb<E, V, NBNXM_ACCUM_FIXED64>(a, flav, cfg, stream);
+        case NBNXM_ACCUM_FP64: return flav ? hipErrorInvalidValue : launchNb<E, V, NBNXM_ACCUM_FP64>(a, nullptr, cfg, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 template<bool E>
-hipError_t launchNbVdw(const NbKernelArgs& a, int accumulation, int vdwType, const NbLaunchConfig& cfg, hipStream_t stream)
+hipError_t launchNbVdw(const NbKernelArgs& a, const NbFlavourParams* flav, int accumulation, int vdwType, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     switch (vdwType)
     {
-        case NBNXM_VDW_CUT_TYPE_TABLE: return launchNbAcc<E, NBNXM_VDW_CUT_TYPE_TABLE>(a, accumulation, cfg, stream);
-        case NBNXM_VDW_CUT_COMB_GEOM: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_GEOM>(a, accumulation, cfg, stream);
-        case NBNXM_VDW_CUT_COMB_LB: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_LB>(a, accumulation, cfg, stream);
+        case NBNXM_VDW_CUT_TYPE_TABLE: return launchNbAcc<E, NBNXM_VDW_CUT_TYPE_TABLE>(a, flav, accumulation, cfg, stream);
+        case NBNXM_VDW_CUT_COMB_GEOM: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_GEOM>(a, flav, accumulation, cfg, stream);
+        case NBNXM_VDW_CUT_COMB_LB: return launchNbAcc<E, NBNXM_VDW_CUT_COMB_LB>(a, flav, accumulation, cfg, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 } // namespace
 
-hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
+hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream,
+                           const NbFlavourParams* flav)
 {
     if (cfg.wavesPerBlock < 1 || cfg.wavesPerBlock > 4)
     {
         return hipErrorInvalidValue;
     }
-    return computeEnergy ? launchNbVdw<true>(a, accumulation, vdwType, cfg, stream)
-                         : launchNbVdw<false>(a, accumulation, vdwType, cfg, stream);
+    // The generic instantiation reads LJ-PME grid factors by type: type table only.
+    if (flav && flav->ljPme && (vdwType != NBNXM_VDW_CUT_TYPE_TABLE || !flav->gridC6))
+    {
+        return hipErrorInvalidValue;
+    }
+    if (flav && flav->tabulatedEwald && (!flav->ewaldTable || !(flav->ewaldTableScale > 0)))
+    {
+        return hipErrorInvalidValue;
+    }
+    return computeEnergy ? launchNbVdw<true>(a, flav, accumulation, vdwType, cfg, stream)
+                         : launchNbVdw<false>(a, flav, accumulation, vdwType, cfg, stream);
 }
 
 hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t stream)

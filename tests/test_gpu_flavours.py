@@ -128,6 +128,22 @@ def test_flavour_water3k(water3k, name):
     assert eeld == pytest.approx(eel64, rel=1e-10)
 
 
+@pytest.mark.parametrize("vdw", [nb.VDW_CUT_COMB_GEOM, nb.VDW_CUT_COMB_LB])
+def test_ljpme_with_combination_kernel(vdw):
+    """LJ-PME with a combination-rule LJ kernel: the grid part still reads the
+    types and the type table (the standalone flavour kernel path)."""
+    announce(f"ljpme comb vdw={vdw}")
+    s = nb.systems.water_box(1000, None, seed=43)
+    search, plist, atoms = nb.build(s, 1.1, vdw)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5, lj_pme=True, ewald_rtol_lj=1e-3)
+    f64, fs64, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    real = atoms.atom_indices >= 0
+    task, f, elj, eel, fshift = _run(ic, atoms, plist, 1.1, nb.ACCUM_FIXED64)
+    assert rel_rms(f[real], f64[real]) < 5e-6
+    assert elj == pytest.approx(elj64, rel=2e-5)
+    assert eel == pytest.approx(eel64, rel=2e-5)
+
+
 def test_flavour_is_kernel_ewald(water3k):
     s, plist, atoms = water3k
     for coulomb, expect in (("ewald", 1), ("rf", 0)):
