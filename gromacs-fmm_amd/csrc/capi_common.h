@@ -26,6 +26,10 @@ inline double one4PiEps0()
 int fail(int code, const std::string& msg);
 int hipFail(hipError_t e, const char* what);
 
+// The HIP stream of a GPU task (for callers inside the library that order
+// their own work against it).
+hipStream_t gpuTaskStream(const nbnxm_gpu_t* nb);
+
 #define NBNXM_HIP_CHECK(call)                          \
     do                                                 \
     {                                                  \

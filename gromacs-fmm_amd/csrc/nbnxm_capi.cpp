@@ -259,6 +259,11 @@ struct nbnxm_gpu
     }
 };
 
+hipStream_t nbnxm_amd::gpuTaskStream(const nbnxm_gpu_t* nb)
+{
+    return nb ? nb->stream : nullptr;
+}
+
 namespace
 {
 

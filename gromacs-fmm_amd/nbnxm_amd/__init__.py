@@ -13,11 +13,12 @@ from .core import (AtomData, GpuPairSearch, GpuTask, PairSearch, Pairlist, build
 from . import systems
 from .ensemble import MODE_BATCHED, MODE_PLAIN, Ensemble
 from .calculator import ForceCalculator, NBKernelOptions, detect_comb_rule
+from .verlet import NonbondedVerlet
 
 __all__ = [
     "ACCUM_FIXED64", "ACCUM_FLOAT", "ACCUM_FP64", "CENTRAL_SHIFT", "NUM_SHIFTS",
     "VDW_CUT_COMB_GEOM", "VDW_CUT_COMB_LB", "VDW_CUT_TYPE_TABLE", "NbnxmError", "load",
     "AtomData", "GpuPairSearch", "GpuTask", "PairSearch", "Pairlist", "build", "comb_params", "ewald_coeff",
     "ewald_coeff_lj", "ewald_correction_table", "interaction_const", "interaction_const_ex", "is_default_flavour", "nbfp_table", "systems", "_lib", "Ensemble", "MODE_BATCHED", "MODE_PLAIN",
-    "ForceCalculator", "NBKernelOptions", "detect_comb_rule",
+    "ForceCalculator", "NBKernelOptions", "detect_comb_rule", "NonbondedVerlet",
 ]

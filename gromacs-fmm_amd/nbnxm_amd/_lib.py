@@ -82,6 +82,11 @@ class IcOptions(ctypes.Structure):
                 ("tabulatedEwald", ctypes.c_int)]
 
 
+class VerletParams(ctypes.Structure):
+    _fields_ = [("nstlist", ctypes.c_int), ("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
+                ("nstlistPrune", ctypes.c_int), ("accumulation", ctypes.c_int)]
+
+
 class PairlistParams(ctypes.Structure):
     _fields_ = [("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
                 ("useDynamicPruning", ctypes.c_int), ("numRollingPruningParts", ctypes.c_int)]
@@ -174,6 +179,13 @@ SIGNATURES = [
     ("nbnxm_gpu_search_num_entries", _c_int, [_vp, _vp]),
     ("nbnxm_gpu_search_system_list", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("nbnxm_gpu_search_free", None, [_vp]),
+    ("nbnxm_verlet_create", _c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _PTR(InteractionConst), _PTR(VerletParams), _PTR(_vp)]),
+    ("nbnxm_verlet_step", _c_int, [_vp, ctypes.c_longlong, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int]),
+    ("nbnxm_verlet_get_energies", _c_int, [_vp, _PTR(_c_double), _PTR(_c_double)]),
+    ("nbnxm_verlet_get_virial", _c_int, [_vp, _vp]),
+    ("nbnxm_verlet_get_stats", _c_int, [_vp, _vp, _vp]),
+    ("nbnxm_verlet_get_list", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("nbnxm_verlet_free", None, [_vp]),
 ]
 
 COULOMB_PME = 0

@@ -526,6 +526,80 @@ int nbnxm_gpu_search_num_entries(const nbnxm_gpu_search_t* s, int64_t* counts);
 int nbnxm_gpu_search_system_list(const nbnxm_gpu_search_t* s, int system, nbnxm_sci_t* sci, nbnxm_cj_packed_t* cjPacked, nbnxm_excl_t* excl, int* atomIndices, int* sizes);
 void nbnxm_gpu_search_free(nbnxm_gpu_search_t* s);
 
+/* ---------------------------------------------------------------------------
+ * Nonbonded part of a device-resident MD step: the pair-search / list /
+ * kernel / force-reduction sequence of nonbonded_verlet_t as do_force drives
+ * it (src/gromacs/mdlib/sim_util.cpp:1619-1680, 925-946), with the
+ * reference's list lifetime and dynamic-pruning schedule
+ * (src/gromacs/nbnxm/pairlistsets.h:91-113, pairlist_tuning.cpp:505-600,
+ * 690-708, prunekerneldispatch.cpp:99-114) on one GPU, one domain.
+ * ------------------------------------------------------------------------- */
+typedef struct nbnxm_verlet nbnxm_verlet_t;
+
+/* PairlistParams of one domain (src/gromacs/nbnxm/pairlistparams.h:117-160). */
+typedef struct nbnxm_verlet_params
+{
+    int nstlist;         /* steps per outer list (inputrec nstlist); lifetime = nstlist - 1 */
+    double rlistOuter;   /* outer list radius, >= max(rcoulomb, rvdw) */
+    double rlistInner;   /* pruned radius in [cut-off, rlistOuter]; rlistOuter = no dynamic pruning */
+    int nstlistPrune;    /* steps between prunes of the whole list (even, >= 2) */
+    int accumulation;    /* enum nbnxm_accumulation */
+} nbnxm_verlet_params_t;
+
+/* Sets up the system: per-atom types (numTypes types, nbfp (6C6, 12C12)
+ * per type pair as nblib's createNonBondedParameters), charges, exclusions as
+ * list of lists (exclRanges[numAtoms+1] offsets into exclElements, both may
+ * be NULL), the interaction constants (any flavour) and the list parameters.
+ * Dynamic pruning is used when rlistInner < rlistOuter and nstlistPrune <
+ * nstlist - 1, with nstlistPrune / 2 rolling parts (pairlist_tuning.cpp:586-591,
+ * 690-708). */
+int nbnxm_verlet_create(int numAtoms,
+                        const int* types,
+                        int numTypes,
+                        const double* nbfp,
+                        const double* charges,
+                        const int* exclRanges,
+                        const int* exclElements,
+                        const nbnxm_interaction_const_t* ic,
+                        const nbnxm_verlet_params_t* params,
+                        nbnxm_verlet_t** out);
+/* The nonbonded work of step `step` on device arrays in atom order:
+ *  - on the first call, every nstlist steps and when the box changes: copies
+ *    d_x to the host, puts the atoms in the box IN PLACE (as do_md does on
+ *    search steps, d_x is rewritten with the wrapped coordinates), builds the
+ *    grid and the outer list (rlistOuter) and uploads it; the first kernel
+ *    of a fresh list prunes it to rlistInner;
+ *  - every step: d_x -> xq (nbnxn_gpu_x_to_nbat_x), the nonbonded kernel
+ *    (energies / shift forces and virial when asked), the rolling prune of
+ *    one part when isDynamicPruningStepGpu holds (age > 0, age < lifetime,
+ *    age even), and d_f (+)= nonbonded forces (accumulateForces selects add or
+ *    set).
+ * waitStream (a hipStream_t, may be NULL): the step's work starts after the
+ * work already queued on it (the coordinates' producer).  Returns after the
+ * step's device work has finished. */
+int nbnxm_verlet_step(nbnxm_verlet_t* v,
+                      long long step,
+                      const double box[3],
+                      float* d_x,
+                      void* waitStream,
+                      int computeEnergy,
+                      int computeVirial,
+                      float* d_f,
+                      int accumulateForces);
+/* Energies (LJ, Coulomb) and the virial (9, row-major, -1/2 sum x (x) f) of
+ * the last step that computed them. */
+int nbnxm_verlet_get_energies(const nbnxm_verlet_t* v, double* eLJ, double* eCoul);
+int nbnxm_verlet_get_virial(const nbnxm_verlet_t* v, double* virial);
+/* counts[0] searches, [1] rolling prune launches, [2] steps, [3] outer-list
+ * cluster pairs, [4] step of the last search; flags[0] dynamic pruning used,
+ * [1] rolling parts. */
+int nbnxm_verlet_get_stats(const nbnxm_verlet_t* v, long long* counts, int* flags);
+/* The current list with its inner (pruned) masks, its exclusion masks and
+ * the slot -> atom map, for inspection: sizes[4] receives the grid slots,
+ * entries, packed groups and exclusion masks; any output may be NULL. */
+int nbnxm_verlet_get_list(nbnxm_verlet_t* v, nbnxm_sci_t* sci, nbnxm_cj_packed_t* cjPacked, nbnxm_excl_t* excl, int* atomIndices, int* sizes);
+void nbnxm_verlet_free(nbnxm_verlet_t* v);
+
 #ifdef __cplusplus
 }
 #endif

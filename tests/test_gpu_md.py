@@ -1,0 +1,153 @@
+"""The nonbonded part of an MD loop on the GPU (nbnxm_verlet_*): search every
+nstlist steps, dynamic rolling prune, device buffer ops, inside a leapfrog
+integrator on device arrays.  Checked against the CPU oracle at the current
+positions and by energy conservation.  Calls go through the C ABI."""
+import math
+import types
+
+import numpy as np
+import pytest
+
+import nbnxm_amd as nb
+import oracle
+from conftest import announce
+from helpers import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+KB = 0.0083144626  # kJ/mol/K
+MASS = 39.948      # amu
+SIGMA, EPS = 0.34, 0.996
+
+
+def lj_fluid(n, density=21.0, charge=0.2, seed=7):
+    """n LJ atoms (argon-like) on a jittered cubic lattice, alternating
+    charges +-charge, one atom type."""
+    rng = np.random.default_rng(seed)
+    L = (n / density) ** (1.0 / 3.0)
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    grid = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    x = (grid + 0.5) * (L / m) + rng.uniform(-0.05, 0.05, size=(n, 3))
+    q = np.where(np.arange(n) % 2 == 0, charge, -charge)
+    c6 = 4 * EPS * SIGMA ** 6
+    c12 = 4 * EPS * SIGMA ** 12
+    params = np.array([[[6 * c6, 12 * c12]]])
+    system = nb.systems.System(box=np.array([L, L, L]), x=x, q=q, types=np.zeros(n, dtype=np.int32),
+                               c6c12=np.array([[[c6, c12]]]), excl_ranges=np.zeros(n + 1, dtype=np.int32),
+                               excl_elements=np.zeros(0, dtype=np.int32))
+    return system, params
+
+
+def oracle_atoms(x_atoms, q, sci, cj, excl, ai, box):
+    """Grid-order inputs of the oracle for the verlet's current list."""
+    ng = ai.size
+    xq = np.zeros((ng, 4), dtype=np.float32)
+    xq[:, :3] = -1.0e6
+    real = ai >= 0
+    xq[real, :3] = x_atoms[ai[real]]
+    xq[real, 3] = q[ai[real]]
+    sv = np.zeros(3 * nb.NUM_SHIFTS, dtype=np.float32)
+    nb.load().nbnxm_calc_shifts(nb._lib.ptr(np.ascontiguousarray(box, dtype=np.float64)), nb._lib.ptr(sv))
+    atoms = types.SimpleNamespace(num_atoms=ng, xq=xq.reshape(-1), num_types=2,
+                                  types=np.zeros(ng, dtype=np.int32), nbfp=np.zeros(8, dtype=np.float32),
+                                  vdw_type=nb.VDW_CUT_TYPE_TABLE, lj_comb=np.zeros(2 * ng, dtype=np.float32),
+                                  shift_vec=sv)
+    plist = types.SimpleNamespace(sci=sci, cj_packed=cj, excl=excl, rlist=0.0)
+    return plist, atoms
+
+
+class Leapfrog:
+    """v(t+dt/2) = v(t-dt/2) + f(t)/m dt; x(t+dt) = x(t) + v(t+dt/2) dt on the GPU."""
+
+    def __init__(self, torch, system, temperature, dt, seed=3):
+        self.torch = torch
+        rng = np.random.default_rng(seed)
+        v = rng.normal(0.0, math.sqrt(KB * temperature / MASS), size=system.x.shape)
+        v -= v.mean(axis=0)
+        self.x = torch.tensor(system.x, dtype=torch.float32, device="cuda").contiguous()
+        self.v = torch.tensor(v, dtype=torch.float32, device="cuda").contiguous()
+        self.f = torch.zeros_like(self.x)
+        self.dt = dt
+
+    def kinetic(self, v):
+        return 0.5 * MASS * float((v.double() ** 2).sum())
+
+    def advance(self):
+        v_old = self.v.clone()
+        self.v += self.f * (self.dt / MASS)
+        self.x += self.v * self.dt
+        return 0.5 * (self.kinetic(v_old) + self.kinetic(self.v))
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+@pytest.mark.parametrize("accumulation", [nb.ACCUM_FIXED64, nb.ACCUM_FLOAT])
+def test_md_list_and_pruning_cover_all_pairs(torch, accumulation):
+    """60 leapfrog steps with nstlist 20 and rolling pruning of the inner list:
+    before each search, the list in use holds every pair within the cut-off
+    and the forces and energies match the list-free FP64 sum at the current
+    positions."""
+    announce(f"md lj fluid accumulation={accumulation}")
+    system, params = lj_fluid(3000)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
+    ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
+                             rlist_inner=1.05, nstlist_prune=4, accumulation=accumulation)
+    md = Leapfrog(torch, system, 120.0, 0.004)
+    stream = torch.cuda.current_stream().cuda_stream
+    checked = 0
+    for step in range(60):
+        check = step % 20 == 19
+        ver.step(step, system.box, md.x.data_ptr(), md.f.data_ptr(), stream, compute_energy=check)
+        if check:
+            x_now = md.x.cpu().numpy().astype(np.float64)
+            cur = nb.systems.System(box=system.box, x=x_now, q=system.q, types=system.types,
+                                    c6c12=system.c6c12, excl_ranges=system.excl_ranges,
+                                    excl_elements=system.excl_elements)
+            fb, eljb, eelb, npairs = oracle.bruteforce(cur, ic)
+            f = md.f.cpu().numpy()
+            assert rel_rms(f, fb) < 5e-6
+            elj, eel = ver.energies()
+            assert elj == pytest.approx(eljb, rel=2e-5)
+            assert eel == pytest.approx(eelb, rel=2e-5, abs=1e-3)
+            sci, cj, excl, ai = ver.current_list()
+            plist, atoms = oracle_atoms(x_now.astype(np.float32), system.q, sci, cj, excl, ai, system.box)
+            assert oracle.count_pairs(plist, atoms, 1.0) == npairs
+            checked += 1
+        md.advance()
+    st = ver.stats()
+    assert checked == 3
+    assert st["searches"] == 3 and st["steps"] == 60
+    assert st["dynamic_pruning"] and st["rolling_parts"] == 2
+    # isDynamicPruningStepGpu: age in (0, 19) and even -> 9 prunes per list.
+    assert st["prunes"] == 27
+
+
+def test_md_energy_conservation(torch):
+    """NVE leapfrog with the force-switched LJ fluid: the total energy (kinetic
+    averaged over the half steps) stays within 0.2 % of the kinetic energy
+    over 600 steps with dynamic pruning."""
+    announce("md energy conservation")
+    system, params = lj_fluid(3000, charge=0.0, seed=11)
+    ic = nb.interaction_const_ex(1.0, coulomb="rf", plain_cutoff=True, vdw_modifier="forceswitch", rvdw_switch=0.8)
+    ic.epsfac = 0.0
+    ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
+                             rlist_inner=1.05, nstlist_prune=4)
+    md = Leapfrog(torch, system, 100.0, 0.004)
+    stream = torch.cuda.current_stream().cuda_stream
+    energies = []
+    for step in range(600):
+        ver.step(step, system.box, md.x.data_ptr(), md.f.data_ptr(), stream, compute_energy=True)
+        elj, _ = ver.energies()
+        ke = md.advance()
+        if step >= 100:
+            energies.append((elj + ke, ke))
+    e = np.array(energies)
+    drift = abs(e[-1, 0] - e[0, 0])
+    assert drift < 2e-3 * e[:, 1].mean(), (e[0], e[-1])
+    assert np.std(e[:, 0]) < 2e-3 * e[:, 1].mean()

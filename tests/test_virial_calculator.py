@@ -150,3 +150,26 @@ def test_force_calculator_argument_checks():
     assert lib.nbnxm_force_calculator_update_pairlist(None, 0, None, None) == -1
     assert lib.nbnxm_force_calculator_compute(None, 0, None, None, None, None, None) == -1
     assert lib.nbnxm_gpu_get_virial(None, None) == -1
+
+
+def test_verlet_argument_checks():
+    """nbnxm_verlet_* rejects bad arguments before touching the GPU."""
+    lib = nb.load()
+    ic = nb.interaction_const(1.0, 1e-5)
+    p = nb._lib.VerletParams()
+    p.nstlist, p.rlistOuter, p.rlistInner, p.nstlistPrune, p.accumulation = 20, 1.1, 1.05, 4, nb.ACCUM_FIXED64
+    types = np.zeros(4, dtype=np.int32)
+    nbfp = np.array([1e-3, 1e-6])
+    q = np.zeros(4)
+    out = ctypes.c_void_p()
+    ptr = nb._lib.ptr
+    assert lib.nbnxm_verlet_create(4, ptr(types), 1, ptr(nbfp), ptr(q), None, None, ctypes.byref(ic), ctypes.byref(p), None) == -1
+    bad = nb._lib.VerletParams()
+    bad.nstlist, bad.rlistOuter, bad.rlistInner, bad.accumulation = 20, 1.1, 0.9, nb.ACCUM_FIXED64
+    assert lib.nbnxm_verlet_create(4, ptr(types), 1, ptr(nbfp), ptr(q), None, None, ctypes.byref(ic), ctypes.byref(bad), ctypes.byref(out)) == -1
+    assert "rlistInner" in lib.nbnxm_last_error().decode()
+    bad_types = np.array([0, 0, 3, 0], dtype=np.int32)
+    assert lib.nbnxm_verlet_create(4, ptr(bad_types), 1, ptr(nbfp), ptr(q), None, None, ctypes.byref(ic), ctypes.byref(p), ctypes.byref(out)) == -1
+    assert lib.nbnxm_verlet_step(None, 0, None, None, None, 0, 0, None, 0) == -1
+    assert lib.nbnxm_verlet_get_stats(None, None, None) == -1
+    assert lib.nbnxm_verlet_get_list(None, None, None, None, None, None) == -1
