@@ -80,6 +80,99 @@ def cpu_baseline(systems, seconds, num_systems=16):
                       f"on {cpu_model or platform.processor()} (nproc {os.cpu_count()})"}
 
 
+# Reference flop weights per pair (src/gromacs/gmxlib/nrnb.cpp:90-103): F, F+E.
+FLAVOUR_WEIGHTS = {
+    "ewald_analytical": (66, 107),
+    "ewald_tabulated": (41, 59),
+    "reaction_field": (38, 54),
+    "ewald_twin": (66, 107),
+    "ewald_forceswitch": (66 + 12, 107 + 22),
+    "ewald_potswitch": (66 + 27, 107 + 20),
+    "ewald_ljpme": (66 + 36, 107 + 33),
+}
+FLAVOUR_OPTIONS = {
+    "ewald_analytical": {},
+    "ewald_tabulated": dict(tabulated=True),
+    "reaction_field": dict(coulomb="rf"),
+    "ewald_twin": dict(rvdw=0.9),
+    "ewald_forceswitch": dict(vdw_modifier="forceswitch", rvdw_switch=0.8),
+    "ewald_potswitch": dict(vdw_modifier="potswitch", rvdw_switch=0.8),
+    "ewald_ljpme": dict(lj_pme=True, ewald_rtol_lj=1e-3),
+}
+
+
+def water96k_setup(nb, vdw):
+    s = nb.systems.config_water_96k()
+    search, plist, atoms = nb.build(s, RLIST_OUTER, vdw)
+    return s, search, plist, atoms
+
+
+def flavour_table(launches):
+    """One JSON line per interaction flavour and F / F+E: one evaluation of the
+    96,000-atom water box (rc 1.0 nm, list 1.1 nm, fixed point) per launch,
+    wall clock over `launches` launches after warm-up."""
+    import nbnxm_amd as nb
+
+    geom = water96k_setup(nb, nb.VDW_CUT_COMB_GEOM)
+    table = water96k_setup(nb, nb.VDW_CUT_TYPE_TABLE)
+    useful = geom[1].count_pairs(RC)
+    for name, kw in FLAVOUR_OPTIONS.items():
+        s, search, plist, atoms = table if kw.get("lj_pme") else geom
+        ic = nb.interaction_const(RC, EWALD_RTOL) if not kw else nb.interaction_const_ex(RC, ewald_rtol=EWALD_RTOL, **kw)
+        for energy in (False, True):
+            task = nb.GpuTask(ic, atoms, RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
+            task.init_pairlist(plist)
+            for _ in range(3):
+                task.launch(energy, False)
+            task.fetch()
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                task.launch(energy, False)
+            task.fetch()
+            ms = (time.perf_counter() - t0) / launches * 1e3
+            flops = FLAVOUR_WEIGHTS[name][1 if energy else 0]
+            print(json.dumps({"flavour": name, "energies": energy, "ms_per_launch": ms,
+                              "gpairs_per_s": useful / ms / 1e6, "useful_pairs": useful,
+                              "algorithmic_tflops": useful * flops / (ms * 1e-3) / 1e12, "flops_per_pair": flops,
+                              "lj": "type table" if kw.get("lj_pme") else "geometric"}), flush=True)
+            del task
+
+
+def md_step_table(steps):
+    """The nonbonded part of an MD step (nbnxm_verlet) on the 96,000-atom water
+    box: search every 100 steps on the host or on the device, rolling prune,
+    x -> xq, kernel (energies every 100 steps), force reduction; coordinates
+    held fixed, so only the nonbonded work is timed."""
+    import torch
+    import nbnxm_amd as nb
+
+    s = nb.systems.config_water_96k()
+    params = np.stack([6.0 * s.c6c12[..., 0], 12.0 * s.c6c12[..., 1]], axis=-1)
+    ic = nb.interaction_const(RC, EWALD_RTOL)
+    x = torch.tensor(s.x, dtype=torch.float32, device="cuda").contiguous()
+    f = torch.zeros_like(x)
+    stream = torch.cuda.current_stream().cuda_stream
+    for gpu_search in (False, True):
+        ver = nb.NonbondedVerlet(s.types, params, s.q, ic, nstlist=100, rlist_outer=RLIST_OUTER, rlist_inner=1.02,
+                                 nstlist_prune=10, excl_ranges=s.excl_ranges, excl_elements=s.excl_elements,
+                                 gpu_search=gpu_search)
+        ver.step(0, s.box, x.data_ptr(), f.data_ptr(), stream)
+        search_ms, other = [], []
+        for step in range(1, steps + 1):
+            t0 = time.perf_counter()
+            ver.step(step, s.box, x.data_ptr(), f.data_ptr(), stream, compute_energy=(step % 100 == 0))
+            dt = (time.perf_counter() - t0) * 1e3
+            (search_ms if step % 100 == 0 else other).append(dt)
+        st = ver.stats()
+        total = sum(search_ms) + sum(other)
+        print(json.dumps({"md_step": "nonbonded part", "search": "device" if gpu_search else "host",
+                          "atoms": s.n, "steps": steps, "ms_per_step": total / steps,
+                          "search_step_ms": float(np.mean(search_ms)) if search_ms else None,
+                          "other_step_ms": float(np.mean(other)), "nstlist": 100, "nstlist_prune": 10,
+                          "stats": st}), flush=True)
+        del ver
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +188,17 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration for HBM bytes) and exit")
+    ap.add_argument("--flavours", type=int, default=0, metavar="LAUNCHES",
+                    help="only print the per-flavour kernel table for the 96k-atom water box and exit")
+    ap.add_argument("--md-step", type=int, default=0, metavar="STEPS",
+                    help="only time the nonbonded part of an MD step (96k-atom water) and exit")
     args = ap.parse_args()
+    if args.flavours:
+        flavour_table(args.flavours)
+        return
+    if args.md_step:
+        md_step_table(args.md_step)
+        return
     if args.calibrate_copy:
         import torch
 

@@ -686,6 +686,20 @@ int nbnxm_search_fill_ljcomb(const nbnxm_search_t* s, const int* type, const flo
     return NBNXM_OK;
 }
 
+int nbnxm_search_count_pairs(const nbnxm_search_t* s, double rc, int64_t* count)
+{
+    if (!s || !count)
+    {
+        return fail(NBNXM_ERR_INVALID, "search_count_pairs: null pointer");
+    }
+    if (!(rc > 0) || rc > s->grid.rlist() * (1.0 + 1e-6))
+    {
+        return fail(NBNXM_ERR_INVALID, "search_count_pairs: need 0 < rc <= the list radius");
+    }
+    *count = s->grid.countPairsWithin(rc);
+    return NBNXM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // GPU task
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "capi_common.h"
+#include "gpu_search.h"
 #include "pairsearch.h"
 
 using namespace nbnxm_amd;
@@ -48,6 +49,8 @@ struct nbnxm_verlet
     int lifetime = 0;
 
     GridSearch search;
+    nbnxm_gpu_search_t* gpuSearch = nullptr; // device search, when asked for
+    std::vector<double> hxd;                 // its FP64 host input
     nbnxm_gpu_t* gpu = nullptr;
     hipEvent_t waitEvent = nullptr;
     bool haveList = false;
@@ -73,6 +76,10 @@ struct nbnxm_verlet
         {
             nbnxm_gpu_free(gpu);
         }
+        if (gpuSearch)
+        {
+            nbnxm_gpu_search_free(gpuSearch);
+        }
         if (waitEvent)
         {
             (void)hipEventDestroy(waitEvent);
@@ -87,7 +94,44 @@ struct nbnxm_verlet
     }
 
     int searchAndUpload(float* d_x, const double* newBox);
+    int buildList(int maxPairsPerEntry);
 };
+
+// Grid and list into `search`: on the host, or by the device search (which
+// wraps the coordinates itself) handed over to the host object.
+int nbnxm_verlet::buildList(int maxPairsPerEntry)
+{
+    const int* er = exclRanges.empty() ? nullptr : exclRanges.data();
+    const int* ee = exclRanges.empty() ? nullptr : exclElements.data();
+    int rc;
+    if (params.gpuSearch)
+    {
+        if (!gpuSearch && (rc = nbnxm_gpu_search_create(&gpuSearch)) != NBNXM_OK)
+        {
+            gpuSearch = nullptr;
+            return rc;
+        }
+        hxd.assign(hx.begin(), hx.end());
+        nbnxm_system_desc_t d{};
+        d.numAtoms = numAtoms;
+        d.x = hxd.data();
+        d.type = types.data();
+        d.exclRanges = er;
+        d.exclElements = ee;
+        std::copy(box, box + 3, d.box);
+        if ((rc = nbnxm_gpu_search_build(gpuSearch, 1, &d, params.rlistOuter, maxPairsPerEntry)) != NBNXM_OK)
+        {
+            return rc;
+        }
+        return gpuSearchExportSystem(gpuSearch, 0, search);
+    }
+    if (maxPairsPerEntry == 0 && (rc = search.putAtomsOnGrid(box, numAtoms, hx.data())) != NBNXM_OK)
+    {
+        return fail(rc, "verlet_step: grid construction failed");
+    }
+    rc = search.constructPairlist(params.rlistOuter, er, ee, maxPairsPerEntry);
+    return rc == NBNXM_OK ? rc : fail(rc, "verlet_step: pair list construction failed");
+}
 
 int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
 {
@@ -98,10 +142,10 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
         NBNXM_HIP_CHECK(hipMemcpy(hx.data(), d_x, sizeof(float) * n3, hipMemcpyDeviceToHost));
     }
     std::copy(newBox, newBox + 3, box);
-    int rc = search.putAtomsOnGrid(box, numAtoms, hx.data());
+    int rc = buildList(0);
     if (rc != NBNXM_OK)
     {
-        return fail(rc, "verlet_step: grid construction failed");
+        return rc;
     }
     // Atoms are put in the box on search steps (do_md, put_atoms_in_box_omp):
     // the state coordinates become the wrapped ones the grid was built from.
@@ -109,13 +153,6 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
     if (numAtoms > 0)
     {
         NBNXM_HIP_CHECK(hipMemcpy(d_x, xw.data(), sizeof(float) * n3, hipMemcpyHostToDevice));
-    }
-    const int* er = exclRanges.empty() ? nullptr : exclRanges.data();
-    const int* ee = exclRanges.empty() ? nullptr : exclElements.data();
-    rc = search.constructPairlist(params.rlistOuter, er, ee, 0);
-    if (rc != NBNXM_OK)
-    {
-        return fail(rc, "verlet_step: pair list construction failed");
     }
     const int ng = search.numGridAtoms();
     xq.resize(4 * static_cast<size_t>(ng));
@@ -165,10 +202,9 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
     {
         const int64_t pairs = search.numClusterPairs();
         const int target = static_cast<int>(std::max<int64_t>(36, (pairs + minCi - 1) / minCi));
-        rc = search.constructPairlist(params.rlistOuter, er, ee, target);
-        if (rc != NBNXM_OK)
+        if ((rc = buildList(target)) != NBNXM_OK)
         {
-            return fail(rc, "verlet_step: pair list construction failed");
+            return rc;
         }
         view = search.listView();
     }

@@ -84,7 +84,7 @@ class IcOptions(ctypes.Structure):
 
 class VerletParams(ctypes.Structure):
     _fields_ = [("nstlist", ctypes.c_int), ("rlistOuter", ctypes.c_double), ("rlistInner", ctypes.c_double),
-                ("nstlistPrune", ctypes.c_int), ("accumulation", ctypes.c_int)]
+                ("nstlistPrune", ctypes.c_int), ("accumulation", ctypes.c_int), ("gpuSearch", ctypes.c_int)]
 
 
 class PairlistParams(ctypes.Structure):
@@ -134,6 +134,7 @@ SIGNATURES = [
     ("nbnxm_search_shift_vec", _c_int, [_vp, _vp]),
     ("nbnxm_search_fill_xq", _c_int, [_vp, _vp, _vp, _vp]),
     ("nbnxm_search_fill_types", _c_int, [_vp, _vp, _c_int, _vp]),
+    ("nbnxm_search_count_pairs", _c_int, [_vp, _c_double, _PTR(ctypes.c_int64)]),
     ("nbnxm_search_fill_ljcomb", _c_int, [_vp, _vp, _vp, _vp]),
     ("nbnxm_gpu_init", _c_int, [_PTR(InteractionConst), _PTR(PairlistParams), _PTR(Atomdata), _c_int, _PTR(_vp)]),
     ("nbnxm_gpu_init_atomdata", _c_int, [_vp, _PTR(Atomdata)]),

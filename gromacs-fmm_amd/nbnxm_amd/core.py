@@ -137,6 +137,12 @@ class PairSearch:
             e = np.zeros(1, dtype=np.int32)
         check(self._lib.nbnxm_search_construct_pairlist(self._h, rlist, ptr(r), ptr(e), int(max_pairs_per_entry)), "construct_pairlist")
 
+    def count_pairs(self, rc):
+        """Non-excluded pairs (all images) within rc in the current list, FP64."""
+        c = ctypes.c_int64(0)
+        check(self._lib.nbnxm_search_count_pairs(self._h, float(rc), ctypes.byref(c)), "search_count_pairs")
+        return c.value
+
     def pairlist(self):
         v = PairlistGpu()
         check(self._lib.nbnxm_search_get_pairlist(self._h, ctypes.byref(v)), "get_pairlist")

@@ -19,7 +19,8 @@ __all__ = ["NonbondedVerlet"]
 
 class NonbondedVerlet:
     def __init__(self, types, nonbonded_params, charges, ic, nstlist, rlist_outer, rlist_inner=None,
-                 nstlist_prune=0, excl_ranges=None, excl_elements=None, accumulation=ACCUM_FIXED64):
+                 nstlist_prune=0, excl_ranges=None, excl_elements=None, accumulation=ACCUM_FIXED64,
+                 gpu_search=False):
         """nonbonded_params: (numTypes, numTypes, 2) array of (6*C6, 12*C12)."""
         self._lib = _lib.load()
         self.types = np.ascontiguousarray(types, dtype=np.int32)
@@ -36,6 +37,7 @@ class NonbondedVerlet:
         p.rlistInner = float(rlist_outer if rlist_inner is None else rlist_inner)
         p.nstlistPrune = int(nstlist_prune)
         p.accumulation = int(accumulation)
+        p.gpuSearch = int(bool(gpu_search))
         self.ic = ic
         h = ctypes.c_void_p()
         check(self._lib.nbnxm_verlet_create(self.n, ptr(self.types), self.num_types, ptr(self.nbfp), ptr(self.q),

@@ -252,6 +252,11 @@ int nbnxm_search_shift_vec(const nbnxm_search_t* s, float* shiftVec /* 3*45 */);
 int nbnxm_search_fill_xq(const nbnxm_search_t* s, const double* x, const double* q, float* xq);
 int nbnxm_search_fill_types(const nbnxm_search_t* s, const int* type, int dummyType, int* slotType);
 int nbnxm_search_fill_ljcomb(const nbnxm_search_t* s, const int* type, const float* perTypeComb, float* slotComb);
+/* Useful pairs of the current list: non-excluded atom pairs (all images)
+ * within rc, FP64 distances on the wrapped search coordinates; rc must not
+ * exceed the list radius.  The exact count the benchmark harness estimates in
+ * closed form (src/gromacs/nbnxm/benchmark/bench_setup.cpp:291-295). */
+int nbnxm_search_count_pairs(const nbnxm_search_t* s, double rc, int64_t* count);
 
 /* ---------------------------------------------------------------------------
  * Single-system GPU task: the Nbnxm::gpu_* API
@@ -544,6 +549,7 @@ typedef struct nbnxm_verlet_params
     double rlistInner;   /* pruned radius in [cut-off, rlistOuter]; rlistOuter = no dynamic pruning */
     int nstlistPrune;    /* steps between prunes of the whole list (even, >= 2) */
     int accumulation;    /* enum nbnxm_accumulation */
+    int gpuSearch;       /* nonzero: grid and list built by the device search (nbnxm_gpu_search_*) */
 } nbnxm_verlet_params_t;
 
 /* Sets up the system: per-atom types (numTypes types, nbfp (6C6, 12C12)

@@ -87,17 +87,18 @@ def torch():
     return t
 
 
-@pytest.mark.parametrize("accumulation", [nb.ACCUM_FIXED64, nb.ACCUM_FLOAT])
-def test_md_list_and_pruning_cover_all_pairs(torch, accumulation):
+@pytest.mark.parametrize("accumulation,gpu_search", [(nb.ACCUM_FIXED64, False), (nb.ACCUM_FLOAT, False),
+                                                     (nb.ACCUM_FIXED64, True)])
+def test_md_list_and_pruning_cover_all_pairs(torch, accumulation, gpu_search):
     """60 leapfrog steps with nstlist 20 and rolling pruning of the inner list:
     before each search, the list in use holds every pair within the cut-off
     and the forces and energies match the list-free FP64 sum at the current
-    positions."""
-    announce(f"md lj fluid accumulation={accumulation}")
+    positions.  With the host search and with the device search."""
+    announce(f"md lj fluid accumulation={accumulation} gpu_search={gpu_search}")
     system, params = lj_fluid(3000)
     ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
     ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
-                             rlist_inner=1.05, nstlist_prune=4, accumulation=accumulation)
+                             rlist_inner=1.05, nstlist_prune=4, accumulation=accumulation, gpu_search=gpu_search)
     md = Leapfrog(torch, system, 120.0, 0.004)
     stream = torch.cuda.current_stream().cuda_stream
     checked = 0
