@@ -1346,7 +1346,8 @@ int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb)
 
 int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb)
 {
-    return nb ? (nb->ic.eeltype == NBNXM_ELEC_EWALD ? 1 : 0) : NBNXM_ERR_INVALID;
+    // EwaldAna or EwaldAnaTwin (nbnxm_gpu_data_mgmt.cpp:823-827).
+    return nb ? ((nb->ic.eeltype == NBNXM_ELEC_EWALD && !nb->ic.tabulatedEwald) ? 1 : 0) : NBNXM_ERR_INVALID;
 }
 
 int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int* imaskOuter)

@@ -324,8 +324,9 @@ int nbnxm_gpu_x_to_nbat_x(nbnxm_gpu_t* nb, const float* d_x);
 int nbnxm_gpu_add_nbat_f_to_f(nbnxm_gpu_t* nb, float* d_f, const float* d_fToAdd, int accumulate);
 /* gpu_min_ci_balanced (gpu_data_mgmt.h:119). */
 int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb);
-/* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123): 1 for Ewald (always the
- * analytical correction here), 0 for reaction field / plain cut-off. */
+/* gpu_is_kernel_ewald_analytical (gpu_data_mgmt.h:123,
+ * nbnxm_gpu_data_mgmt.cpp:823-827): 1 for analytical Ewald (twin or not), 0 for
+ * tabulated Ewald, reaction field and plain cut-off. */
 int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb);
 /* Downloads the per-group masks (cjPacked imask and the rlistOuter mask,
  * gpu_plist::imask); no counterpart in the reference, used by tests. */

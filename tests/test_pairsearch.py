@@ -4,6 +4,8 @@ import numpy as np
 import pytest
 
 import nbnxm_amd as nb
+import oracle
+from helpers import wrapped_float_system
 
 
 @pytest.fixture(scope="module")
@@ -161,3 +163,15 @@ def test_empty_and_single_atom():
     pl = search.pairlist()
     # One central diagonal entry holding only the self cluster pair.
     assert len(pl.sci) == 1 and pl.sci[0]["shift"] == nb.CENTRAL_SHIFT
+
+
+def test_search_count_pairs_equals_oracle():
+    """The exact useful-pair count of a search equals the oracle's list count
+    and the brute-force count."""
+    s = wrapped_float_system(nb.systems.water_box(300, None, seed=17))
+    search, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_COMB_GEOM)
+    n = search.count_pairs(1.0)
+    assert n == oracle.count_pairs(plist, atoms, 1.0)
+    assert n == oracle.bruteforce(s, nb.interaction_const(1.0, 1e-5))[3]
+    with pytest.raises(nb.NbnxmError):
+        search.count_pairs(1.2)
