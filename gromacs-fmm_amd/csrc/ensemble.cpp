@@ -479,6 +479,10 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "ensemble_create: unknown vdwType");
     }
+    if (opt->vdwType == NBNXM_VDW_CUT_TYPE_TABLE && opt->numTypes + 1 > c_maxLdsTableTypes)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "ensemble_create: the type table kernel stages at most 64 types in LDS");
+    }
     if (opt->accumulation < NBNXM_ACCUM_FLOAT || opt->accumulation > NBNXM_ACCUM_FP64)
     {
         return fail(NBNXM_ERR_INVALID, "ensemble_create: unknown accumulation mode");

@@ -1117,18 +1117,21 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     a.fFixed = nb->fFixed.ptr;
     a.entryPartials = nb->partials.ptr;
     a.computeFshift = computeVirial ? 1 : 0;
-    if (isDefaultFlavour(nb->ic))
+    // The tuned list walk stages the type table in LDS up to 65 types (64 + dummy).
+    const bool tableFitsLds = nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE || nb->numTypes <= c_maxLdsTableTypes;
+    if (isDefaultFlavour(nb->ic) && (tableFitsLds || nb->accumulation == NBNXM_ACCUM_FP64))
     {
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
     }
-    else if (nb->accumulation != NBNXM_ACCUM_FP64 && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE))
+    else if (nb->accumulation != NBNXM_ACCUM_FP64 && tableFitsLds && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE))
     {
         // Other FP32 flavours: the tuned list walk with the generic pair interaction.
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream, &nb->flavour));
     }
     else
     {
-        // FP64 mode, and LJ-PME with a combination-rule kernel: the standalone flavour kernel.
+        // FP64 mode of the other flavours, LJ-PME with a combination-rule kernel
+        // and type tables beyond the LDS staging: the standalone flavour kernel.
         NBNXM_HIP_CHECK(launchNonbondedFlavour(a, nb->flavour, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->stream));
     }
     nb->lastComputeEnergy = computeEnergy != 0;

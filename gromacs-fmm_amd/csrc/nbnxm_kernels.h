@@ -81,6 +81,9 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic);
 // Every flavour (one wave per entry, FP32 or, with NBNXM_ACCUM_FP64, FP64).
 hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream);
 
+// Types (with the dummy type) of the largest LJ table the tuned kernel stages in LDS.
+constexpr int c_maxLdsTableTypes = 65;
+
 // Launch tuning chosen per size bucket by the autotuner.
 struct NbLaunchConfig
 {

@@ -123,6 +123,22 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
     const int tj = lane >> 3;
     // Wave-uniform by construction; readfirstlane lets the compiler keep it scalar.
     const int work = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + (threadIdx.x >> 6));
+    // The LJ type table in LDS (dynamic size numTypes^2 float2, staged by the
+    // whole block before any wave leaves): per-pair (6C6, 12C12) reads become
+    // LDS reads, which do not wait behind the in-flight force atomics as a
+    // vector load's vmcnt wait would.
+    [[maybe_unused]] float2* sNbfp = nullptr;
+    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        HIP_DYNAMIC_SHARED(float2, sNbfpDyn);
+        sNbfp = sNbfpDyn;
+        const int numEntries = a.numTypes * a.numTypes;
+        for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
+        {
+            sNbfp[i] = a.nbfp[i];
+        }
+        __syncthreads();
+    }
     if (work >= a.nsci)
     {
         return;
@@ -148,11 +164,12 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
     const float4 sh = loadF4(asConst(a.shiftVec), s.shift);
     [[maybe_unused]] const FlavourConsts<float> fk(flav);
 
-    // i-atom coordinates and charges of all 8 clusters in registers; their LJ
-    // parameters (type row offset or combination parameters) in per-wave LDS.
+    // i-atom coordinates and charges of all 8 clusters in registers, and the
+    // type-table row offsets too; combination parameters in per-wave LDS.
     __shared__ float2 sLi[4][c_ncps * c_cl];
     float2* stageLi = sLi[threadIdx.x >> 6];
     float xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
+    [[maybe_unused]] int tOffI[c_ncps]; // type table: row offset numTypes * type_i, in registers
 #pragma unroll
     for (int c = 0; c < c_ncps; c++)
     {
@@ -162,17 +179,15 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
         yi[c] = v.y + sh.y;
         zi[c] = v.z + sh.z;
         qi[c] = v.w * epsfac;
-    }
-    {
-        const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
         if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
         {
-            stageLi[lane] = make_float2(__int_as_float(a.numTypes * asConst(a.atomTypes)[ai]), 0.0f);
+            tOffI[c] = a.numTypes * a.atomTypes[ai];
         }
-        else
-        {
-            stageLi[lane] = loadF2(asConst(a.ljComb), ai);
-        }
+    }
+    if constexpr (vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
+        stageLi[lane] = loadF2(asConst(a.ljComb), ai);
     }
 
     // LJ-PME grid factors of the i-atoms (generic flavour with the type table).
@@ -346,20 +361,22 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
                 {
                     float c6, c12;
                     float sigma = 0.0f, epsilon = 0.0f;
-                    const float2 ljci = stageLi[c * c_cl + ti];
                     if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
                     {
-                        const float2 p = loadF2(asConst(a.nbfp), __float_as_int(ljci.x) + typej);
+                        // One LDS read: the i row offset is a register.
+                        const float2 p = sNbfp[tOffI[c] + typej];
                         c6 = p.x;
                         c12 = p.y;
                     }
                     else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
                     {
+                        const float2 ljci = stageLi[c * c_cl + ti];
                         c6 = ljci.x * ljcj.x;
                         c12 = ljci.y * ljcj.y;
                     }
                     else
                     {
+                        const float2 ljci = stageLi[c * c_cl + ti];
                         sigma = ljci.x + ljcj.x;
                         epsilon = ljci.y * ljcj.y;
                         const float sigma2 = sigma * sigma;
@@ -1241,6 +1258,9 @@ __global__ void fillI32Kernel(int* p, size_t n, int value)
     }
 }
 
+// Largest LJ type table staged in LDS: 64 types (+ dummy), 33.8 KB.
+constexpr size_t c_maxLdsTypeTableBytes = sizeof(float2) * c_maxLdsTableTypes * c_maxLdsTableTypes;
+
 template<bool E, int V, int A>
 hipError_t launchNb(const NbKernelArgs& a, const NbFlavourParams* flav, const NbLaunchConfig& cfg, hipStream_t stream)
 {
@@ -1254,13 +1274,22 @@ hipError_t launchNb(const NbKernelArgs& a, const NbFlavourParams* flav, const Nb
     {
         hipLaunchKernelGGL((nbnxmNonbondedKernelFp64<E, V>), dim3(blocks), dim3(64 * wpb), 0, stream, a, wpb);
     }
-    else if (flav)
-    {
-        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, true>), dim3(blocks), dim3(64 * wpb), 0, stream, a, *flav, wpb);
-    }
     else
     {
-        hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, false>), dim3(blocks), dim3(64 * wpb), 0, stream, a, NbFlavourParams{}, wpb);
+        // The type table kernel stages the whole table in LDS.
+        const size_t ldsBytes = (V == NBNXM_VDW_CUT_TYPE_TABLE) ? sizeof(float2) * a.numTypes * a.numTypes : 0;
+        if (ldsBytes > c_maxLdsTypeTableBytes)
+        {
+            return hipErrorInvalidValue;
+        }
+        if (flav)
+        {
+            hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, true>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, *flav, wpb);
+        }
+        else
+        {
+            hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, false>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, NbFlavourParams{}, wpb);
+        }
     }
     return hipGetLastError();
 }
