@@ -57,6 +57,83 @@ struct FlavourConsts
     }
 };
 
+// What a flavour class fixes at compile time; -1 = read from the parameters.
+template<int kFlav>
+struct FlavourTraits
+{
+    static constexpr bool fixed = kFlav == c_flavRfPotShift || kFlav == c_flavEwaldForceSwitch;
+    static constexpr int elec = kFlav == c_flavRfPotShift ? NBNXM_ELEC_RF : (kFlav == c_flavEwaldForceSwitch ? NBNXM_ELEC_EWALD : -1);
+    static constexpr int modifier = kFlav == c_flavRfPotShift ? NBNXM_VDW_MOD_POTSHIFT
+                                                             : (kFlav == c_flavEwaldForceSwitch ? NBNXM_VDW_MOD_FORCESWITCH : -1);
+};
+
+template<int kFlav>
+__device__ __forceinline__ bool flavRf(const NbFlavourParams& f)
+{
+    if constexpr (FlavourTraits<kFlav>::fixed)
+    {
+        return FlavourTraits<kFlav>::elec == NBNXM_ELEC_RF;
+    }
+    else
+    {
+        return f.eeltype == NBNXM_ELEC_RF;
+    }
+}
+
+template<int kFlav>
+__device__ __forceinline__ bool flavLjPme(const NbFlavourParams& f)
+{
+    if constexpr (FlavourTraits<kFlav>::fixed)
+    {
+        return false;
+    }
+    else
+    {
+        return f.ljPme != 0;
+    }
+}
+
+template<int kFlav>
+__device__ __forceinline__ bool flavModifier(const NbFlavourParams& f, int modifier)
+{
+    if constexpr (FlavourTraits<kFlav>::fixed)
+    {
+        return FlavourTraits<kFlav>::modifier == modifier;
+    }
+    else
+    {
+        return f.vdwModifier == modifier;
+    }
+}
+
+// The LJ cut-off check of twin cut-offs and the tabulated correction exist
+// only in run-time flavours.
+template<int kFlav>
+__device__ __forceinline__ bool flavTwin(const NbFlavourParams& f)
+{
+    if constexpr (FlavourTraits<kFlav>::fixed)
+    {
+        return false;
+    }
+    else
+    {
+        return f.vdwCutoffCheck != 0;
+    }
+}
+
+template<int kFlav>
+__device__ __forceinline__ bool flavTable(const NbFlavourParams& f)
+{
+    if constexpr (FlavourTraits<kFlav>::fixed)
+    {
+        return false;
+    }
+    else
+    {
+        return f.tabulatedEwald != 0;
+    }
+}
+
 template<typename Real>
 __device__ __forceinline__ Real rsqrtR(Real x)
 {
@@ -87,7 +164,8 @@ __device__ __forceinline__ Real expR(Real x)
 // clamp r^2, LJ with the PotShift constants, force switch (unmasked c6/c12 as
 // the reference GPU kernels use them), LJ-PME grid part, potential switch, LJ
 // cut-off check of twin cut-offs, Coulomb.  Returns F/r; adds energies.
-template<typename Real, bool computeEnergy>
+// kFlav fixes the flavour at compile time where its class allows.
+template<typename Real, bool computeEnergy, int kFlav = c_flavRuntime>
 __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
                                           const FlavourConsts<Real>& k,
                                           Real r2,
@@ -108,7 +186,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
     const Real inv_r6 = inv_r6_nm * int_bit;
     Real F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
     Real E_lj_p = int_bit * (c12 * (inv_r6 * inv_r6 + k.cpot_rep) * oneTwelveth - c6 * (inv_r6 + k.cpot_disp) * oneSixth);
-    if (flav.vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
+    if (flavModifier<kFlav>(flav, NBNXM_VDW_MOD_FORCESWITCH))
     {
         const Real r = r2 * inv_r;
         Real rs = r - k.rvdw_switch;
@@ -120,7 +198,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
                       * rs * rs * rs;
         }
     }
-    if (flav.ljPme)
+    if (flavLjPme<kFlav>(flav))
     {
         const Real cr2 = k.lje_coeff2 * r2;
         const Real expmcr2 = expR(-cr2);
@@ -131,7 +209,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
             E_lj_p += oneSixth * c6grid * (inv_r6_nm * (Real(1) - expmcr2 * poly) + k.sh_lj_ewald * int_bit);
         }
     }
-    if (flav.vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    if (flavModifier<kFlav>(flav, NBNXM_VDW_MOD_POTSWITCH))
     {
         const Real r = r2 * inv_r;
         Real rs = r - k.rvdw_switch;
@@ -141,7 +219,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
         F_invr = F_invr * sw - inv_r * E_lj_p * dsw;
         E_lj_p *= sw;
     }
-    if (flav.vdwCutoffCheck && !(r2 < k.rvdw_sq))
+    if (flavTwin<kFlav>(flav) && !(r2 < k.rvdw_sq))
     {
         F_invr = Real(0);
         E_lj_p = Real(0);
@@ -150,7 +228,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
     {
         eLJ += E_lj_p;
     }
-    if (flav.eeltype == NBNXM_ELEC_RF)
+    if (flavRf<kFlav>(flav))
     {
         F_invr += qq * (int_bit * inv_r2 * inv_r - k.two_k_rf);
         if constexpr (computeEnergy)
@@ -158,7 +236,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
             eEl += qq * (int_bit * inv_r + Real(0.5) * k.two_k_rf * r2 - k.c_rf);
         }
     }
-    else if (flav.tabulatedEwald)
+    else if (flavTable<kFlav>(flav))
     {
         // interpolate_coulomb_force_r (nbnxm_cuda_kernel_utils.cuh:345-385): linear
         // interpolation of the F(r) table, lerp as two fused multiply-adds.

@@ -321,6 +321,20 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
     p.ewaldcoeff_lj = ic.ewaldcoeff_lj;
     p.sh_lj_ewald = ic.sh_lj_ewald;
     p.tabulatedEwald = ic.tabulatedEwald;
+    // Flavour classes compiled with their flavour fixed (one instantiation each).
+    const bool plain = !ic.ljPme && !ic.tabulatedEwald && ic.rvdw == ic.rcoulomb;
+    if (plain && ic.eeltype == NBNXM_ELEC_RF && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT)
+    {
+        p.flavourClass = c_flavRfPotShift;
+    }
+    else if (plain && ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
+    {
+        p.flavourClass = c_flavEwaldForceSwitch;
+    }
+    else
+    {
+        p.flavourClass = c_flavRuntime;
+    }
     return p;
 }
 

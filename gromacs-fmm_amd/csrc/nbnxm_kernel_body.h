@@ -1,0 +1,714 @@
+// The tuned nonbonded kernel of the nbnxm cluster-pair path (gfx950, CDNA4,
+// wave64), shared by the translation units that instantiate its flavours.
+//
+// One wavefront owns one pair-list entry (an i-super-cluster of 8 clusters x 8
+// atoms and a run of packed j-groups).  Lane l handles i-atom ti = l & 7 and
+// j-atom tj = l >> 3 of every 8x8 cluster pair, the thread layout of the
+// reference kernel (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh:216-218) with
+// the cluster-pair split removed so the whole 8x8 pair is one wave.
+//  - i-cluster coordinates, charges and LJ parameters live in registers for the
+//    whole entry (the reference stages them in shared memory, :330-351);
+//  - j-forces are reduced over the 8 i-lanes with three DPP adds per component
+//    (row_half_mirror, quad_perm xor1, quad_perm xor2) instead of warp shuffles
+//    (:677, nbnxm_cuda_kernel_utils.cuh:467-496);
+//  - forces are accumulated as int64 fixed point (2^-32 units), so every
+//    evaluation is bit-reproducible whatever the atomic order;
+//  - energies and shift forces are reduced in FP64 per entry and written to a
+//    per-entry slot, then summed per system in a fixed order by a second kernel.
+#pragma once
+
+#include "device_common.h"
+#include "flavour_math.h"
+#include "nbnxm_kernels.h"
+
+namespace nbnxm_amd
+{
+
+namespace
+{
+
+using namespace dev;
+
+template<int ctrl>
+__device__ __forceinline__ float dppF(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+// Sum over the 8 consecutive lanes of a group; every lane of the group gets the
+// same bits (the adds are pairwise commutative images of each other).
+__device__ __forceinline__ float reduceOver8Lanes(float v)
+{
+    v += dppF<0x141>(v); // row_half_mirror: lane k <-> 7-k within 8
+    v += dppF<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dppF<0x4E>(v);  // quad_perm [2,3,0,1]
+    return v;
+}
+
+// Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), 16, 32.
+__device__ __forceinline__ float reduceOverStride8(float v)
+{
+    v += dppF<0x128>(v);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// Read-only list and parameter data through the constant address space: with
+// wave-uniform indices the loads become scalar (s_load, counted by lgkmcnt),
+// so waiting for them never waits on the vector-memory counter that also
+// tracks the in-flight force atomics.
+template<typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+template<typename T>
+__device__ __forceinline__ ConstPtr<T> asConst(const T* p)
+{
+    return (ConstPtr<T>)(p);
+}
+
+__device__ __forceinline__ nbnxm_sci_t loadSci(ConstPtr<nbnxm_sci_t> p, int i)
+{
+    const ConstPtr<int> q = (ConstPtr<int>)(p + i);
+    nbnxm_sci_t s;
+    s.sci = q[0];
+    s.shift = q[1];
+    s.cjPackedBegin = q[2];
+    s.cjPackedEnd = q[3];
+    return s;
+}
+
+__device__ __forceinline__ float4 loadF4(ConstPtr<float4> p, int i)
+{
+    const ConstPtr<float> q = (ConstPtr<float>)(p + i);
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ float2 loadF2(ConstPtr<float2> p, int i)
+{
+    const ConstPtr<float> q = (ConstPtr<float>)(p + i);
+    return make_float2(q[0], q[1]);
+}
+
+// erfc(z) with relative accuracy for z in [0, 4.5]: erfc(z) = exp(-z^2) R(t),
+// t = 1/(1 + 0.7 z), R a degree-9 polynomial fitted to erfc(z) exp(z^2) by
+// relative least squares in FP64 (max relative error 2.7e-7 evaluated in FP32).
+// Used for pairs without exclusions, where 1 - erf(z) would lose the small
+// erfc(z) near the cut-off.  Takes t; the caller forms it with beta folded in.
+__device__ __forceinline__ float erfcxPolyT(float t)
+{
+    float p = 0.06597486883401871f;
+    p = __builtin_fmaf(p, t, -0.3136824071407318f);
+    p = __builtin_fmaf(p, t, 0.5114253163337708f);
+    p = __builtin_fmaf(p, t, -0.12743264436721802f);
+    p = __builtin_fmaf(p, t, -0.5417962670326233f);
+    p = __builtin_fmaf(p, t, 0.4081534445285797f);
+    p = __builtin_fmaf(p, t, 0.18638932704925537f);
+    p = __builtin_fmaf(p, t, 0.4186546206474304f);
+    p = __builtin_fmaf(p, t, 0.39217787981033325f);
+    p = __builtin_fmaf(p, t, 0.00013602453691419214f);
+    return p;
+}
+
+// No occupancy floor: capping the force-only kernel at 128 VGPRs (4 waves per
+// SIMD) spills, and measured 25 % slower than its 3-wave allocation.
+// kFlav = c_flavTuned is the tuned flavour (analytical Ewald, LJ potential
+// shift, rvdw = rcoulomb); every other flavour class runs the same list walk
+// with the pair interaction of flavour_math.h: fixed at compile time for the
+// common classes, read from flav (wave-uniform, scalar branches) for
+// c_flavRuntime.
+template<bool computeEnergy, int vdwType, int accumulation, int kFlav>
+__global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
+{
+    constexpr bool generic = kFlav != c_flavTuned;
+    const int lane = threadIdx.x & 63;
+    const int ti = lane & 7;
+    const int tj = lane >> 3;
+    // Wave-uniform by construction; readfirstlane lets the compiler keep it scalar.
+    const int work = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + (threadIdx.x >> 6));
+    // The LJ type table in LDS (dynamic size numTypes^2 float2, staged by the
+    // whole block before any wave leaves): per-pair (6C6, 12C12) reads become
+    // LDS reads, which do not wait behind the in-flight force atomics as a
+    // vector load's vmcnt wait would.
+    [[maybe_unused]] float2* sNbfp = nullptr;
+    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        HIP_DYNAMIC_SHARED(float2, sNbfpDyn);
+        sNbfp = sNbfpDyn;
+        const int numEntries = a.numTypes * a.numTypes;
+        for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
+        {
+            sNbfp[i] = a.nbfp[i];
+        }
+        __syncthreads();
+    }
+    if (work >= a.nsci)
+    {
+        return;
+    }
+    const ConstPtr<nbnxm_cj_packed_t> cjList = asConst(a.cjPacked);
+    const ConstPtr<float4> xqc = asConst(a.xq);
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? asConst(a.sciOrder)[work] : work);
+    const nbnxm_sci_t s = loadSci(asConst(a.sci), entry);
+    const int sys = s.shift / NBNXM_NUM_SHIFTS;
+    const int shiftInSys = s.shift - sys * NBNXM_NUM_SHIFTS;
+    const bool centralShift = (shiftInSys == NBNXM_CENTRAL_SHIFT);
+
+    const float4 pA = loadF4(asConst(a.sysParamsA), sys);
+    const float2 pB = loadF2(asConst(a.sysParamsB), sys);
+    const float rcoulomb_sq = pA.x;
+    const float epsfac = pA.y;
+    const float beta = pA.z;
+    const float beta2 = beta * beta;
+    const float beta3 = beta * beta * beta;
+    const float ewaldShift = pA.w;
+    const float cpotRep = pB.x;
+    const float cpotDisp = pB.y;
+    const float4 sh = loadF4(asConst(a.shiftVec), s.shift);
+    [[maybe_unused]] const FlavourConsts<float> fk(flav);
+
+    // i-atom coordinates and charges of all 8 clusters in registers, and the
+    // type-table row offsets too; combination parameters in per-wave LDS.
+    __shared__ float2 sLi[4][c_ncps * c_cl];
+    float2* stageLi = sLi[threadIdx.x >> 6];
+    float xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
+    [[maybe_unused]] int tOffI[c_ncps]; // type table: row offset numTypes * type_i, in registers
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        const float4 v = loadF4(xqc, ai);
+        xi[c] = v.x + sh.x;
+        yi[c] = v.y + sh.y;
+        zi[c] = v.z + sh.z;
+        qi[c] = v.w * epsfac;
+        if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+        {
+            tOffI[c] = a.numTypes * a.atomTypes[ai];
+        }
+    }
+    if constexpr (vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
+        stageLi[lane] = loadF2(asConst(a.ljComb), ai);
+    }
+
+    // LJ-PME grid factors of the i-atoms (generic flavour with the type table).
+    [[maybe_unused]] float gci[c_ncps];
+    if constexpr (generic)
+    {
+#pragma unroll
+        for (int c = 0; c < c_ncps; c++)
+        {
+            gci[c] = flavLjPme<kFlav>(flav) ? flav.gridC6[a.atomTypes[(s.sci * c_ncps + c) * c_cl + ti]] : 0.0f;
+        }
+    }
+
+    float fciX[c_ncps], fciY[c_ncps], fciZ[c_ncps];
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        fciX[c] = 0.0f;
+        fciY[c] = 0.0f;
+        fciZ[c] = 0.0f;
+    }
+
+    using EAcc = typename std::conditional<accumulation == NBNXM_ACCUM_FLOAT, float, double>::type;
+    EAcc eLJAcc = 0;
+    EAcc eElAcc = 0;
+    if constexpr (computeEnergy)
+    {
+        // Self energy on the diagonal entry (:381-418): -epsfac*beta/sqrt(pi)*sum q^2,
+        // added once per i-atom by the tj == 0 lanes.
+        if (centralShift && s.cjPackedEnd > s.cjPackedBegin && cjList[s.cjPackedBegin].cj[0] == s.sci * c_ncps && tj == 0)
+        {
+            if constexpr (generic)
+            {
+                // Coulomb: -epsfac sum q^2 times beta/sqrt(pi) (Ewald) or c_rf/2
+                // (reaction field); LJ-PME: + sum 6C6_ii/12 beta_lj^6/6.
+                double q2 = 0.0, c6self = 0.0;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+                    const double q = loadF4(xqc, ai).w;
+                    q2 += q * q;
+                    if (flavLjPme<kFlav>(flav))
+                    {
+                        c6self += a.nbfp[a.atomTypes[ai] * (a.numTypes + 1)].x;
+                    }
+                }
+                const double selfFactor = flavRf<kFlav>(flav) ? 0.5 * flav.c_rf : flav.beta * c_oneSqrtPi;
+                eElAcc -= static_cast<EAcc>(flav.epsfac * q2 * selfFactor);
+                if (flavLjPme<kFlav>(flav))
+                {
+                    const double b2 = flav.ewaldcoeff_lj * flav.ewaldcoeff_lj;
+                    eLJAcc += static_cast<EAcc>(c6self / 12.0 * (b2 * b2 * b2 / 6.0));
+                }
+            }
+            else if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+            {
+                float q2 = 0.0f;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    q2 += qi[c] * qi[c];
+                }
+                eElAcc -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
+            }
+            else
+            {
+                // The self term is the largest energy contribution and cancels
+                // most of the pair sum: evaluate it with FP64 charges and constants.
+                const double* pD = a.sysParamsD + 8 * sys;
+                double q2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
+                {
+                    const double q = loadF4(xqc, (s.sci * c_ncps + c) * c_cl + ti).w;
+                    q2 += q * q;
+                }
+                eElAcc -= pD[1] * q2 * pD[2] * c_oneSqrtPi;
+            }
+        }
+    }
+
+    // Newton / self masking on the diagonal of the central shift (:423): on a
+    // cluster pair with ci == cj only lanes with tj > ti interact.
+    const bool upperLane = tj > ti;
+    const int firstCluster = __builtin_amdgcn_readfirstlane(s.sci * c_ncps);
+    // Constants folded for the exclusion-free path.
+    const float negBeta2Log2e = -beta2 * 1.44269504088896340736f; // exp(-b^2 r^2) = exp2(r^2 * this)
+    const float beta07 = 0.7f * beta;                              // t = 1/(1 + 0.7 beta r)
+    const float twoBetaSqrtPi = beta * static_cast<float>(2.0 * c_oneSqrtPi);
+
+    // Per-wave LDS staging of one packed group: 32 j-atoms' xq and LJ data.
+    // The next group is fetched into registers (one coalesced load per lane)
+    // before the current group's force atomics are issued, so waiting for
+    // j-data never waits for those atomics (vmcnt counts both, in order).
+    __shared__ float4 sXq[4][c_jg * c_cl];
+    __shared__ float2 sLj[4][c_jg * c_cl];
+    float4* stageXq = sXq[threadIdx.x >> 6];
+    float2* stageLj = sLj[threadIdx.x >> 6];
+    const int stageAtom = lane & 31; // j-cluster stageAtom/8, atom stageAtom%8
+
+    // Deferred j-force atomics: the reduced force component of lane ti < 3
+    // for each j-cluster of the group, issued at the start of the next group.
+    float pendV[c_jg];
+    int pendCj[c_jg];
+    unsigned int pendMask = 0u;
+
+    // One packed group of 4 j-clusters against the 8 i-clusters, instantiated
+    // with and without exclusion handling.  In a group with exclusions, only
+    // cluster pairs that hold an excluded lane take the interaction-bit path;
+    // the others, like every pair of an exclusion-free group, take the erfc
+    // path, where one exp(-z^2) serves both Coulomb force and energy.
+    // Energies are summed in FP32 over the group and added in FP64 per group;
+    // the LJ energy is kept as 12*E without its constant shift, plus the
+    // sums of C6 and C12 that carry the shift.
+    auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
+        constexpr bool hasExcl = decltype(withExcl)::value;
+        float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
+        [[maybe_unused]] float eLJg = 0.0f;
+#pragma unroll
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if (!(imask & (0xFFu << (jm * c_ncps))))
+            {
+                continue;
+            }
+            const int cj = (jm == 0) ? cj0 : (jm == 1) ? cj1 : (jm == 2) ? cj2 : cj3;
+            const float4 xqj = stageXq[jm * c_cl + tj];
+            const float2 ljStaged = stageLj[jm * c_cl + tj];
+            const int typej = __float_as_int(ljStaged.x);
+            const float2 ljcj = ljStaged;
+            [[maybe_unused]] float gcj = 0.0f;
+            if constexpr (generic && vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+            {
+                gcj = flavLjPme<kFlav>(flav) ? flav.gridC6[typej] : 0.0f;
+            }
+            float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
+            unsigned long long anyInRange = 0ull;
+#pragma unroll
+            for (int c = 0; c < c_ncps; c++)
+            {
+                const int bit = jm * c_ncps + c;
+                if (!(imask & (1u << bit)))
+                {
+                    continue;
+                }
+                const float dx = xi[c] - xqj.x;
+                const float dy = yi[c] - xqj.y;
+                const float dz = zi[c] - xqj.z;
+                float r2 = norm2Fixed(dx, dy, dz);
+                // Cluster pairs on the diagonal of the central shift always carry
+                // the Newton mask in their exclusion words, so they take the
+                // interaction-bit path, the only one that needs the j > i test.
+                bool exclHere = false;
+                if constexpr (hasExcl)
+                {
+                    exclHere = __builtin_amdgcn_ballot_w64(((wexcl >> bit) & 1u) == 0u) != 0ull;
+                }
+                bool inRange;
+                if (hasExcl && exclHere)
+                {
+                    const bool diag = centralShift && (firstCluster + c == cj);
+                    inRange = (r2 < rcoulomb_sq) && (!diag || upperLane);
+                }
+                else
+                {
+                    inRange = r2 < rcoulomb_sq;
+                }
+                anyInRange |= __builtin_amdgcn_ballot_w64(inRange);
+                if (inRange)
+                {
+                    float c6, c12;
+                    float sigma = 0.0f, epsilon = 0.0f;
+                    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                    {
+                        // One LDS read: the i row offset is a register.
+                        const float2 p = sNbfp[tOffI[c] + typej];
+                        c6 = p.x;
+                        c12 = p.y;
+                    }
+                    else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                    {
+                        const float2 ljci = stageLi[c * c_cl + ti];
+                        c6 = ljci.x * ljcj.x;
+                        c12 = ljci.y * ljcj.y;
+                    }
+                    else
+                    {
+                        const float2 ljci = stageLi[c * c_cl + ti];
+                        sigma = ljci.x + ljcj.x;
+                        epsilon = ljci.y * ljcj.y;
+                        const float sigma2 = sigma * sigma;
+                        const float sigma6 = sigma2 * sigma2 * sigma2;
+                        c6 = epsilon * sigma6;
+                        c12 = c6 * sigma6;
+                    }
+                    r2 = __builtin_amdgcn_fmed3f(r2, NBNXM_MIN_DISTANCE_SQUARED, rcoulomb_sq);
+                    const float inv_r = __builtin_amdgcn_rsqf(r2);
+                    const float inv_r2 = inv_r * inv_r;
+                    const float qq = qi[c] * xqj.w;
+                    float F_invr;
+                    if constexpr (generic)
+                    {
+                        const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
+                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG);
+                    }
+                    else if (hasExcl && exclHere)
+                    {
+                        // The reference form with the interaction bit (:548-659).
+                        const float int_bit = static_cast<float>((wexcl >> bit) & 1u);
+                        const float inv_r6 = int_bit * (inv_r2 * inv_r2 * inv_r2);
+                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                        {
+                            const float sig_r = sigma * inv_r;
+                            const float sig_r2 = sig_r * sig_r;
+                            const float sig_r6 = int_bit * (sig_r2 * sig_r2 * sig_r2);
+                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
+                        }
+                        else
+                        {
+                            F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                        }
+                        const float z2 = beta2 * r2;
+                        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(z2) * beta3);
+                        if constexpr (computeEnergy)
+                        {
+                            eLJ12 = __builtin_fmaf(inv_r6, __builtin_fmaf(c12, inv_r6, -2.0f * c6), eLJ12);
+                            eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
+                            eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
+                            eElG += qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
+                        }
+                    }
+                    else
+                    {
+                        const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                        {
+                            const float sig_r = sigma * inv_r;
+                            const float sig_r2 = sig_r * sig_r;
+                            const float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2 + qq * (inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
+                        }
+                        else if constexpr (computeEnergy)
+                        {
+                            // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2, V/qq = e R / r - sh_ewald,
+                            // with e = exp(-beta^2 r^2) and erfc(beta r) = e R.
+                            const float r = r2 * inv_r;
+                            const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(beta07, r, 1.0f));
+                            const float R = erfcxPolyT(t);
+                            const float e = __builtin_amdgcn_exp2f(r2 * negBeta2Log2e);
+                            const float w = qq * e;
+                            const float u = __builtin_fmaf(R, inv_r, twoBetaSqrtPi);
+                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                            F_invr = __builtin_fmaf(inv_r6, tl, w * u) * inv_r2;
+                            eLJ12 = __builtin_fmaf(inv_r6, tl - c6, eLJ12);
+                            eSumC6 += c6;
+                            eSumC12 += c12;
+                            eElG = __builtin_fmaf(qq, __builtin_fmaf(e * R, inv_r, -ewaldShift), eElG);
+                        }
+                        else
+                        {
+                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                            const float coul = qq * __builtin_fmaf(pmecorrF(beta2 * r2), beta3, inv_r2 * inv_r);
+                            F_invr = __builtin_fmaf(inv_r6 * tl, inv_r2, coul);
+                        }
+                    }
+                    fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
+                    fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
+                    fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
+                    fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
+                    fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
+                    fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
+                }
+            }
+            // j-force: sum over the 8 i-lanes, skipped when no pair was inside
+            // the cut-off; lanes ti = 0, 1, 2 keep x, y, z for the deferred
+            // atomic (consecutive lanes on consecutive addresses coalesce).
+            if (anyInRange != 0ull)
+            {
+                fcjX = reduceOver8Lanes(fcjX);
+                fcjY = reduceOver8Lanes(fcjY);
+                fcjZ = reduceOver8Lanes(fcjZ);
+                pendV[jm] = (ti == 0) ? fcjX : ((ti == 1) ? fcjY : fcjZ);
+                pendCj[jm] = cj;
+                pendMask |= 1u << jm;
+            }
+        }
+        if constexpr (computeEnergy && generic)
+        {
+            eLJAcc += static_cast<EAcc>(eLJg);
+            eElAcc += static_cast<EAcc>(eElG);
+        }
+        else if constexpr (computeEnergy)
+        {
+            const float eLJ = (eLJ12 + cpotRep * eSumC12 - 2.0f * cpotDisp * eSumC6) * (1.0f / 12.0f);
+            eLJAcc += static_cast<EAcc>(eLJ);
+            eElAcc += static_cast<EAcc>(eElG);
+        }
+    };
+
+    // Lanes that issue the j-force atomics (ti = 0, 1, 2 for x, y, z) and the
+    // lane part of the address: component ti of j-atom tj of a cluster.
+    const bool jForceLane = ti < 3;
+    const int jForceLaneOffset = 3 * tj + ti;
+    auto flushPending = [&]() {
+#pragma unroll
+        for (int jm = 0; jm < c_jg; jm++)
+        {
+            if ((pendMask >> jm) & 1u)
+            {
+                if (jForceLane)
+                {
+                    // Uniform cluster base (scalar) plus a 32-bit lane offset.
+                    const size_t base = static_cast<size_t>(pendCj[jm]) * (3 * c_cl);
+                    if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                    {
+                        atomicAdd(a.fFloat + base + jForceLaneOffset, pendV[jm]);
+                    }
+                    else
+                    {
+                        atomicAdd(a.fFixed + base + jForceLaneOffset, toFixed(pendV[jm]));
+                    }
+                }
+            }
+        }
+        pendMask = 0u;
+    };
+
+    auto nextNonEmpty = [&](int g) {
+        while (g < s.cjPackedEnd && cjList[g].imei[0].imask == 0u)
+        {
+            g++;
+        }
+        return g;
+    };
+    // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
+    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool stageK1 = (stageAtom >> 3) == 1;
+    const bool stageK2 = (stageAtom >> 3) == 2;
+    const bool stageK3 = (stageAtom >> 3) == 3;
+    unsigned int pfExcl = 0xFFFFFFFFu;
+    auto prefetch = [&](int g) {
+        // Scalar loads first, then a per-lane select (a select of the
+        // addresses would become a vector load whose wait also waits for the
+        // in-flight force atomics).
+        const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
+        const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
+        const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
+        const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
+        const int cjl = stageK1 ? g1 : (stageK2 ? g2 : (stageK3 ? g3 : g0));
+        const int atom = cjl * c_cl + (stageAtom & 7);
+        if (lane < 32)
+        {
+            pf = loadF4(xqc, atom);
+        }
+        else if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+        {
+            pf = make_float4(__int_as_float(asConst(a.atomTypes)[atom]), 0.0f, 0.0f, 0.0f);
+        }
+        else
+        {
+            const float2 lj = loadF2(asConst(a.ljComb), atom);
+            pf = make_float4(lj.x, lj.y, 0.0f, 0.0f);
+        }
+        const int ex = cjList[g].imei[0].excl_ind;
+        pfExcl = (ex != 0) ? asConst(a.excl)[ex].pair[lane] : 0xFFFFFFFFu;
+    };
+    unsigned int stagedExcl = 0xFFFFFFFFu;
+    auto stage = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 32)
+        {
+            stageXq[stageAtom] = pf;
+        }
+        else
+        {
+            stageLj[stageAtom] = make_float2(pf.x, pf.y);
+        }
+        stagedExcl = pfExcl;
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    int jp = nextNonEmpty(s.cjPackedBegin);
+    if (jp < s.cjPackedEnd)
+    {
+        prefetch(jp);
+        stage();
+    }
+    while (jp < s.cjPackedEnd)
+    {
+        const unsigned int imask = cjList[jp].imei[0].imask;
+        const int exclInd = cjList[jp].imei[0].excl_ind;
+        const int cj0 = cjList[jp].cj[0];
+        const int cj1 = cjList[jp].cj[1];
+        const int cj2 = cjList[jp].cj[2];
+        const int cj3 = cjList[jp].cj[3];
+        const unsigned int wexcl = stagedExcl;
+        // Atomics of the previous group, then the next group's loads.
+        flushPending();
+        const int jn = nextNonEmpty(jp + 1);
+        if (jn < s.cjPackedEnd)
+        {
+            prefetch(jn);
+        }
+        if (exclInd != 0)
+        {
+            group(cj0, cj1, cj2, cj3, imask, wexcl, std::true_type{});
+        }
+        else
+        {
+            group(cj0, cj1, cj2, cj3, imask, 0xFFFFFFFFu, std::false_type{});
+        }
+        if (jn < s.cjPackedEnd)
+        {
+            stage();
+        }
+        jp = jn;
+    }
+    flushPending();
+
+    // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
+    double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
+#pragma unroll
+    for (int c = 0; c < c_ncps; c++)
+    {
+        const float fx = reduceOverStride8(fciX[c]);
+        const float fy = reduceOverStride8(fciY[c]);
+        const float fz = reduceOverStride8(fciZ[c]);
+        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+        if (tj < 3)
+        {
+            const float v = (tj == 0) ? fx : ((tj == 1) ? fy : fz);
+            if (v != 0.0f)
+            {
+                if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                {
+                    atomicAdd(a.fFloat + 3 * ai + tj, v);
+                }
+                else
+                {
+                    atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v));
+                }
+            }
+        }
+        fsX += static_cast<double>(fx);
+        fsY += static_cast<double>(fy);
+        fsZ += static_cast<double>(fz);
+    }
+
+    if (computeEnergy || a.computeFshift)
+    {
+        // Every lane of an i-column holds the column's sums: take tj == 0 lanes.
+        const double keep = (tj == 0) ? 1.0 : 0.0;
+        double v0 = 0.0, v1 = 0.0;
+        if constexpr (computeEnergy)
+        {
+            v0 = reduceWaveD(static_cast<double>(eLJAcc));
+            v1 = reduceWaveD(static_cast<double>(eElAcc));
+        }
+        const bool doShift = a.computeFshift && !centralShift;
+        const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
+        const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
+        const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
+        if (lane < 8)
+        {
+            const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
+            a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+        }
+    }
+}
+
+// Largest LJ type table staged in LDS: 64 types (+ dummy), 33.8 KB.
+constexpr size_t c_maxLdsTypeTableBytes = sizeof(float2) * c_maxLdsTableTypes * c_maxLdsTableTypes;
+
+template<bool E, int V, int A, int kFlav>
+hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    const int wpb = cfg.wavesPerBlock;
+    const int blocks = (a.nsci + wpb - 1) / wpb;
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    // The type table kernel stages the whole table in LDS.
+    const size_t ldsBytes = (V == NBNXM_VDW_CUT_TYPE_TABLE) ? sizeof(float2) * a.numTypes * a.numTypes : 0;
+    if (ldsBytes > c_maxLdsTypeTableBytes)
+    {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
+    return hipGetLastError();
+}
+
+// FP32 accumulation modes (float atomics or int64 fixed point) of one flavour class.
+template<int kFlav>
+hipError_t launchTunedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
+{
+    auto acc = [&](auto e, auto v) -> hipError_t {
+        constexpr bool E = decltype(e)::value;
+        constexpr int V = decltype(v)::value;
+        switch (accumulation)
+        {
+            case NBNXM_ACCUM_FLOAT: return launchTunedKernel<E, V, NBNXM_ACCUM_FLOAT, kFlav>(a, flav, cfg, stream);
+            case NBNXM_ACCUM_FIXED64: return launchTunedKernel<E, V, NBNXM_ACCUM_FIXED64, kFlav>(a, flav, cfg, stream);
+            default: return hipErrorInvalidValue;
+        }
+    };
+    auto vdw = [&](auto e) -> hipError_t {
+        switch (vdwType)
+        {
+            case NBNXM_VDW_CUT_TYPE_TABLE: return acc(e, std::integral_constant<int, NBNXM_VDW_CUT_TYPE_TABLE>{});
+            case NBNXM_VDW_CUT_COMB_GEOM: return acc(e, std::integral_constant<int, NBNXM_VDW_CUT_COMB_GEOM>{});
+            case NBNXM_VDW_CUT_COMB_LB: return acc(e, std::integral_constant<int, NBNXM_VDW_CUT_COMB_LB>{});
+            default: return hipErrorInvalidValue;
+        }
+    };
+    return computeEnergy ? vdw(std::true_type{}) : vdw(std::false_type{});
+}
+
+} // namespace
+} // namespace nbnxm_amd

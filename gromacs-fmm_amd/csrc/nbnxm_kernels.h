@@ -46,6 +46,25 @@ enum class KernelFlavor : int
     VF = 1
 };
 
+// Launch tuning chosen per size bucket by the autotuner.
+struct NbLaunchConfig
+{
+    int wavesPerBlock = 4;
+};
+
+// Flavour classes of the tuned list walk: the tuned flavour, the run-time
+// flavour (any combination, read from NbFlavourParams), and combinations
+// compiled in their own instantiation because they are common: reaction field
+// with the LJ potential shift (plain cut-off included), and analytical Ewald
+// with the LJ force switch; both with rvdw = rcoulomb, no LJ-PME, no table.
+enum FlavourClass : int
+{
+    c_flavTuned = 0,
+    c_flavRuntime = 1,
+    c_flavRfPotShift = 2,
+    c_flavEwaldForceSwitch = 3
+};
+
 // Flavour of the generic kernel: the interaction constants of one launch
 // (nbnxm_interaction_const_t, wave-uniform), plus the per-type LJ-PME grid
 // factors sqrt(6 C6_ii) on the device in FP32 and FP64.
@@ -66,6 +85,7 @@ struct NbFlavourParams
     double ewaldTableScale = 0;
     const float* ewaldTable = nullptr;
     const double* ewaldTableD = nullptr;
+    int flavourClass = c_flavRuntime;
 };
 
 // True for the flavour of the tuned kernels: analytical Ewald + LJ potential
@@ -78,17 +98,22 @@ inline bool isDefaultFlavour(const nbnxm_interaction_const_t& ic)
 
 NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic);
 
+// The tuned list walk for a flavour other than the tuned one (FP32
+// accumulation modes; the class comes from flav.flavourClass).
+hipError_t launchNonbondedFlavoured(const NbKernelArgs& a,
+                                    const NbFlavourParams& flav,
+                                    int accumulation,
+                                    int vdwType,
+                                    bool computeEnergy,
+                                    const NbLaunchConfig& cfg,
+                                    hipStream_t stream);
+
 // Every flavour (one wave per entry, FP32 or, with NBNXM_ACCUM_FP64, FP64).
 hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream);
 
 // Types (with the dummy type) of the largest LJ table the tuned kernel stages in LDS.
 constexpr int c_maxLdsTableTypes = 65;
 
-// Launch tuning chosen per size bucket by the autotuner.
-struct NbLaunchConfig
-{
-    int wavesPerBlock = 4;
-};
 
 // flav == nullptr: the tuned flavour (per-system parameters from sysParams*);
 // otherwise the generic FP32 instantiation with the flavour of flav (single
