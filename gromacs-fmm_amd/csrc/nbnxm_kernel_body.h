@@ -45,6 +45,36 @@ __device__ __forceinline__ float reduceOver8Lanes(float v)
     return v;
 }
 
+// Sums of (x, y, z) over the 8 consecutive lanes of a group with 8 VALU
+// instructions instead of 9 DPP adds and 2 selects: a reduce-scatter.  After
+// the mirror step the lower 4 lanes carry x and y pair sums, the upper 4 the z
+// pair sums; the xor-2 step splits the lower quad into x (lanes 0, 1) and
+// y (lanes 2, 3).  Returns x in group lanes 0 and 1, y in lanes 2 and 3, z in
+// lanes 4 to 7.  lower = (lane & 7) < 4, xLanes = (lane & 7) < 2,
+// yLanes = (lane & 7) is 2 or 3.
+__device__ __forceinline__ float reduceScatterOver8Lanes(float x, float y, float z, bool lower, bool xLanes, bool yLanes)
+{
+    // row_half_mirror pairs lane k with 7 - k: a lower lane reads its upper
+    // partner's x, an upper lane its lower partner's z.
+    const float r1 = (lower ? x : z) + dppF<0x141>(lower ? z : x);
+    const float r2 = y + dppF<0x141>(y);
+    // quad_perm [2,3,0,1] pairs k with k ^ 2: lanes 0, 1 read x from lanes
+    // 2, 3, which read y from lanes 0, 1; upper lanes keep z.
+    const float r3 = (yLanes ? r2 : r1) + dppF<0x4E>(xLanes ? r2 : r1);
+    return r3 + dppF<0xB1>(r3); // quad_perm [1,0,3,2]
+}
+
+// OR of a 32-bit value over the whole wave, as a wave-uniform value.
+__device__ __forceinline__ unsigned int waveOr(unsigned int v)
+{
+    v |= static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v |= static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v |= static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false)); // row_half_mirror
+    v |= static_cast<unsigned int>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xF, 0xF, false)); // row_mirror
+    return static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 0) | __builtin_amdgcn_readlane(static_cast<int>(v), 16)
+                                     | __builtin_amdgcn_readlane(static_cast<int>(v), 32) | __builtin_amdgcn_readlane(static_cast<int>(v), 48));
+}
+
 // Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), 16, 32.
 __device__ __forceinline__ float reduceOverStride8(float v)
 {
@@ -110,15 +140,24 @@ __device__ __forceinline__ float erfcxPolyT(float t)
     return p;
 }
 
-// No occupancy floor: capping the force-only kernel at 128 VGPRs (4 waves per
-// SIMD) spills, and measured 25 % slower than its 3-wave allocation.
+// Occupancy floor: 3 waves per SIMD (at most 168 VGPRs) for the run-time
+// flavour's force-only kernels with a combination rule, which the allocator
+// otherwise leaves one register above it; no floor elsewhere (capping the
+// tuned force-only kernel at 128 VGPRs, 4 waves per SIMD, spills and measured
+// 25 % slower than its 3-wave allocation).
 // kFlav = c_flavTuned is the tuned flavour (analytical Ewald, LJ potential
 // shift, rvdw = rcoulomb); every other flavour class runs the same list walk
 // with the pair interaction of flavour_math.h: fixed at compile time for the
 // common classes, read from flav (wave-uniform, scalar branches) for
 // c_flavRuntime.
+template<bool computeEnergy, int vdwType, int kFlav>
+constexpr int minWavesPerSimd()
+{
+    return (kFlav == c_flavRuntime && !computeEnergy && vdwType != NBNXM_VDW_CUT_TYPE_TABLE) ? 3 : 1;
+}
+
 template<bool computeEnergy, int vdwType, int accumulation, int kFlav>
-__global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
+__global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
 {
     constexpr bool generic = kFlav != c_flavTuned;
     const int lane = threadIdx.x & 63;
@@ -275,7 +314,11 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
 
     // Newton / self masking on the diagonal of the central shift (:423): on a
     // cluster pair with ci == cj only lanes with tj > ti interact.
+    // Lanes with tj > ti: the pairs of a diagonal cluster pair that interact.
     const bool upperLane = tj > ti;
+    const bool lowerLane = ti < 4;
+    const bool xLane = ti < 2;
+    const bool yLane = (ti >> 1) == 1;
     const int firstCluster = __builtin_amdgcn_readfirstlane(s.sci * c_ncps);
     // Constants folded for the exclusion-free path.
     const float negBeta2Log2e = -beta2 * 1.44269504088896340736f; // exp(-b^2 r^2) = exp2(r^2 * this)
@@ -308,6 +351,9 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
     // sums of C6 and C12 that carry the shift.
     auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
         constexpr bool hasExcl = decltype(withExcl)::value;
+        // Bit b set: some lane of cluster pair b has an excluded (or Newton
+        // masked) pair.
+        const unsigned int exclBits = hasExcl ? waveOr(~wexcl) : 0u;
         float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
         [[maybe_unused]] float eLJg = 0.0f;
 #pragma unroll
@@ -344,21 +390,14 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
                 // Cluster pairs on the diagonal of the central shift always carry
                 // the Newton mask in their exclusion words, so they take the
                 // interaction-bit path, the only one that needs the j > i test.
-                bool exclHere = false;
-                if constexpr (hasExcl)
+                // Wave-uniform: exclHere from the group's OR of its exclusion
+                // words, diag from list indices.
+                const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
+                if (exclHere && centralShift && (firstCluster + c == cj))
                 {
-                    exclHere = __builtin_amdgcn_ballot_w64(((wexcl >> bit) & 1u) == 0u) != 0ull;
+                    r2 = upperLane ? r2 : 1.0e30f; // lanes with tj <= ti leave the cut-off
                 }
-                bool inRange;
-                if (hasExcl && exclHere)
-                {
-                    const bool diag = centralShift && (firstCluster + c == cj);
-                    inRange = (r2 < rcoulomb_sq) && (!diag || upperLane);
-                }
-                else
-                {
-                    inRange = r2 < rcoulomb_sq;
-                }
+                const bool inRange = r2 < rcoulomb_sq;
                 anyInRange |= __builtin_amdgcn_ballot_w64(inRange);
                 if (inRange)
                 {
@@ -470,10 +509,7 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
             // atomic (consecutive lanes on consecutive addresses coalesce).
             if (anyInRange != 0ull)
             {
-                fcjX = reduceOver8Lanes(fcjX);
-                fcjY = reduceOver8Lanes(fcjY);
-                fcjZ = reduceOver8Lanes(fcjZ);
-                pendV[jm] = (ti == 0) ? fcjX : ((ti == 1) ? fcjY : fcjZ);
+                pendV[jm] = reduceScatterOver8Lanes(fcjX, fcjY, fcjZ, lowerLane, xLane, yLane);
                 pendCj[jm] = cj;
                 pendMask |= 1u << jm;
             }
@@ -493,8 +529,8 @@ __global__ __launch_bounds__(256, 1) void nbnxmNonbondedKernel(const NbKernelArg
 
     // Lanes that issue the j-force atomics (ti = 0, 1, 2 for x, y, z) and the
     // lane part of the address: component ti of j-atom tj of a cluster.
-    const bool jForceLane = ti < 3;
-    const int jForceLaneOffset = 3 * tj + ti;
+    const bool jForceLane = (ti & 1) == 0 && ti < 6;
+    const int jForceLaneOffset = 3 * tj + (ti >> 1);
     auto flushPending = [&]() {
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)

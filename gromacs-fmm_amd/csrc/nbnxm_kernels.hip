@@ -7,8 +7,8 @@
 // the cluster-pair split removed so the whole 8x8 pair is one wave.
 //  - i-cluster coordinates, charges and LJ parameters live in registers for the
 //    whole entry (the reference stages them in shared memory, :330-351);
-//  - j-forces are reduced over the 8 i-lanes with three DPP adds per component
-//    (row_half_mirror, quad_perm xor1, quad_perm xor2) instead of warp shuffles
+//  - j-forces are reduced over the 8 i-lanes with a DPP reduce-scatter of the
+//    three components (8 VALU instructions) instead of warp shuffles
 //    (:677, nbnxm_cuda_kernel_utils.cuh:467-496);
 //  - forces are accumulated as int64 fixed point (2^-32 units), so every
 //    evaluation is bit-reproducible whatever the atomic order;
