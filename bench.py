@@ -27,7 +27,7 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 # + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy) from two rocprofv3 --pmc
 # runs, profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
 # inside the timed run, so the measured figure is carried here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.901e9
+TRAFFIC_BYTES_PER_LAUNCH = 1.924e9
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_VF = 107       # Ewald + LJ force+energy (src/gromacs/gmxlib/nrnb.cpp:90-103)
 RC, RLIST_OUTER, RLIST_INNER, EWALD_RTOL, SIGMA = 1.0, 1.1, 1.04, 1e-5, 0.002

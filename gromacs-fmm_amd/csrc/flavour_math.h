@@ -61,10 +61,10 @@ struct FlavourConsts
 template<int kFlav>
 struct FlavourTraits
 {
-    static constexpr bool fixed = kFlav == c_flavRfPotShift || kFlav == c_flavEwaldForceSwitch;
-    static constexpr int elec = kFlav == c_flavRfPotShift ? NBNXM_ELEC_RF : (kFlav == c_flavEwaldForceSwitch ? NBNXM_ELEC_EWALD : -1);
-    static constexpr int modifier = kFlav == c_flavRfPotShift ? NBNXM_VDW_MOD_POTSHIFT
-                                                             : (kFlav == c_flavEwaldForceSwitch ? NBNXM_VDW_MOD_FORCESWITCH : -1);
+    static constexpr bool fixed = kFlav == c_flavRfPotShift || kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldTabPotShift;
+    static constexpr int elec = kFlav == c_flavRfPotShift ? NBNXM_ELEC_RF : (fixed ? NBNXM_ELEC_EWALD : -1);
+    static constexpr int modifier = kFlav == c_flavEwaldForceSwitch ? NBNXM_VDW_MOD_FORCESWITCH : (fixed ? NBNXM_VDW_MOD_POTSHIFT : -1);
+    static constexpr bool table = kFlav == c_flavEwaldTabPotShift;
 };
 
 template<int kFlav>
@@ -106,8 +106,8 @@ __device__ __forceinline__ bool flavModifier(const NbFlavourParams& f, int modif
     }
 }
 
-// The LJ cut-off check of twin cut-offs and the tabulated correction exist
-// only in run-time flavours.
+// The LJ cut-off check of twin cut-offs exists only in run-time flavours; the
+// tabulated correction in those and in its own class.
 template<int kFlav>
 __device__ __forceinline__ bool flavTwin(const NbFlavourParams& f)
 {
@@ -126,7 +126,7 @@ __device__ __forceinline__ bool flavTable(const NbFlavourParams& f)
 {
     if constexpr (FlavourTraits<kFlav>::fixed)
     {
-        return false;
+        return FlavourTraits<kFlav>::table;
     }
     else
     {
@@ -175,7 +175,8 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
                                           Real c12,
                                           Real c6grid,
                                           Real& eLJ,
-                                          Real& eEl)
+                                          Real& eEl,
+                                          const float* tableF = nullptr)
 {
     constexpr Real oneSixth = Real(1) / Real(6);
     constexpr Real oneTwelveth = Real(1) / Real(12);
@@ -246,8 +247,11 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
         Real corr;
         if constexpr (std::is_same_v<Real, float>)
         {
-            const float d0 = flav.ewaldTable[index];
-            const float d1 = flav.ewaldTable[index + 1];
+            // The FP32 table: the caller's LDS copy in the tabulated class
+            // (a compile-time choice, so its reads stay LDS reads).
+            const float* t = FlavourTraits<kFlav>::table ? tableF : flav.ewaldTable;
+            const float d0 = t[index];
+            const float d1 = t[index + 1];
             corr = __builtin_fmaf(fraction, d1, __builtin_fmaf(-fraction, d0, d0));
         }
         else

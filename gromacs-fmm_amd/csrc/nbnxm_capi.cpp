@@ -771,6 +771,7 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
         nb->flavour.ewaldTableScale = scale;
         nb->flavour.ewaldTable = nb->ewaldTable.ptr;
         nb->flavour.ewaldTableD = nb->ewaldTableD.ptr;
+        nb->flavour.ewaldTableSize = np + 1;
     }
     nb->accumulation = accumulation;
     rc = uploadParams(nb.get());

@@ -323,6 +323,7 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
     p.tabulatedEwald = ic.tabulatedEwald;
     // Flavour classes compiled with their flavour fixed (one instantiation each).
     const bool plain = !ic.ljPme && !ic.tabulatedEwald && ic.rvdw == ic.rcoulomb;
+    const bool plainTabulated = !ic.ljPme && ic.tabulatedEwald && ic.rvdw == ic.rcoulomb;
     if (plain && ic.eeltype == NBNXM_ELEC_RF && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT)
     {
         p.flavourClass = c_flavRfPotShift;
@@ -330,6 +331,10 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
     else if (plain && ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_FORCESWITCH)
     {
         p.flavourClass = c_flavEwaldForceSwitch;
+    }
+    else if (plainTabulated && ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT)
+    {
+        p.flavourClass = c_flavEwaldTabPotShift;
     }
     else
     {

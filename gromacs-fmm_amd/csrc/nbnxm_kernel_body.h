@@ -149,6 +149,13 @@ __device__ __forceinline__ float erfcxPolyT(float t)
 // with the pair interaction of flavour_math.h: fixed at compile time for the
 // common classes, read from flav (wave-uniform, scalar branches) for
 // c_flavRuntime.
+// Bytes of the LJ type table in dynamic LDS, rounded up to 16 so that what
+// follows it stays 16-byte aligned.
+__host__ __device__ constexpr size_t typeTableLdsBytes(int numTypes)
+{
+    return (sizeof(float2) * numTypes * numTypes + 15) / 16 * 16;
+}
+
 template<bool computeEnergy, int vdwType, int kFlav>
 constexpr int minWavesPerSimd()
 {
@@ -168,15 +175,28 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     // whole block before any wave leaves): per-pair (6C6, 12C12) reads become
     // LDS reads, which do not wait behind the in-flight force atomics as a
     // vector load's vmcnt wait would.
-    [[maybe_unused]] float2* sNbfp = nullptr;
-    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+    // The tabulated Ewald class stages its FP32 correction table after it, so
+    // that the two table reads per pair are LDS reads too.
+    constexpr bool ldsEwaldTable = FlavourTraits<kFlav>::table;
+    HIP_DYNAMIC_SHARED(float4, sDyn);
+    [[maybe_unused]] float2* sNbfp = reinterpret_cast<float2*>(sDyn);
+    [[maybe_unused]] float* sEwald = reinterpret_cast<float*>(reinterpret_cast<char*>(sDyn) + typeTableLdsBytes(vdwType == NBNXM_VDW_CUT_TYPE_TABLE ? a.numTypes : 0));
+    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE || ldsEwaldTable)
     {
-        HIP_DYNAMIC_SHARED(float2, sNbfpDyn);
-        sNbfp = sNbfpDyn;
-        const int numEntries = a.numTypes * a.numTypes;
-        for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
+        if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
         {
-            sNbfp[i] = a.nbfp[i];
+            const int numEntries = a.numTypes * a.numTypes;
+            for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
+            {
+                sNbfp[i] = a.nbfp[i];
+            }
+        }
+        if constexpr (ldsEwaldTable)
+        {
+            for (int i = threadIdx.x; i < flav.ewaldTableSize; i += blockDim.x)
+            {
+                sEwald[i] = flav.ewaldTable[i];
+            }
         }
         __syncthreads();
     }
@@ -433,7 +453,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     if constexpr (generic)
                     {
                         const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
-                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG);
+                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG,
+                                                                        ldsEwaldTable ? sEwald : nullptr);
                     }
                     else if (hasExcl && exclHere)
                     {
@@ -699,6 +720,9 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
 
 // Largest LJ type table staged in LDS: 64 types (+ dummy), 33.8 KB.
 constexpr size_t c_maxLdsTypeTableBytes = sizeof(float2) * c_maxLdsTableTypes * c_maxLdsTableTypes;
+// Largest tabulated Ewald correction table staged in LDS (the tabulated class
+// falls back to the run-time class, which reads it from global memory).
+constexpr size_t c_maxLdsEwaldTableBytes = 32768;
 
 template<bool E, int V, int A, int kFlav>
 hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav, const NbLaunchConfig& cfg, hipStream_t stream)
@@ -710,11 +734,13 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
         return hipSuccess;
     }
     // The type table kernel stages the whole table in LDS.
-    const size_t ldsBytes = (V == NBNXM_VDW_CUT_TYPE_TABLE) ? sizeof(float2) * a.numTypes * a.numTypes : 0;
-    if (ldsBytes > c_maxLdsTypeTableBytes)
+    const size_t typeBytes = (V == NBNXM_VDW_CUT_TYPE_TABLE) ? sizeof(float2) * a.numTypes * a.numTypes : 0;
+    const size_t ewaldBytes = FlavourTraits<kFlav>::table ? sizeof(float) * flav.ewaldTableSize : 0;
+    if (typeBytes > c_maxLdsTypeTableBytes || ewaldBytes > c_maxLdsEwaldTableBytes)
     {
         return hipErrorInvalidValue;
     }
+    const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes;
     hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
     return hipGetLastError();
 }

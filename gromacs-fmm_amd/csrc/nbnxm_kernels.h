@@ -55,14 +55,16 @@ struct NbLaunchConfig
 // Flavour classes of the tuned list walk: the tuned flavour, the run-time
 // flavour (any combination, read from NbFlavourParams), and combinations
 // compiled in their own instantiation because they are common: reaction field
-// with the LJ potential shift (plain cut-off included), and analytical Ewald
-// with the LJ force switch; both with rvdw = rcoulomb, no LJ-PME, no table.
+// with the LJ potential shift (plain cut-off included), analytical Ewald with
+// the LJ force switch, and tabulated Ewald with the LJ potential shift; all
+// with rvdw = rcoulomb and no LJ-PME.
 enum FlavourClass : int
 {
     c_flavTuned = 0,
     c_flavRuntime = 1,
     c_flavRfPotShift = 2,
-    c_flavEwaldForceSwitch = 3
+    c_flavEwaldForceSwitch = 3,
+    c_flavEwaldTabPotShift = 4
 };
 
 // Flavour of the generic kernel: the interaction constants of one launch
@@ -85,6 +87,7 @@ struct NbFlavourParams
     double ewaldTableScale = 0;
     const float* ewaldTable = nullptr;
     const double* ewaldTableD = nullptr;
+    int ewaldTableSize = 0; // entries, padding included
     int flavourClass = c_flavRuntime;
 };
 
@@ -107,6 +110,15 @@ hipError_t launchNonbondedFlavoured(const NbKernelArgs& a,
                                     bool computeEnergy,
                                     const NbLaunchConfig& cfg,
                                     hipStream_t stream);
+
+// The run-time flavour class of the tuned list walk (nbnxm_kernels_runtime.hip).
+hipError_t launchNonbondedRuntimeFlavour(const NbKernelArgs& a,
+                                         const NbFlavourParams& flav,
+                                         int accumulation,
+                                         int vdwType,
+                                         bool computeEnergy,
+                                         const NbLaunchConfig& cfg,
+                                         hipStream_t stream);
 
 // Every flavour (one wave per entry, FP32 or, with NBNXM_ACCUM_FP64, FP64).
 hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream);
