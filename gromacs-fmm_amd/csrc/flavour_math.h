@@ -1,5 +1,6 @@
-// Pair interaction of every kernel flavour, shared by the tuned kernel's
-// generic instantiation and the standalone flavour kernel (not part of the C ABI).
+// Pair interaction of every kernel flavour, shared by the tuned list walk's
+// flavour-class instantiations and the standalone flavour kernel (not part of
+// the C ABI).
 #pragma once
 
 #include <hip/hip_runtime.h>

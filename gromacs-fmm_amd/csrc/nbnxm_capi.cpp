@@ -1126,7 +1126,7 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     }
     else if (nb->accumulation != NBNXM_ACCUM_FP64 && tableFitsLds && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE))
     {
-        // Other FP32 flavours: the tuned list walk with the generic pair interaction.
+        // Other FP32 flavours: the tuned list walk with the pair interaction of the flavour's class.
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream, &nb->flavour));
     }
     else

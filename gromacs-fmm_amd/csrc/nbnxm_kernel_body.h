@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         stageLi[lane] = loadF2(asConst(a.ljComb), ai);
     }
 
-    // LJ-PME grid factors of the i-atoms (generic flavour with the type table).
+    // LJ-PME grid factors of the i-atoms (flavour classes, with the type table).
     [[maybe_unused]] float gci[c_ncps];
     if constexpr (generic)
     {

@@ -67,7 +67,7 @@ enum FlavourClass : int
     c_flavEwaldTabPotShift = 4
 };
 
-// Flavour of the generic kernel: the interaction constants of one launch
+// Flavour of a non-tuned launch: the interaction constants of one launch
 // (nbnxm_interaction_const_t, wave-uniform), plus the per-type LJ-PME grid
 // factors sqrt(6 C6_ii) on the device in FP32 and FP64.
 struct NbFlavourParams
@@ -128,8 +128,8 @@ constexpr int c_maxLdsTableTypes = 65;
 
 
 // flav == nullptr: the tuned flavour (per-system parameters from sysParams*);
-// otherwise the generic FP32 instantiation with the flavour of flav (single
-// parameter set, FLOAT or FIXED64 accumulation, LJ-PME with the type table).
+// otherwise the FP32 instantiation of flav's flavour class (single parameter
+// set, FLOAT or FIXED64 accumulation, LJ-PME with the type table).
 hipError_t launchNonbonded(const NbKernelArgs& a,
                            int accumulation,
                            int vdwType,

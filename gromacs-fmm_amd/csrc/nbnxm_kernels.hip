@@ -664,7 +664,7 @@ hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType,
     }
     if (flav)
     {
-        // The generic instantiations read LJ-PME grid factors by type: type table only.
+        // The flavour-class instantiations read LJ-PME grid factors by type: type table only.
         if (accumulation == NBNXM_ACCUM_FP64 || (flav->ljPme && (vdwType != NBNXM_VDW_CUT_TYPE_TABLE || !flav->gridC6))
             || (flav->tabulatedEwald && (!flav->ewaldTable || !(flav->ewaldTableScale > 0))))
         {
