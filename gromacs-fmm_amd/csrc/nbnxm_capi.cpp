@@ -208,6 +208,7 @@ struct nbnxm_gpu
     DeviceBuffer<int> iota;
     DeviceBuffer<char> sortTemp;
     bool haveFreshList = false;
+    bool haveList = false; // gpu_init_pairlist has run (the list may be empty)
     bool haveSortedOrder = false;
 
     DeviceBuffer<float> fFloat;
@@ -918,6 +919,8 @@ int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list)
     }
     const int nsci = list->nsci;
     const int ncj = list->ncjPacked;
+    // Until the upload below completes there is no usable list.
+    nb->haveList = false;
     nb->nsci = nsci;
     nb->ncjPacked = ncj;
     NBNXM_HIP_CHECK(nb->sci.resize(nsci));
@@ -988,6 +991,7 @@ int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list)
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     nb->haveFreshList = true;
     nb->haveSortedOrder = false;
+    nb->haveList = true;
     return NBNXM_OK;
 }
 
@@ -1073,7 +1077,7 @@ int nbnxm_gpu_launch_kernel_pruneonly(nbnxm_gpu_t* nb, int numParts)
     {
         return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel_pruneonly: null handle or numParts < 1");
     }
-    if (nb->nsci == 0 && nb->ncjPacked == 0 && !nb->sci.ptr)
+    if (!nb->haveList)
     {
         return fail(NBNXM_ERR_STATE, "gpu_launch_kernel_pruneonly: no pair list uploaded");
     }
@@ -1086,7 +1090,7 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     {
         return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel: null handle");
     }
-    if (!nb->shiftVec.ptr || !nb->sci.ptr)
+    if (!nb->shiftVec.ptr || !nb->haveList)
     {
         return fail(NBNXM_ERR_STATE, "gpu_launch_kernel: shift vectors and pair list must be uploaded first");
     }
