@@ -117,6 +117,9 @@ struct nbnxm_force_calculator
     GridSearch search;
     nbnxm_gpu_t* gpu = nullptr;
     bool havePairlist = false;
+    // Entry split target of the previous list (0: none; -1: no list yet),
+    // reused so that an update builds its list once.
+    int splitTarget = -1;
     double box[3] = { 0, 0, 0 };
 
     // Grid-order host staging.
@@ -301,7 +304,7 @@ extern "C" int nbnxm_force_calculator_update_pairlist(nbnxm_force_calculator_t* 
     }
     const int* er = c->exclRanges.empty() ? nullptr : c->exclRanges.data();
     const int* ee = c->exclRanges.empty() ? nullptr : c->exclElements.data();
-    rc = c->search.constructPairlist(c->rlist, er, ee, 0);
+    rc = c->search.constructPairlist(c->rlist, er, ee, c->splitTarget > 0 ? c->splitTarget : 0);
     if (rc != NBNXM_OK)
     {
         return fail(rc, "force_calculator_update_pairlist: pair list construction failed");
@@ -359,19 +362,31 @@ extern "C" int nbnxm_force_calculator_update_pairlist(nbnxm_force_calculator_t* 
 
     // Split entries when there are too few for the GPU (get_nsubpair_target,
     // src/gromacs/nbnxm/pairlist.cpp:2548-2640): aim at min_ci_balanced
-    // entries, with at least 36 cluster pairs each.
+    // entries, with at least 36 cluster pairs each.  The reference estimates
+    // the target from the grid before building; here the first update builds
+    // the list unsplit to count its pairs and then split, and later updates
+    // build once with the target of the previous list.
     const int minCi = nbnxm_gpu_min_ci_balanced(c->gpu);
     nbnxm_pairlist_gpu_t view = c->search.listView();
-    if (minCi > 0 && view.nsci > 0 && view.nsci < minCi)
+    const int64_t pairs = c->search.numClusterPairs();
+    const int target = (minCi > 0 && pairs > 0) ? static_cast<int>(std::max<int64_t>(36, (pairs + minCi - 1) / minCi)) : 0;
+    if (c->splitTarget < 0)
     {
-        const int64_t pairs = c->search.numClusterPairs();
-        const int target = static_cast<int>(std::max<int64_t>(36, (pairs + minCi - 1) / minCi));
-        rc = c->search.constructPairlist(c->rlist, er, ee, target);
-        if (rc != NBNXM_OK)
+        c->splitTarget = 0;
+        if (minCi > 0 && view.nsci > 0 && view.nsci < minCi)
         {
-            return fail(rc, "force_calculator_update_pairlist: pair list construction failed");
+            rc = c->search.constructPairlist(c->rlist, er, ee, target);
+            if (rc != NBNXM_OK)
+            {
+                return fail(rc, "force_calculator_update_pairlist: pair list construction failed");
+            }
+            view = c->search.listView();
+            c->splitTarget = target;
         }
-        view = c->search.listView();
+    }
+    else if (c->splitTarget > 0)
+    {
+        c->splitTarget = target;
     }
     rc = nbnxm_gpu_init_pairlist(c->gpu, &view);
     if (rc != NBNXM_OK)

@@ -59,6 +59,9 @@ struct nbnxm_verlet
     long long numSearches = 0;
     long long numPrunes = 0;
     long long numSteps = 0;
+    // Entry split target of the last search (0: no split), reused by the next
+    // search so that a search builds its list once.
+    int splitTarget = -1;
 
     std::vector<float> hx;
     std::vector<float> xq;
@@ -94,12 +97,14 @@ struct nbnxm_verlet
     }
 
     int searchAndUpload(float* d_x, const double* newBox);
-    int buildList(int maxPairsPerEntry);
+    int buildList(int maxPairsPerEntry, bool regrid);
 };
 
 // Grid and list into `search`: on the host, or by the device search (which
-// wraps the coordinates itself) handed over to the host object.
-int nbnxm_verlet::buildList(int maxPairsPerEntry)
+// wraps the coordinates itself) handed over to the host object.  `regrid`
+// false rebuilds the host list on the grid of the previous call (same
+// coordinates, another entry split); the device search always regrids.
+int nbnxm_verlet::buildList(int maxPairsPerEntry, bool regrid)
 {
     const int* er = exclRanges.empty() ? nullptr : exclRanges.data();
     const int* ee = exclRanges.empty() ? nullptr : exclElements.data();
@@ -125,7 +130,7 @@ int nbnxm_verlet::buildList(int maxPairsPerEntry)
         }
         return gpuSearchExportSystem(gpuSearch, 0, search);
     }
-    if (maxPairsPerEntry == 0 && (rc = search.putAtomsOnGrid(box, numAtoms, hx.data())) != NBNXM_OK)
+    if (regrid && (rc = search.putAtomsOnGrid(box, numAtoms, hx.data())) != NBNXM_OK)
     {
         return fail(rc, "verlet_step: grid construction failed");
     }
@@ -142,7 +147,12 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
         NBNXM_HIP_CHECK(hipMemcpy(hx.data(), d_x, sizeof(float) * n3, hipMemcpyDeviceToHost));
     }
     std::copy(newBox, newBox + 3, box);
-    int rc = buildList(0);
+    // After the first search the split target of the previous list is used
+    // directly (the reference estimates it from the grid before building,
+    // get_nsubpair_target, src/gromacs/nbnxm/pairlist.cpp:2548-2640; here it
+    // comes from the previous list's cluster-pair count, which changes little
+    // between searches of one system).
+    int rc = buildList(splitTarget > 0 ? splitTarget : 0, true);
     if (rc != NBNXM_OK)
     {
         return rc;
@@ -198,15 +208,26 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
     // src/gromacs/nbnxm/pairlist.cpp:2548-2640, simplified as in the calculator).
     const int minCi = nbnxm_gpu_min_ci_balanced(gpu);
     nbnxm_pairlist_gpu_t view = search.listView();
-    if (minCi > 0 && view.nsci > 0 && view.nsci < minCi)
+    const int64_t pairs = search.numClusterPairs();
+    const int target = (minCi > 0 && pairs > 0) ? static_cast<int>(std::max<int64_t>(36, (pairs + minCi - 1) / minCi)) : 0;
+    if (splitTarget < 0)
     {
-        const int64_t pairs = search.numClusterPairs();
-        const int target = static_cast<int>(std::max<int64_t>(36, (pairs + minCi - 1) / minCi));
-        if ((rc = buildList(target)) != NBNXM_OK)
+        // First search: the unsplit list gave the entry and pair counts.
+        splitTarget = 0;
+        if (minCi > 0 && view.nsci > 0 && view.nsci < minCi)
         {
-            return rc;
+            if ((rc = buildList(target, false)) != NBNXM_OK)
+            {
+                return rc;
+            }
+            view = search.listView();
+            splitTarget = target;
         }
-        view = search.listView();
+    }
+    else if (splitTarget > 0)
+    {
+        // Splitting keeps the cluster-pair count: refresh the target from it.
+        splitTarget = target;
     }
     if ((rc = nbnxm_gpu_init_pairlist(gpu, &view)) != NBNXM_OK)
     {

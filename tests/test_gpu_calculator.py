@@ -40,6 +40,27 @@ def water9k_oracle(water9k):
                 atoms=atoms, plist=plist, ic=ic)
 
 
+
+def test_second_pairlist_update_reuses_split(water9k):
+    """A second update_pairlist builds its list once, with the entry split of
+    the first list: forces on new coordinates equal those of a calculator
+    whose first list is built on them, up to the FP32 summation order."""
+    announce("calculator second pair-list update")
+    opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1)
+    args = (water9k.types, nb_params(water9k), water9k.q, water9k.excl_ranges, water9k.excl_elements, opt)
+    calc = nb.ForceCalculator(*args)
+    calc.update_pairlist(water9k.x.copy(), water9k.box)
+    rng = np.random.default_rng(31)
+    x_new = water9k.x + rng.normal(0.0, 0.01, size=water9k.x.shape)
+    x2 = calc.update_pairlist(x_new.copy(), water9k.box)
+    f2, _, en2 = calc.compute(x2, water9k.box, energies=True)
+    fresh = nb.ForceCalculator(*args)
+    x3 = fresh.update_pairlist(x_new.copy(), water9k.box)
+    f3, _, en3 = fresh.compute(x3, water9k.box, energies=True)
+    assert np.array_equal(x2, x3)
+    assert rel_rms(f2, f3) <= 1e-6
+    assert en2[:2] == pytest.approx(en3[:2], rel=1e-6)
+
 def test_calculator_water9k(water9k, water9k_oracle):
     announce("calculator water9k fixed64")
     opt = nb.NBKernelOptions(pairlistCutoff=1.0, rlist=1.1)
