@@ -62,10 +62,14 @@ struct FlavourConsts
 template<int kFlav>
 struct FlavourTraits
 {
-    static constexpr bool fixed = kFlav == c_flavRfPotShift || kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldTabPotShift;
+    static constexpr bool fixed = kFlav >= c_flavRfPotShift;
     static constexpr int elec = kFlav == c_flavRfPotShift ? NBNXM_ELEC_RF : (fixed ? NBNXM_ELEC_EWALD : -1);
-    static constexpr int modifier = kFlav == c_flavEwaldForceSwitch ? NBNXM_VDW_MOD_FORCESWITCH : (fixed ? NBNXM_VDW_MOD_POTSHIFT : -1);
+    static constexpr int modifier = kFlav == c_flavEwaldForceSwitch
+                                            ? NBNXM_VDW_MOD_FORCESWITCH
+                                            : (kFlav == c_flavEwaldPotSwitch ? NBNXM_VDW_MOD_POTSWITCH : (fixed ? NBNXM_VDW_MOD_POTSHIFT : -1));
     static constexpr bool table = kFlav == c_flavEwaldTabPotShift;
+    static constexpr bool twin = kFlav == c_flavEwaldTwin;
+    static constexpr bool ljPme = kFlav == c_flavEwaldLjPme;
 };
 
 template<int kFlav>
@@ -86,7 +90,7 @@ __device__ __forceinline__ bool flavLjPme(const NbFlavourParams& f)
 {
     if constexpr (FlavourTraits<kFlav>::fixed)
     {
-        return false;
+        return FlavourTraits<kFlav>::ljPme;
     }
     else
     {
@@ -107,14 +111,14 @@ __device__ __forceinline__ bool flavModifier(const NbFlavourParams& f, int modif
     }
 }
 
-// The LJ cut-off check of twin cut-offs exists only in run-time flavours; the
-// tabulated correction in those and in its own class.
+// The LJ cut-off check of twin cut-offs and the tabulated correction: fixed by
+// their own classes, read from the parameters in run-time flavours.
 template<int kFlav>
 __device__ __forceinline__ bool flavTwin(const NbFlavourParams& f)
 {
     if constexpr (FlavourTraits<kFlav>::fixed)
     {
-        return false;
+        return FlavourTraits<kFlav>::twin;
     }
     else
     {

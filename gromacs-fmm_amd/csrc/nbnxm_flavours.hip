@@ -336,6 +336,20 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
     {
         p.flavourClass = c_flavEwaldTabPotShift;
     }
+    else if (plain && ic.eeltype == NBNXM_ELEC_EWALD && ic.vdwModifier == NBNXM_VDW_MOD_POTSWITCH)
+    {
+        p.flavourClass = c_flavEwaldPotSwitch;
+    }
+    else if (!ic.ljPme && !ic.tabulatedEwald && ic.rvdw < ic.rcoulomb && ic.eeltype == NBNXM_ELEC_EWALD
+             && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT)
+    {
+        p.flavourClass = c_flavEwaldTwin;
+    }
+    else if (ic.ljPme && !ic.tabulatedEwald && ic.rvdw == ic.rcoulomb && ic.eeltype == NBNXM_ELEC_EWALD
+             && ic.vdwModifier == NBNXM_VDW_MOD_POTSHIFT)
+    {
+        p.flavourClass = c_flavEwaldLjPme;
+    }
     else
     {
         p.flavourClass = c_flavRuntime;

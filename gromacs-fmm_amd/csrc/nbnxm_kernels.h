@@ -56,15 +56,20 @@ struct NbLaunchConfig
 // flavour (any combination, read from NbFlavourParams), and combinations
 // compiled in their own instantiation because they are common: reaction field
 // with the LJ potential shift (plain cut-off included), analytical Ewald with
-// the LJ force switch, and tabulated Ewald with the LJ potential shift; all
-// with rvdw = rcoulomb and no LJ-PME.
+// the LJ force switch, tabulated Ewald with the LJ potential shift, analytical
+// Ewald with the LJ potential switch (these four with rvdw = rcoulomb and no
+// LJ-PME), analytical Ewald + LJ potential shift with a shorter LJ cut-off
+// (twin cut-off), and analytical Ewald with LJ-PME (rvdw = rcoulomb).
 enum FlavourClass : int
 {
     c_flavTuned = 0,
     c_flavRuntime = 1,
     c_flavRfPotShift = 2,
     c_flavEwaldForceSwitch = 3,
-    c_flavEwaldTabPotShift = 4
+    c_flavEwaldTabPotShift = 4,
+    c_flavEwaldPotSwitch = 5,
+    c_flavEwaldTwin = 6,
+    c_flavEwaldLjPme = 7
 };
 
 // Flavour of a non-tuned launch: the interaction constants of one launch
@@ -110,6 +115,15 @@ hipError_t launchNonbondedFlavoured(const NbKernelArgs& a,
                                     bool computeEnergy,
                                     const NbLaunchConfig& cfg,
                                     hipStream_t stream);
+
+// Classes 5-7 of the tuned list walk (nbnxm_kernels_flavoured_b.hip).
+hipError_t launchNonbondedFlavouredB(const NbKernelArgs& a,
+                                     const NbFlavourParams& flav,
+                                     int accumulation,
+                                     int vdwType,
+                                     bool computeEnergy,
+                                     const NbLaunchConfig& cfg,
+                                     hipStream_t stream);
 
 // The run-time flavour class of the tuned list walk (nbnxm_kernels_runtime.hip).
 hipError_t launchNonbondedRuntimeFlavour(const NbKernelArgs& a,

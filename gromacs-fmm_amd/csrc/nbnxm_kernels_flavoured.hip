@@ -28,6 +28,9 @@ hipError_t launchNonbondedFlavoured(const NbKernelArgs& a,
                 return launchNonbondedRuntimeFlavour(a, flav, accumulation, vdwType, computeEnergy, cfg, stream);
             }
             return launchTunedFlavour<c_flavEwaldTabPotShift>(a, flav, accumulation, vdwType, computeEnergy, cfg, stream);
+        case c_flavEwaldPotSwitch:
+        case c_flavEwaldTwin:
+        case c_flavEwaldLjPme: return launchNonbondedFlavouredB(a, flav, accumulation, vdwType, computeEnergy, cfg, stream);
         case c_flavRuntime: return launchNonbondedRuntimeFlavour(a, flav, accumulation, vdwType, computeEnergy, cfg, stream);
         default: return hipErrorInvalidValue;
     }
