@@ -15,6 +15,25 @@ namespace nbnxm_amd
 namespace dev
 {
 
+// erfc(z) with relative accuracy for z in [0, 4.5]: erfc(z) = exp(-z^2) R(t),
+// t = 1/(1 + 0.4 z), R a degree-6 polynomial fitted to erfc(z) exp(z^2) in FP64
+// by relative least squares reweighted towards the minimax fit (max relative
+// error 2.4e-7 in FP64, 5.4e-7 evaluated in FP32 with fused multiply-adds).
+// Used for pairs without exclusions, where 1 - erf(z) would lose the small
+// erfc(z) near the cut-off.  Takes t; the caller forms it with beta folded in.
+constexpr float c_erfcxPolyScale = 0.4f;
+__device__ __forceinline__ float erfcxPolyT(float t)
+{
+    float p = -0.216648832f;
+    p = __builtin_fmaf(p, t, 0.7811663151f);
+    p = __builtin_fmaf(p, t, -0.5001354218f);
+    p = __builtin_fmaf(p, t, 0.5890582204f);
+    p = __builtin_fmaf(p, t, 0.1001023054f);
+    p = __builtin_fmaf(p, t, 0.2481535822f);
+    p = __builtin_fmaf(p, t, -0.001696089399f);
+    return p;
+}
+
 // Flavour constants in the kernel's arithmetic type, derived the way the
 // reference kernel derives them from its FP32 parameter block
 // (nbnxm_gpu_data_mgmt.cpp:93-159, nbnxm_cuda_kernel.cuh:311-376).
@@ -26,6 +45,9 @@ struct FlavourConsts
     Real disp_c2, disp_c3, rep_c2, rep_c3, sw_c3, sw_c4, sw_c5;
     Real lje_coeff2, lje_coeff6_6, sh_lj_ewald;
     Real tableScale;
+    // Exclusion-free analytical Ewald: exp(-b^2 r^2) = exp2(r^2 negBeta2Log2e),
+    // t = 1/(1 + betaT r), and 2 beta/sqrt(pi).
+    Real negBeta2Log2e, betaT, twoBetaSqrtPi;
 
     __device__ explicit FlavourConsts(const NbFlavourParams& p)
     {
@@ -55,6 +77,9 @@ struct FlavourConsts
         lje_coeff6_6 = lje_coeff2 * lje_coeff2 * lje_coeff2 * (Real(1) / Real(6));
         sh_lj_ewald = static_cast<Real>(p.sh_lj_ewald);
         tableScale = static_cast<Real>(p.ewaldTableScale);
+        negBeta2Log2e = -beta2 * static_cast<Real>(1.44269504088896340736);
+        betaT = static_cast<Real>(c_erfcxPolyScale) * beta;
+        twoBetaSqrtPi = beta * static_cast<Real>(2.0 * c_oneSqrtPi);
     }
 };
 
@@ -181,7 +206,8 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
                                           Real c6grid,
                                           Real& eLJ,
                                           Real& eEl,
-                                          const float* tableF = nullptr)
+                                          const float* tableF = nullptr,
+                                          bool exclFree = false)
 {
     constexpr Real oneSixth = Real(1) / Real(6);
     constexpr Real oneTwelveth = Real(1) / Real(12);
@@ -270,7 +296,17 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
         {
             if constexpr (std::is_same_v<Real, float>)
             {
-                eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+                if (exclFree)
+                {
+                    // erfc(beta r) = exp(-beta^2 r^2) R(t) instead of 1 - erff.
+                    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(k.betaT, r2 * inv_r, 1.0f));
+                    const float e = __builtin_amdgcn_exp2f(r2 * k.negBeta2Log2e);
+                    eEl = __builtin_fmaf(qq, __builtin_fmaf(e * erfcxPolyT(t), inv_r, -k.sh_ewald), eEl);
+                }
+                else
+                {
+                    eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+                }
             }
             else
             {
@@ -280,10 +316,29 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
     }
     else if constexpr (std::is_same_v<Real, float>)
     {
-        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(k.beta2 * r2) * k.beta3);
-        if constexpr (computeEnergy)
+        if (exclFree)
         {
-            eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+            // No excluded lane (int_bit = 1, wave-uniform; the F+E kernels of the
+            // tuned list walk pass it, where it replaces erff): erfc(beta r) =
+            // e R(t) with e = exp(-beta^2 r^2) serves force and energy,
+            // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2, V/qq = e R / r - sh_ewald.
+            const float r = r2 * inv_r;
+            const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(k.betaT, r, 1.0f));
+            const float R = erfcxPolyT(t);
+            const float e = __builtin_amdgcn_exp2f(r2 * k.negBeta2Log2e);
+            F_invr = __builtin_fmaf(qq * e, __builtin_fmaf(R, inv_r, k.twoBetaSqrtPi) * inv_r2, F_invr);
+            if constexpr (computeEnergy)
+            {
+                eEl = __builtin_fmaf(qq, __builtin_fmaf(e * R, inv_r, -k.sh_ewald), eEl);
+            }
+        }
+        else
+        {
+            F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(k.beta2 * r2) * k.beta3);
+            if constexpr (computeEnergy)
+            {
+                eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+            }
         }
     }
     else

@@ -120,25 +120,6 @@ __device__ __forceinline__ float2 loadF2(ConstPtr<float2> p, int i)
     return make_float2(q[0], q[1]);
 }
 
-// erfc(z) with relative accuracy for z in [0, 4.5]: erfc(z) = exp(-z^2) R(t),
-// t = 1/(1 + 0.4 z), R a degree-6 polynomial fitted to erfc(z) exp(z^2) in FP64
-// by relative least squares reweighted towards the minimax fit (max relative
-// error 2.4e-7 in FP64, 5.4e-7 evaluated in FP32 with fused multiply-adds).
-// Used for pairs without exclusions, where 1 - erf(z) would lose the small
-// erfc(z) near the cut-off.  Takes t; the caller forms it with beta folded in.
-constexpr float c_erfcxPolyScale = 0.4f;
-__device__ __forceinline__ float erfcxPolyT(float t)
-{
-    float p = -0.216648832f;
-    p = __builtin_fmaf(p, t, 0.7811663151f);
-    p = __builtin_fmaf(p, t, -0.5001354218f);
-    p = __builtin_fmaf(p, t, 0.5890582204f);
-    p = __builtin_fmaf(p, t, 0.1001023054f);
-    p = __builtin_fmaf(p, t, 0.2481535822f);
-    p = __builtin_fmaf(p, t, -0.001696089399f);
-    return p;
-}
-
 // Occupancy floor: 3 waves per SIMD (at most 168 VGPRs) for the run-time
 // flavour's force-only kernels with a combination rule, which the allocator
 // otherwise leaves one register above it; no floor elsewhere (capping the
@@ -454,7 +435,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     {
                         const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
                         F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG,
-                                                                        ldsEwaldTable ? sEwald : nullptr);
+                                                                        ldsEwaldTable ? sEwald : nullptr, computeEnergy && !(hasExcl && exclHere));
                     }
                     else if (hasExcl && exclHere)
                     {

@@ -11,7 +11,7 @@ import pytest
 
 scipy_special = pytest.importorskip("scipy.special")
 
-SOURCE = pathlib.Path(__file__).resolve().parents[1] / "gromacs-fmm_amd" / "csrc" / "nbnxm_kernel_body.h"
+SOURCE = pathlib.Path(__file__).resolve().parents[1] / "gromacs-fmm_amd" / "csrc" / "flavour_math.h"
 FLOAT = r"(-?\d+(?:\.\d*)?(?:[eE][-+]?\d+)?)f"
 
 
