@@ -18,14 +18,57 @@ constexpr double c_fixedScaleD = 4294967296.0;
 constexpr float c_fixedMaxF = 1073741824.0f;         // 2^30: keeps both 32-bit halves defined
 constexpr double c_oneSqrtPi = 0.56418958354775628695;
 
+// Lane exchanges of gfx950 that stay in the VALU (no LDS): with both operands
+// equal, v_permlane16_swap / v_permlane32_swap leave each lane its own value
+// in one result and the value of lane ^ 16 (lane ^ 32) in the other, so the
+// sum of the two is one butterfly step.
+__device__ __forceinline__ float addXor16(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float addXor32(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// One DPP move of a double (both halves with the same control).
+template<int ctrl>
+__device__ __forceinline__ double dppD(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b & 0xFFFFFFFFull), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32)
+                                                       | static_cast<unsigned int>(lo)));
+}
+
+template<bool k32>
+__device__ __forceinline__ double addXorRowsD(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned int lo = static_cast<unsigned int>(b & 0xFFFFFFFFull);
+    const unsigned int hi = static_cast<unsigned int>(b >> 32);
+    const auto l = k32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false) : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = k32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false) : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double a = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(h[0]) << 32) | l[0]));
+    const double c = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(h[1]) << 32) | l[1]));
+    return a + c;
+}
+
+// Sum over the wave, the same bits in every lane: quad xor 1, xor 2, then
+// the quads of 8 (row_half_mirror) and the halves of 16 (row_mirror), whose
+// lanes already hold equal values, then rows xor 16 and xor 32.
 __device__ __forceinline__ double reduceWaveD(double v)
 {
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1)
-    {
-        v += __shfl_xor(v, m);
-    }
-    return v;
+    v += dppD<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dppD<0x4E>(v);  // quad_perm [2,3,0,1]
+    v += dppD<0x141>(v); // row_half_mirror
+    v += dppD<0x140>(v); // row_mirror
+    v = addXorRowsD<false>(v);
+    return addXorRowsD<true>(v);
 }
 
 // FP32 force -> int64 fixed point (2^-32 units) in two exact 32-bit halves:

@@ -75,13 +75,13 @@ __device__ __forceinline__ unsigned int waveOr(unsigned int v)
                                      | __builtin_amdgcn_readlane(static_cast<int>(v), 32) | __builtin_amdgcn_readlane(static_cast<int>(v), 48));
 }
 
-// Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), 16, 32.
+// Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), then rows
+// xor 16 and xor 32 with the VALU lane swaps.
 __device__ __forceinline__ float reduceOverStride8(float v)
 {
     v += dppF<0x128>(v);
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+    v = addXor16(v);
+    return addXor32(v);
 }
 
 // Read-only list and parameter data through the constant address space: with
