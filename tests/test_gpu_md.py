@@ -152,3 +152,33 @@ def test_md_energy_conservation(torch):
     drift = abs(e[-1, 0] - e[0, 0])
     assert drift < 2e-3 * e[:, 1].mean(), (e[0], e[-1])
     assert np.std(e[:, 0]) < 2e-3 * e[:, 1].mean()
+
+
+@pytest.mark.parametrize("gpu_search", [False, True])
+def test_box_change_triggers_search(torch, gpu_search):
+    """A box change between searches (pressure coupling scales box and
+    coordinates) starts a new list at once; forces and energies then match
+    the list-free FP64 sum in the new box."""
+    announce(f"md box change gpu_search={gpu_search}")
+    system, params = lj_fluid(3000)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
+    ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
+                             rlist_inner=1.05, nstlist_prune=4, gpu_search=gpu_search)
+    x = torch.tensor(system.x, dtype=torch.float32, device="cuda").contiguous()
+    f = torch.zeros_like(x)
+    stream = torch.cuda.current_stream().cuda_stream
+    for step in range(3):
+        ver.step(step, system.box, x.data_ptr(), f.data_ptr(), stream)
+    assert ver.stats()["searches"] == 1
+    scale = 1.01
+    box = system.box * scale
+    x *= scale
+    ver.step(3, box, x.data_ptr(), f.data_ptr(), stream, compute_energy=True)
+    assert ver.stats()["searches"] == 2
+    cur = nb.systems.System(box=box, x=x.cpu().numpy().astype(np.float64), q=system.q, types=system.types,
+                            c6c12=system.c6c12, excl_ranges=system.excl_ranges, excl_elements=system.excl_elements)
+    fb, eljb, eelb, _ = oracle.bruteforce(cur, ic)
+    assert rel_rms(f.cpu().numpy(), fb) < 5e-6
+    elj, eel = ver.energies()
+    assert elj == pytest.approx(eljb, rel=2e-5)
+    assert eel == pytest.approx(eelb, rel=2e-5, abs=1e-3)
