@@ -30,6 +30,40 @@ int hipFail(hipError_t e, const char* what);
 // their own work against it).
 hipStream_t gpuTaskStream(const nbnxm_gpu_t* nb);
 
+// A single-system grid and list that live on the device (the output of a
+// device-input search build), for a GPU task to adopt without a host trip.
+// Cluster, group and mask indices are the single-system ones.
+struct DeviceListView
+{
+    int numAtoms = 0;
+    int numSlots = 0;                            // grid slots, 64 per cell
+    const int* atomIndex = nullptr;              // slot -> atom, -1 for fillers
+    const float4* xw = nullptr;                  // wrapped x, y, z per atom
+    const nbnxm_sci_t* sci = nullptr;
+    const nbnxm_sci_t* hostSci = nullptr;        // host copy of sci
+    int nsci = 0;
+    const nbnxm_cj_packed_t* cjPacked = nullptr;
+    int ncjPacked = 0;
+    const nbnxm_excl_t* excl = nullptr;          // excl[0] is all ones
+    int nexcl = 0;
+    hipStream_t stream = nullptr;                // the stream that wrote them
+};
+
+// A GPU task adopts a device list and builds its grid-order atom data on the
+// device: xq from the wrapped coordinates and per-atom charges, types (with
+// the dummy type for fillers) and per-type combination parameters, the
+// slot/atom maps of the buffer ops, the shift vectors of `box`.  Per-atom
+// inputs are device arrays; typeComb may be null for the type-table kernel.
+// Replaces gpu_init_atomdata + gpu_upload_shiftvec + gpu_init_pairlist +
+// gpu_init_x_to_nbat_x of a task created by gpu_init.
+int gpuTaskAdoptDeviceList(nbnxm_gpu_t* nb,
+                           const DeviceListView& v,
+                           const float* charges,
+                           const int* types,
+                           const float2* typeComb,
+                           int dummyType,
+                           const double box[3]);
+
 #define NBNXM_HIP_CHECK(call)                          \
     do                                                 \
     {                                                  \

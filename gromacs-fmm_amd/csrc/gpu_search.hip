@@ -824,6 +824,16 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
     }
 }
 
+// FP32 device coordinates -> the FP64 input of the search (exact).
+__global__ void widenCoordinatesKernel(const float* x, int n3, double* xd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3)
+    {
+        xd[i] = static_cast<double>(x[i]);
+    }
+}
+
 __global__ void fillAllOnesKernel(nbnxm_excl_t* excl)
 {
     excl[0].pair[threadIdx.x] = 0xFFFFFFFFu;
@@ -862,6 +872,13 @@ struct nbnxm_gpu_search
     DeviceBuffer<nbnxm_cj_packed_t> cjPacked;
     DeviceBuffer<nbnxm_excl_t> excl;
     DeviceBuffer<int> error;
+
+    // Exclusions on the device belong to the single system of this many atoms
+    // (device-input builds upload them once; -1: none uploaded that way).
+    int exclUploadedFor = -1;
+    // The host copies below hold the current build (device-input builds copy
+    // only the entries until gpuSearchDownloadViews).
+    bool viewsCurrent = false;
 
     // Host copies for the per-system views.
     std::vector<nbnxm_sci_t> hSci;
@@ -918,127 +935,42 @@ hipError_t exclusiveScan(nbnxm_gpu_search* s, const Tin* in, Tout* out, int n)
     return hipcub::DeviceScan::ExclusiveSum(s->temp.ptr, bytes, in, out, n, s->stream);
 }
 
-} // namespace
-
-extern "C" int nbnxm_gpu_search_create(nbnxm_gpu_search_t** out)
+// Host copies of the lists, slot map and wrapped coordinates of the last
+// build (what nbnxm_gpu_search_system_list and the export read).
+int downloadViewsOf(nbnxm_gpu_search* s)
 {
-    if (!out)
+    hipStream_t st = s->stream;
+    s->hSci.resize(s->nsci);
+    s->hCj.resize(s->ngroups);
+    s->hExcl.resize(s->nexcl);
+    s->hAtomIndex.resize(static_cast<size_t>(s->numCells) * c_cellAtoms);
+    if (s->nsci)
     {
-        return fail(NBNXM_ERR_INVALID, "gpu_search_create: null pointer");
+        NBNXM_HIP_CHECK(hipMemcpyAsync(s->hSci.data(), s->sci.ptr, sizeof(nbnxm_sci_t) * s->nsci, hipMemcpyDeviceToHost, st));
     }
-    auto s = std::make_unique<nbnxm_gpu_search>();
-    NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    *out = s.release();
+    if (s->ngroups)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(s->hCj.data(), s->cjPacked.ptr, sizeof(nbnxm_cj_packed_t) * s->ngroups, hipMemcpyDeviceToHost, st));
+    }
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hExcl.data(), s->excl.ptr, sizeof(nbnxm_excl_t) * s->nexcl, hipMemcpyDeviceToHost, st));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hAtomIndex.data(), s->atomIndex.ptr, sizeof(int) * s->hAtomIndex.size(), hipMemcpyDeviceToHost, st));
+    s->hXw.resize(s->numAtoms);
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hXw.data(), s->xw.ptr, sizeof(float4) * s->numAtoms, hipMemcpyDeviceToHost, st));
+    NBNXM_HIP_CHECK(hipStreamSynchronize(st));
+    s->viewsCurrent = true;
     return NBNXM_OK;
 }
 
-extern "C" void nbnxm_gpu_search_free(nbnxm_gpu_search_t* s)
+// The device part of a build, once the systems' host grids (hostGrids,
+// sysAtomStart, sysColStart), the FP64 coordinates (x) and, with anyExcl,
+// the exclusions are in place: wrap and sort, cells and clusters, the list
+// (count, scan on the host, write); downloadViews also copies everything the
+// per-system views read to the host.
+int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, int maxPairsPerEntry, bool downloadViews)
 {
-    delete s;
-}
-
-extern "C" int nbnxm_gpu_search_build(nbnxm_gpu_search_t* s, int numSystems, const nbnxm_system_desc_t* systems, double rlist, int maxPairsPerEntry)
-{
-    if (!s || numSystems < 1 || !systems || !(rlist > 0))
-    {
-        return fail(NBNXM_ERR_INVALID, "gpu_search_build: null pointer, no systems or rlist <= 0");
-    }
-    s->numSystems = numSystems;
-    s->hostGrids.resize(numSystems);
-    s->sysAtomStart.assign(numSystems + 1, 0);
-    s->sysColStart.assign(numSystems + 1, 0);
-    bool anyExcl = false;
-    for (int k = 0; k < numSystems; k++)
-    {
-        const nbnxm_system_desc_t& d = systems[k];
-        if (d.numAtoms < 0 || (d.numAtoms > 0 && !d.x))
-        {
-            return fail(NBNXM_ERR_INVALID, "gpu_search_build: bad system description");
-        }
-        for (int dd = 0; dd < 3; dd++)
-        {
-            if (!(d.box[dd] > rlist))
-            {
-                return fail(NBNXM_ERR_INVALID, "gpu_search_build: every box edge must exceed rlist");
-            }
-        }
-        SysGrid g = hostGrid(d.box, d.numAtoms);
-        g.atomOffset = s->sysAtomStart[k];
-        g.numAtoms = d.numAtoms;
-        g.colOffset = s->sysColStart[k];
-        s->hostGrids[k] = g;
-        s->sysAtomStart[k + 1] = s->sysAtomStart[k] + d.numAtoms;
-        s->sysColStart[k + 1] = s->sysColStart[k] + g.ncx * g.ncy;
-        anyExcl = anyExcl || (d.exclRanges != nullptr);
-    }
-    const int N = s->sysAtomStart[numSystems];
-    const int ncol = s->sysColStart[numSystems];
-    s->numAtoms = N;
-    s->numCols = ncol;
-    if (N == 0)
-    {
-        return fail(NBNXM_ERR_INVALID, "gpu_search_build: no atoms");
-    }
-
-    // Concatenated coordinates and exclusions (global atom indices).
-    std::vector<double> hx(3 * static_cast<size_t>(N));
-    std::vector<int> hr, he;
-    if (anyExcl)
-    {
-        hr.assign(N + 1, 0);
-    }
-    for (int k = 0; k < numSystems; k++)
-    {
-        const nbnxm_system_desc_t& d = systems[k];
-        std::copy(d.x, d.x + 3 * static_cast<size_t>(d.numAtoms), hx.begin() + 3 * static_cast<size_t>(s->sysAtomStart[k]));
-        if (anyExcl)
-        {
-            for (int i = 0; i < d.numAtoms; i++)
-            {
-                const int a = s->sysAtomStart[k] + i;
-                hr[a] = static_cast<int>(he.size());
-                if (d.exclRanges)
-                {
-                    if (!d.exclElements && d.exclRanges[i + 1] > d.exclRanges[i])
-                    {
-                        return fail(NBNXM_ERR_INVALID, "gpu_search_build: exclusion ranges without elements");
-                    }
-                    for (int e = d.exclRanges[i]; e < d.exclRanges[i + 1]; e++)
-                    {
-                        const int j = d.exclElements[e];
-                        // Out-of-range elements are skipped by the search, as on the host.
-                        he.push_back((j >= 0 && j < d.numAtoms) ? s->sysAtomStart[k] + j : -1);
-                    }
-                }
-            }
-        }
-    }
-    if (anyExcl)
-    {
-        hr[N] = static_cast<int>(he.size());
-        if (he.empty())
-        {
-            he.push_back(-1);
-        }
-    }
-
     hipStream_t st = s->stream;
-    NBNXM_HIP_CHECK(s->x.resize(hx.size()));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->x.ptr, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, st));
-    NBNXM_HIP_CHECK(s->grids.resize(numSystems));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->grids.ptr, s->hostGrids.data(), sizeof(SysGrid) * numSystems, hipMemcpyHostToDevice, st));
-    NBNXM_HIP_CHECK(s->atomStart.resize(numSystems + 1));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->atomStart.ptr, s->sysAtomStart.data(), sizeof(int) * (numSystems + 1), hipMemcpyHostToDevice, st));
-    NBNXM_HIP_CHECK(s->colSysStart.resize(numSystems + 1));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->colSysStart.ptr, s->sysColStart.data(), sizeof(int) * (numSystems + 1), hipMemcpyHostToDevice, st));
-    if (anyExcl)
-    {
-        NBNXM_HIP_CHECK(s->exclRanges.resize(hr.size()));
-        NBNXM_HIP_CHECK(s->exclElements.resize(he.size()));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclRanges.ptr, hr.data(), sizeof(int) * hr.size(), hipMemcpyHostToDevice, st));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclElements.ptr, he.data(), sizeof(int) * he.size(), hipMemcpyHostToDevice, st));
-    }
-
+    const int N = s->numAtoms;
+    const int ncol = s->numCols;
     // 1. wrap + keys, stable sort by (column, z).
     NBNXM_HIP_CHECK(s->xw.resize(N));
     NBNXM_HIP_CHECK(s->keys.resize(N));
@@ -1213,26 +1145,147 @@ extern "C" int nbnxm_gpu_search_build(nbnxm_gpu_search_t* s, int numSystems, con
     hipLaunchKernelGGL(searchKernel<true>, dim3(numTasks), dim3(64), 0, st, P);
     NBNXM_HIP_CHECK(hipGetLastError());
 
-    // Host copies for the per-system views.
+    // Host copies: the entries always (callers derive per-shift lists from
+    // them), the rest for the per-system views unless deferred.
     s->hSci.resize(s->nsci);
-    s->hCj.resize(s->ngroups);
-    s->hExcl.resize(s->nexcl);
-    s->hAtomIndex.resize(static_cast<size_t>(numCells) * c_cellAtoms);
     if (s->nsci)
     {
         NBNXM_HIP_CHECK(hipMemcpyAsync(s->hSci.data(), s->sci.ptr, sizeof(nbnxm_sci_t) * s->nsci, hipMemcpyDeviceToHost, st));
     }
-    if (s->ngroups)
-    {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(s->hCj.data(), s->cjPacked.ptr, sizeof(nbnxm_cj_packed_t) * s->ngroups, hipMemcpyDeviceToHost, st));
-    }
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hExcl.data(), s->excl.ptr, sizeof(nbnxm_excl_t) * s->nexcl, hipMemcpyDeviceToHost, st));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hAtomIndex.data(), s->atomIndex.ptr, sizeof(int) * s->hAtomIndex.size(), hipMemcpyDeviceToHost, st));
-    s->hXw.resize(N);
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->hXw.data(), s->xw.ptr, sizeof(float4) * N, hipMemcpyDeviceToHost, st));
-    NBNXM_HIP_CHECK(hipStreamSynchronize(st));
     s->rlistF = static_cast<float>(rlist);
+    s->viewsCurrent = false;
+    if (downloadViews)
+    {
+        return downloadViewsOf(s);
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(st));
     return NBNXM_OK;
+}
+
+} // namespace
+
+extern "C" int nbnxm_gpu_search_create(nbnxm_gpu_search_t** out)
+{
+    if (!out)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_search_create: null pointer");
+    }
+    auto s = std::make_unique<nbnxm_gpu_search>();
+    NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    *out = s.release();
+    return NBNXM_OK;
+}
+
+extern "C" void nbnxm_gpu_search_free(nbnxm_gpu_search_t* s)
+{
+    delete s;
+}
+
+extern "C" int nbnxm_gpu_search_build(nbnxm_gpu_search_t* s, int numSystems, const nbnxm_system_desc_t* systems, double rlist, int maxPairsPerEntry)
+{
+    if (!s || numSystems < 1 || !systems || !(rlist > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_search_build: null pointer, no systems or rlist <= 0");
+    }
+    s->numSystems = numSystems;
+    s->hostGrids.resize(numSystems);
+    s->sysAtomStart.assign(numSystems + 1, 0);
+    s->sysColStart.assign(numSystems + 1, 0);
+    bool anyExcl = false;
+    for (int k = 0; k < numSystems; k++)
+    {
+        const nbnxm_system_desc_t& d = systems[k];
+        if (d.numAtoms < 0 || (d.numAtoms > 0 && !d.x))
+        {
+            return fail(NBNXM_ERR_INVALID, "gpu_search_build: bad system description");
+        }
+        for (int dd = 0; dd < 3; dd++)
+        {
+            if (!(d.box[dd] > rlist))
+            {
+                return fail(NBNXM_ERR_INVALID, "gpu_search_build: every box edge must exceed rlist");
+            }
+        }
+        SysGrid g = hostGrid(d.box, d.numAtoms);
+        g.atomOffset = s->sysAtomStart[k];
+        g.numAtoms = d.numAtoms;
+        g.colOffset = s->sysColStart[k];
+        s->hostGrids[k] = g;
+        s->sysAtomStart[k + 1] = s->sysAtomStart[k] + d.numAtoms;
+        s->sysColStart[k + 1] = s->sysColStart[k] + g.ncx * g.ncy;
+        anyExcl = anyExcl || (d.exclRanges != nullptr);
+    }
+    const int N = s->sysAtomStart[numSystems];
+    const int ncol = s->sysColStart[numSystems];
+    s->numAtoms = N;
+    s->numCols = ncol;
+    if (N == 0)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_search_build: no atoms");
+    }
+
+    // Concatenated coordinates and exclusions (global atom indices).
+    std::vector<double> hx(3 * static_cast<size_t>(N));
+    std::vector<int> hr, he;
+    if (anyExcl)
+    {
+        hr.assign(N + 1, 0);
+    }
+    for (int k = 0; k < numSystems; k++)
+    {
+        const nbnxm_system_desc_t& d = systems[k];
+        std::copy(d.x, d.x + 3 * static_cast<size_t>(d.numAtoms), hx.begin() + 3 * static_cast<size_t>(s->sysAtomStart[k]));
+        if (anyExcl)
+        {
+            for (int i = 0; i < d.numAtoms; i++)
+            {
+                const int a = s->sysAtomStart[k] + i;
+                hr[a] = static_cast<int>(he.size());
+                if (d.exclRanges)
+                {
+                    if (!d.exclElements && d.exclRanges[i + 1] > d.exclRanges[i])
+                    {
+                        return fail(NBNXM_ERR_INVALID, "gpu_search_build: exclusion ranges without elements");
+                    }
+                    for (int e = d.exclRanges[i]; e < d.exclRanges[i + 1]; e++)
+                    {
+                        const int j = d.exclElements[e];
+                        // Out-of-range elements are skipped by the search, as on the host.
+                        he.push_back((j >= 0 && j < d.numAtoms) ? s->sysAtomStart[k] + j : -1);
+                    }
+                }
+            }
+        }
+    }
+    if (anyExcl)
+    {
+        hr[N] = static_cast<int>(he.size());
+        if (he.empty())
+        {
+            he.push_back(-1);
+        }
+    }
+
+    hipStream_t st = s->stream;
+    NBNXM_HIP_CHECK(s->x.resize(hx.size()));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->x.ptr, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, st));
+    NBNXM_HIP_CHECK(s->grids.resize(numSystems));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->grids.ptr, s->hostGrids.data(), sizeof(SysGrid) * numSystems, hipMemcpyHostToDevice, st));
+    NBNXM_HIP_CHECK(s->atomStart.resize(numSystems + 1));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->atomStart.ptr, s->sysAtomStart.data(), sizeof(int) * (numSystems + 1), hipMemcpyHostToDevice, st));
+    NBNXM_HIP_CHECK(s->colSysStart.resize(numSystems + 1));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->colSysStart.ptr, s->sysColStart.data(), sizeof(int) * (numSystems + 1), hipMemcpyHostToDevice, st));
+    if (anyExcl)
+    {
+        NBNXM_HIP_CHECK(s->exclRanges.resize(hr.size()));
+        NBNXM_HIP_CHECK(s->exclElements.resize(he.size()));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclRanges.ptr, hr.data(), sizeof(int) * hr.size(), hipMemcpyHostToDevice, st));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclElements.ptr, he.data(), sizeof(int) * he.size(), hipMemcpyHostToDevice, st));
+    }
+
+    const int rcRun = runSearch(s, numSystems, anyExcl, rlist, maxPairsPerEntry, true);
+    s->exclUploadedFor = -1;
+    return rcRun;
 }
 
 extern "C" int nbnxm_gpu_search_num_entries(const nbnxm_gpu_search_t* s, int64_t* counts)
@@ -1253,6 +1306,10 @@ extern "C" int nbnxm_gpu_search_system_list(const nbnxm_gpu_search_t* s, int sys
     if (!s || !sizes || system < 0 || system >= s->numSystems)
     {
         return fail(NBNXM_ERR_INVALID, "gpu_search_system_list: null pointer or bad system");
+    }
+    if (!s->viewsCurrent)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_search_system_list: the last build kept its lists on the device");
     }
     const int cell0 = s->sysCellStart[system], cell1 = s->sysCellStart[system + 1];
     const int sci0 = s->sysSciStart[system], sci1 = s->sysSciStart[system + 1];
@@ -1313,6 +1370,106 @@ extern "C" int nbnxm_gpu_search_system_list(const nbnxm_gpu_search_t* s, int sys
 
 namespace nbnxm_amd
 {
+
+hipStream_t gpuSearchStream(const nbnxm_gpu_search_t* s)
+{
+    return s ? s->stream : nullptr;
+}
+
+int gpuSearchDownloadViews(nbnxm_gpu_search_t* s)
+{
+    if (!s || s->numSystems < 1)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_search: no build to copy");
+    }
+    return s->viewsCurrent ? NBNXM_OK : downloadViewsOf(s);
+}
+
+int gpuSearchBuildDevice(nbnxm_gpu_search_t* s,
+                         int numAtoms,
+                         const float* x,
+                         hipEvent_t after,
+                         const double box[3],
+                         const int* exclRanges,
+                         const int* exclElements,
+                         bool uploadExclusions,
+                         double rlist,
+                         int maxPairsPerEntry,
+                         DeviceListView* view)
+{
+    if (!s || numAtoms < 1 || !x || !box || !view || !(rlist > 0))
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_search_build_device: null pointer or bad size");
+    }
+    for (int d = 0; d < 3; d++)
+    {
+        if (!(box[d] > rlist))
+        {
+            return fail(NBNXM_ERR_INVALID, "gpu_search_build_device: every box edge must exceed rlist");
+        }
+    }
+    const bool anyExcl = exclRanges != nullptr;
+    if (anyExcl && !uploadExclusions && s->exclUploadedFor != numAtoms)
+    {
+        return fail(NBNXM_ERR_STATE, "gpu_search_build_device: no exclusions uploaded for this system");
+    }
+    hipStream_t st = s->stream;
+    if (after)
+    {
+        NBNXM_HIP_CHECK(hipStreamWaitEvent(st, after, 0));
+    }
+    s->numSystems = 1;
+    s->hostGrids.resize(1);
+    SysGrid g = hostGrid(box, numAtoms);
+    g.atomOffset = 0;
+    g.numAtoms = numAtoms;
+    g.colOffset = 0;
+    s->hostGrids[0] = g;
+    s->sysAtomStart.assign({ 0, numAtoms });
+    s->sysColStart.assign({ 0, g.ncx * g.ncy });
+    s->numAtoms = numAtoms;
+    s->numCols = g.ncx * g.ncy;
+    NBNXM_HIP_CHECK(s->grids.resize(1));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->grids.ptr, s->hostGrids.data(), sizeof(SysGrid), hipMemcpyHostToDevice, st));
+    NBNXM_HIP_CHECK(s->atomStart.resize(2));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->atomStart.ptr, s->sysAtomStart.data(), 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    NBNXM_HIP_CHECK(s->colSysStart.resize(2));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(s->colSysStart.ptr, s->sysColStart.data(), 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    const int n3 = 3 * numAtoms;
+    NBNXM_HIP_CHECK(s->x.resize(n3));
+    hipLaunchKernelGGL(widenCoordinatesKernel, dim3((n3 + 255) / 256), dim3(256), 0, st, x, n3, s->x.ptr);
+    NBNXM_HIP_CHECK(hipGetLastError());
+    if (anyExcl && uploadExclusions)
+    {
+        const int numElements = exclRanges[numAtoms];
+        NBNXM_HIP_CHECK(s->exclRanges.resize(numAtoms + 1));
+        NBNXM_HIP_CHECK(s->exclElements.resize(std::max(numElements, 1)));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclRanges.ptr, exclRanges, sizeof(int) * (numAtoms + 1), hipMemcpyHostToDevice, st));
+        if (numElements > 0)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclElements.ptr, exclElements, sizeof(int) * numElements, hipMemcpyHostToDevice, st));
+        }
+        s->exclUploadedFor = numAtoms;
+    }
+    const int rc = runSearch(s, 1, anyExcl, rlist, maxPairsPerEntry, false);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    view->numAtoms = numAtoms;
+    view->numSlots = s->numCells * c_cellAtoms;
+    view->atomIndex = s->atomIndex.ptr;
+    view->xw = s->xw.ptr;
+    view->sci = s->sci.ptr;
+    view->hostSci = s->hSci.data();
+    view->nsci = s->nsci;
+    view->cjPacked = s->cjPacked.ptr;
+    view->ncjPacked = s->ngroups;
+    view->excl = s->excl.ptr;
+    view->nexcl = s->nexcl;
+    view->stream = st;
+    return NBNXM_OK;
+}
 
 int gpuSearchExportSystem(const nbnxm_gpu_search_t* s, int system, GridSearch& grid)
 {

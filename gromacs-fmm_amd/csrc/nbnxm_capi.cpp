@@ -237,6 +237,7 @@ struct nbnxm_gpu
     double* hVirial = nullptr;   // pinned, 9
     bool lastComputeEnergy = false;
     bool lastComputeVirial = false;
+    hipEvent_t adoptEvent = nullptr; // orders a device list's producer before this task
 
     ~nbnxm_gpu()
     {
@@ -256,6 +257,10 @@ struct nbnxm_gpu
         {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
+        }
+        if (adoptEvent)
+        {
+            (void)hipEventDestroy(adoptEvent);
         }
     }
 };
@@ -282,6 +287,99 @@ int uploadParams(nbnxm_gpu* nb)
     NBNXM_HIP_CHECK(nb->paramsD.resize(8));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsD.ptr, pd, sizeof(pd), hipMemcpyHostToDevice, nb->stream));
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    return NBNXM_OK;
+}
+
+} // namespace
+
+namespace
+{
+
+// The list of a task from host or device arrays (kind), with the per-shift
+// entry lists derived from the host copy of the entries.
+int setPairlist(nbnxm_gpu* nb,
+                const nbnxm_sci_t* hostSci,
+                const nbnxm_sci_t* sciSrc,
+                int nsci,
+                const nbnxm_cj_packed_t* cjSrc,
+                int ncj,
+                const nbnxm_excl_t* exclSrc,
+                int nexcl,
+                hipMemcpyKind kind)
+{
+    // Until the upload below completes there is no usable list.
+    nb->haveList = false;
+    nb->nsci = nsci;
+    nb->ncjPacked = ncj;
+    NBNXM_HIP_CHECK(nb->sci.resize(nsci));
+    NBNXM_HIP_CHECK(nb->cjPacked.resize(ncj));
+    NBNXM_HIP_CHECK(nb->excl.resize(nexcl));
+    NBNXM_HIP_CHECK(nb->imaskOuter.resize(ncj));
+    NBNXM_HIP_CHECK(nb->sciOrder.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciCount.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciCountSorted.resize(nsci));
+    NBNXM_HIP_CHECK(nb->iota.resize(nsci));
+    NBNXM_HIP_CHECK(nb->sciHistogram.resize(NBNXM_SCI_HISTOGRAM_SIZE));
+    NBNXM_HIP_CHECK(nb->partials.resize(8 * static_cast<size_t>(std::max(nsci, 1))));
+    const int wavesPerBlockPrune = 4;
+    const int rollingSize = ((nsci + wavesPerBlockPrune - 1) / wavesPerBlockPrune + 1) * wavesPerBlockPrune;
+    NBNXM_HIP_CHECK(nb->rollingPart.resize(rollingSize));
+
+    if (nsci > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sci.ptr, sciSrc, sizeof(nbnxm_sci_t) * nsci, kind, nb->stream));
+    }
+    if (ncj > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->cjPacked.ptr, cjSrc, sizeof(nbnxm_cj_packed_t) * ncj, kind, nb->stream));
+    }
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->excl.ptr, exclSrc, sizeof(nbnxm_excl_t) * nexcl, kind, nb->stream));
+
+    // Identity work order and the per-shift entry lists for the FP64 reductions.
+    std::vector<int> iota(nsci);
+    std::iota(iota.begin(), iota.end(), 0);
+    std::vector<int> shiftRanges(NBNXM_NUM_SHIFTS + 1, 0);
+    for (int i = 0; i < nsci; i++)
+    {
+        shiftRanges[hostSci[i].shift + 1]++;
+    }
+    for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
+    {
+        shiftRanges[k + 1] += shiftRanges[k];
+    }
+    std::vector<int> shiftList(nsci);
+    {
+        std::vector<int> fillPos(shiftRanges.begin(), shiftRanges.end() - 1);
+        for (int i = 0; i < nsci; i++)
+        {
+            shiftList[fillPos[hostSci[i].shift]++] = i;
+        }
+    }
+    const int entryRanges[2] = { 0, nsci };
+    NBNXM_HIP_CHECK(nb->entryRanges.resize(2));
+    NBNXM_HIP_CHECK(nb->entryList.resize(std::max(nsci, 1)));
+    NBNXM_HIP_CHECK(nb->shiftRanges.resize(NBNXM_NUM_SHIFTS + 1));
+    NBNXM_HIP_CHECK(nb->shiftEntryList.resize(std::max(nsci, 1)));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryRanges.ptr, entryRanges, sizeof(entryRanges), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftRanges.ptr, shiftRanges.data(), sizeof(int) * shiftRanges.size(), hipMemcpyHostToDevice, nb->stream));
+    if (nsci > 0)
+    {
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryList.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sciOrder.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->iota.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+    }
+    NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, rollingSize, 0, nb->stream));
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->partials.ptr), 2 * 8 * static_cast<size_t>(std::max(nsci, 1)), 0, nb->stream));
+
+    // Radix-sort scratch for the sci ordering.
+    size_t tempBytes = 0;
+    NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, std::max(nsci, 1), 0, 14, nb->stream));
+    NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(tempBytes, 1)));
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    nb->haveFreshList = true;
+    nb->haveSortedOrder = false;
+    nb->haveList = true;
     return NBNXM_OK;
 }
 
@@ -917,82 +1015,7 @@ int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list)
     {
         return fail(NBNXM_ERR_INVALID, "gpu_init_pairlist: excl[0] must be the all-interaction mask");
     }
-    const int nsci = list->nsci;
-    const int ncj = list->ncjPacked;
-    // Until the upload below completes there is no usable list.
-    nb->haveList = false;
-    nb->nsci = nsci;
-    nb->ncjPacked = ncj;
-    NBNXM_HIP_CHECK(nb->sci.resize(nsci));
-    NBNXM_HIP_CHECK(nb->cjPacked.resize(ncj));
-    NBNXM_HIP_CHECK(nb->excl.resize(list->nexcl));
-    NBNXM_HIP_CHECK(nb->imaskOuter.resize(ncj));
-    NBNXM_HIP_CHECK(nb->sciOrder.resize(nsci));
-    NBNXM_HIP_CHECK(nb->sciCount.resize(nsci));
-    NBNXM_HIP_CHECK(nb->sciCountSorted.resize(nsci));
-    NBNXM_HIP_CHECK(nb->iota.resize(nsci));
-    NBNXM_HIP_CHECK(nb->sciHistogram.resize(NBNXM_SCI_HISTOGRAM_SIZE));
-    NBNXM_HIP_CHECK(nb->partials.resize(8 * static_cast<size_t>(std::max(nsci, 1))));
-    const int wavesPerBlockPrune = 4;
-    const int rollingSize = ((nsci + wavesPerBlockPrune - 1) / wavesPerBlockPrune + 1) * wavesPerBlockPrune;
-    NBNXM_HIP_CHECK(nb->rollingPart.resize(rollingSize));
-
-    if (nsci > 0)
-    {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sci.ptr, list->sci, sizeof(nbnxm_sci_t) * nsci, hipMemcpyHostToDevice, nb->stream));
-    }
-    if (ncj > 0)
-    {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->cjPacked.ptr, list->cjPacked, sizeof(nbnxm_cj_packed_t) * ncj, hipMemcpyHostToDevice, nb->stream));
-    }
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->excl.ptr, list->excl, sizeof(nbnxm_excl_t) * list->nexcl, hipMemcpyHostToDevice, nb->stream));
-
-    // Identity work order and the per-shift entry lists for the FP64 reductions.
-    std::vector<int> iota(nsci);
-    std::iota(iota.begin(), iota.end(), 0);
-    std::vector<int> shiftRanges(NBNXM_NUM_SHIFTS + 1, 0);
-    for (int i = 0; i < nsci; i++)
-    {
-        shiftRanges[list->sci[i].shift + 1]++;
-    }
-    for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
-    {
-        shiftRanges[k + 1] += shiftRanges[k];
-    }
-    std::vector<int> shiftList(nsci);
-    {
-        std::vector<int> fillPos(shiftRanges.begin(), shiftRanges.end() - 1);
-        for (int i = 0; i < nsci; i++)
-        {
-            shiftList[fillPos[list->sci[i].shift]++] = i;
-        }
-    }
-    const int entryRanges[2] = { 0, nsci };
-    NBNXM_HIP_CHECK(nb->entryRanges.resize(2));
-    NBNXM_HIP_CHECK(nb->entryList.resize(std::max(nsci, 1)));
-    NBNXM_HIP_CHECK(nb->shiftRanges.resize(NBNXM_NUM_SHIFTS + 1));
-    NBNXM_HIP_CHECK(nb->shiftEntryList.resize(std::max(nsci, 1)));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryRanges.ptr, entryRanges, sizeof(entryRanges), hipMemcpyHostToDevice, nb->stream));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftRanges.ptr, shiftRanges.data(), sizeof(int) * shiftRanges.size(), hipMemcpyHostToDevice, nb->stream));
-    if (nsci > 0)
-    {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryList.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sciOrder.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->iota.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-    }
-    NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, rollingSize, 0, nb->stream));
-    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->partials.ptr), 2 * 8 * static_cast<size_t>(std::max(nsci, 1)), 0, nb->stream));
-
-    // Radix-sort scratch for the sci ordering.
-    size_t tempBytes = 0;
-    NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, std::max(nsci, 1), 0, 14, nb->stream));
-    NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(tempBytes, 1)));
-    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
-    nb->haveFreshList = true;
-    nb->haveSortedOrder = false;
-    nb->haveList = true;
-    return NBNXM_OK;
+    return setPairlist(nb, list->sci, list->sci, list->nsci, list->cjPacked, list->ncjPacked, list->excl, list->nexcl, hipMemcpyHostToDevice);
 }
 
 int nbnxm_gpu_copy_xq_to_gpu(nbnxm_gpu_t* nb, const float* xq, int numAtoms)
@@ -1421,3 +1444,74 @@ void nbnxm_gpu_free(nbnxm_gpu_t* nb)
 }
 
 } // extern "C"
+
+int nbnxm_amd::gpuTaskAdoptDeviceList(nbnxm_gpu_t* nb,
+                                      const DeviceListView& v,
+                                      const float* charges,
+                                      const int* types,
+                                      const float2* typeComb,
+                                      int dummyType,
+                                      const double box[3])
+{
+    const bool needTypes = nb && (nb->vdwType == NBNXM_VDW_CUT_TYPE_TABLE || nb->ic.ljPme);
+    const bool needComb = nb && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE;
+    if (!nb || !box || v.numAtoms < 1 || v.numSlots < v.numAtoms || !v.atomIndex || !v.xw || !charges || !types
+        || (needComb && !typeComb) || v.nsci < 0 || v.ncjPacked < 0 || v.nexcl < 1 || (v.nsci > 0 && (!v.sci || !v.hostSci))
+        || (v.ncjPacked > 0 && !v.cjPacked) || !v.excl)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu adopt device list: null pointer or bad size");
+    }
+    // The list and the wrapped coordinates come from another stream.
+    if (!nb->adoptEvent)
+    {
+        NBNXM_HIP_CHECK(hipEventCreateWithFlags(&nb->adoptEvent, hipEventDisableTiming));
+    }
+    NBNXM_HIP_CHECK(hipEventRecord(nb->adoptEvent, v.stream));
+    NBNXM_HIP_CHECK(hipStreamWaitEvent(nb->stream, nb->adoptEvent, 0));
+
+    // Grid-order atom data (gpu_init_atomdata with fillXq/fillTypes/fillLjComb
+    // done by a gather kernel) and the buffer-op maps (gpu_init_x_to_nbat_x).
+    const int n = v.numSlots;
+    nb->numAtoms = n;
+    NBNXM_HIP_CHECK(nb->xq.resize(n));
+    NBNXM_HIP_CHECK(nb->fFixed.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fFloat.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fOut.resize(3 * static_cast<size_t>(n)));
+    NBNXM_HIP_CHECK(nb->fOutDouble.resize(3 * static_cast<size_t>(n)));
+    if (needTypes)
+    {
+        NBNXM_HIP_CHECK(nb->types.resize(n));
+    }
+    if (needComb)
+    {
+        NBNXM_HIP_CHECK(nb->ljComb.resize(n));
+    }
+    NBNXM_HIP_CHECK(nb->atomIndex.resize(n));
+    NBNXM_HIP_CHECK(nb->cellIndex.resize(v.numAtoms));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->atomIndex.ptr, v.atomIndex, sizeof(int) * n, hipMemcpyDeviceToDevice, nb->stream));
+    NBNXM_HIP_CHECK(launchGatherAtomData(v.atomIndex, n, v.xw, charges, types, typeComb, dummyType, nb->xq.ptr, needTypes ? nb->types.ptr : nullptr,
+                                         needComb ? nb->ljComb.ptr : nullptr, nb->cellIndex.ptr, nb->stream));
+    nb->numRealAtoms = v.numAtoms;
+    NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
+    nb->virialBlocks = std::min(512, std::max(1, (n + 1023) / 1024));
+    const int range[2] = { 0, n };
+    NBNXM_HIP_CHECK(nb->slotRange.resize(2));
+    NBNXM_HIP_CHECK(nb->virialPartials.resize(9 * static_cast<size_t>(nb->virialBlocks)));
+    NBNXM_HIP_CHECK(nb->virial.resize(9));
+    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->slotRange.ptr, range, sizeof(range), hipMemcpyHostToDevice, nb->stream));
+
+    // Shift vectors of the box, then the list (its host steps synchronise
+    // the task stream, which also retires the copies above).
+    float shiftVec[3 * NBNXM_NUM_SHIFTS];
+    int rc = nbnxm_calc_shifts(box, shiftVec);
+    if (rc == NBNXM_OK)
+    {
+        rc = nbnxm_gpu_upload_shiftvec(nb, shiftVec);
+    }
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    return setPairlist(nb, v.hostSci, v.sci, v.nsci, v.cjPacked, v.ncjPacked, v.excl, v.nexcl, hipMemcpyDeviceToDevice);
+}

@@ -209,6 +209,22 @@ hipError_t launchVirialFinish(const double* partials, int numSystems, int blocks
 // Buffer ops around the kernel.  xq.xyz of real slots from coordinates in atom
 // order (3 floats per atom; atomIndices: slot -> atom or -1), w kept.
 hipError_t launchXToXq(const float* x, const int* atomIndices, int numSlots, float4* xq, hipStream_t stream);
+// Grid-order xq (fillers far away, q = 0), types (dummy for fillers) and
+// combination parameters of a device-built list, and cells[atom] = slot.
+hipError_t launchGatherAtomData(const int* atomIndices,
+                                int numSlots,
+                                const float4* xw,
+                                const float* q,
+                                const int* types,
+                                const float2* typeComb,
+                                int dummyType,
+                                float4* xq,
+                                int* slotTypes,
+                                float2* slotComb,
+                                int* cells,
+                                hipStream_t stream);
+// x[3i..3i+2] = xw[i].xyz.
+hipError_t launchWriteCoordinates(const float4* xw, int numAtoms, float* x, hipStream_t stream);
 // f[3a..] (+)= nonbonded force of slot cells[a] (+ fToAdd[3a..] if non-null);
 // fixed-point slots (fFixed non-null) are converted and cleared, else fFloat is read.
 hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, const int* cells, int numAtoms, const float* fToAdd, float* f, bool accumulate, hipStream_t stream);

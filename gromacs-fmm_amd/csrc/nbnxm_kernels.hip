@@ -512,6 +512,68 @@ __global__ void xToXqKernel(const float* x, const int* atomIndices, int numSlots
     xq[slot] = make_float4(x[3 * static_cast<size_t>(atom)], x[3 * static_cast<size_t>(atom) + 1], x[3 * static_cast<size_t>(atom) + 2], w);
 }
 
+// Grid-order atom data of a list built on the device (the host's fillXq,
+// fillTypes and fillLjComb): fillers at -1e6 with q = 0, the dummy type and
+// zero combination parameters; also the atom -> slot map of the force
+// reduction.  slotTypes / slotComb may be null.
+__global__ void gatherAtomDataKernel(const int* atomIndices,
+                                     int numSlots,
+                                     const float4* xw,
+                                     const float* q,
+                                     const int* types,
+                                     const float2* typeComb,
+                                     int dummyType,
+                                     float4* xq,
+                                     int* slotTypes,
+                                     float2* slotComb,
+                                     int* cells)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= numSlots)
+    {
+        return;
+    }
+    const int atom = atomIndices[slot];
+    if (atom < 0)
+    {
+        constexpr float farAway = -1000000.0f;
+        xq[slot] = make_float4(farAway, farAway, farAway, 0.0f);
+        if (slotTypes)
+        {
+            slotTypes[slot] = dummyType;
+        }
+        if (slotComb)
+        {
+            slotComb[slot] = make_float2(0.0f, 0.0f);
+        }
+        return;
+    }
+    const float4 p = xw[atom];
+    xq[slot] = make_float4(p.x, p.y, p.z, q[atom]);
+    if (slotTypes)
+    {
+        slotTypes[slot] = types[atom];
+    }
+    if (slotComb)
+    {
+        slotComb[slot] = typeComb[types[atom]];
+    }
+    cells[atom] = slot;
+}
+
+// Wrapped coordinates back into a 3-float-per-atom array.
+__global__ void writeCoordinatesKernel(const float4* xw, int numAtoms, float* x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numAtoms)
+    {
+        const float4 p = xw[i];
+        x[3 * static_cast<size_t>(i)] = p.x;
+        x[3 * static_cast<size_t>(i) + 1] = p.y;
+        x[3 * static_cast<size_t>(i) + 2] = p.z;
+    }
+}
+
 // Force reduction into atom order (reduceKernel,
 // src/gromacs/mdlib/gpuforcereduction_impl_internal.cu:62-120): f[a] is set
 // to (or, with accumulate, incremented by) the nonbonded force of slot
@@ -763,6 +825,38 @@ hipError_t launchVirialFinish(const double* partials, int numSystems, int blocks
         return hipSuccess;
     }
     hipLaunchKernelGGL(virialFinishKernel, dim3(numSystems), dim3(64), 0, stream, partials, blocksPerSystem, shiftVec, fshift, virial);
+    return hipGetLastError();
+}
+
+hipError_t launchGatherAtomData(const int* atomIndices,
+                                int numSlots,
+                                const float4* xw,
+                                const float* q,
+                                const int* types,
+                                const float2* typeComb,
+                                int dummyType,
+                                float4* xq,
+                                int* slotTypes,
+                                float2* slotComb,
+                                int* cells,
+                                hipStream_t stream)
+{
+    if (numSlots <= 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(gatherAtomDataKernel, dim3((numSlots + 255) / 256), dim3(256), 0, stream, atomIndices, numSlots, xw, q, types, typeComb, dummyType, xq,
+                       slotTypes, slotComb, cells);
+    return hipGetLastError();
+}
+
+hipError_t launchWriteCoordinates(const float4* xw, int numAtoms, float* x, hipStream_t stream)
+{
+    if (numAtoms <= 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(writeCoordinatesKernel, dim3((numAtoms + 255) / 256), dim3(256), 0, stream, xw, numAtoms, x);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 
 #include "capi_common.h"
 #include "gpu_search.h"
+#include "nbnxm_kernels.h"
 #include "pairsearch.h"
 
 using namespace nbnxm_amd;
@@ -50,7 +51,16 @@ struct nbnxm_verlet
 
     GridSearch search;
     nbnxm_gpu_search_t* gpuSearch = nullptr; // device search, when asked for
-    std::vector<double> hxd;                 // its FP64 host input
+    std::vector<double> hxd;                 // its FP64 host input (first search)
+    // Per-atom inputs of device searches after the first: charges, types and
+    // per-type combination parameters on the device.
+    DeviceBuffer<float> dCharges;
+    DeviceBuffer<int> dTypes;
+    DeviceBuffer<float2> dTypeComb;
+    bool exclUploaded = false;
+    // `search` holds the current list (false after a device search, until a
+    // caller asks for the list).
+    bool hostListCurrent = false;
     nbnxm_gpu_t* gpu = nullptr;
     hipEvent_t waitEvent = nullptr;
     bool haveList = false;
@@ -97,6 +107,8 @@ struct nbnxm_verlet
     }
 
     int searchAndUpload(float* d_x, const double* newBox);
+    int searchOnDevice(float* d_x, const double* newBox);
+    int ensureHostList();
     int buildList(int maxPairsPerEntry, bool regrid);
 };
 
@@ -140,6 +152,10 @@ int nbnxm_verlet::buildList(int maxPairsPerEntry, bool regrid)
 
 int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
 {
+    if (params.gpuSearch && gpu && gpuSearch && numAtoms > 0)
+    {
+        return searchOnDevice(d_x, newBox);
+    }
     const size_t n3 = 3 * static_cast<size_t>(numAtoms);
     hx.resize(n3);
     if (numAtoms > 0)
@@ -239,8 +255,71 @@ int nbnxm_verlet::searchAndUpload(float* d_x, const double* newBox)
         return rc;
     }
     haveList = true;
+    hostListCurrent = true;
     numSearches++;
     return NBNXM_OK;
+}
+
+// Searches after the first with the device search: coordinates, list and
+// grid-order atom data stay on the device (the list is built from d_x, the
+// wrapped coordinates are written back into it, the task adopts the list and
+// gathers its atom data); only the entries pass through the host.  The entry
+// split target of the previous search is kept.
+int nbnxm_verlet::searchOnDevice(float* d_x, const double* newBox)
+{
+    if (dCharges.size == 0)
+    {
+        NBNXM_HIP_CHECK(dCharges.resize(numAtoms));
+        NBNXM_HIP_CHECK(dTypes.resize(numAtoms));
+        NBNXM_HIP_CHECK(hipMemcpy(dCharges.ptr, charges.data(), sizeof(float) * numAtoms, hipMemcpyHostToDevice));
+        NBNXM_HIP_CHECK(hipMemcpy(dTypes.ptr, types.data(), sizeof(int) * numAtoms, hipMemcpyHostToDevice));
+        if (vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
+        {
+            NBNXM_HIP_CHECK(dTypeComb.resize(numTypes));
+            NBNXM_HIP_CHECK(hipMemcpy(dTypeComb.ptr, perTypeComb.data(), sizeof(float2) * numTypes, hipMemcpyHostToDevice));
+        }
+    }
+    std::copy(newBox, newBox + 3, box);
+    const int* er = exclRanges.empty() ? nullptr : exclRanges.data();
+    const int* ee = exclRanges.empty() ? nullptr : exclElements.data();
+    DeviceListView view;
+    int rc = gpuSearchBuildDevice(gpuSearch, numAtoms, d_x, nullptr, box, er, ee, !exclUploaded, params.rlistOuter,
+                                  splitTarget > 0 ? splitTarget : 0, &view);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    exclUploaded = true;
+    // Atoms are put in the box on search steps (do_md, put_atoms_in_box_omp).
+    NBNXM_HIP_CHECK(launchWriteCoordinates(view.xw, numAtoms, d_x, gpuSearchStream(gpuSearch)));
+    if ((rc = gpuTaskAdoptDeviceList(gpu, view, dCharges.ptr, dTypes.ptr, dTypeComb.ptr, numTypes, box)) != NBNXM_OK)
+    {
+        return rc;
+    }
+    haveList = true;
+    hostListCurrent = false;
+    numSearches++;
+    return NBNXM_OK;
+}
+
+// The host copy of the current list (after a device search: copied from the
+// device search and handed to `search`).
+int nbnxm_verlet::ensureHostList()
+{
+    if (hostListCurrent || !haveList)
+    {
+        return NBNXM_OK;
+    }
+    int rc = gpuSearchDownloadViews(gpuSearch);
+    if (rc == NBNXM_OK)
+    {
+        rc = gpuSearchExportSystem(gpuSearch, 0, search);
+    }
+    if (rc == NBNXM_OK)
+    {
+        hostListCurrent = true;
+    }
+    return rc;
 }
 
 namespace
@@ -506,6 +585,12 @@ int nbnxm_verlet_get_stats(const nbnxm_verlet_t* v, long long* counts, int* flag
     }
     if (counts)
     {
+        // Refreshing the host copy of the list changes no observable state.
+        const int rcList = const_cast<nbnxm_verlet*>(v)->ensureHostList();
+        if (rcList != NBNXM_OK)
+        {
+            return rcList;
+        }
         counts[0] = v->numSearches;
         counts[1] = v->numPrunes;
         counts[2] = v->numSteps;
@@ -529,6 +614,11 @@ int nbnxm_verlet_get_list(nbnxm_verlet_t* v, nbnxm_sci_t* sci, nbnxm_cj_packed_t
     if (!v->haveList)
     {
         return fail(NBNXM_ERR_STATE, "verlet_get_list: no list yet");
+    }
+    int rcList = v->ensureHostList();
+    if (rcList != NBNXM_OK)
+    {
+        return rcList;
     }
     const nbnxm_pairlist_gpu_t view = v->search.listView();
     if (sizes)
