@@ -27,7 +27,7 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 # + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy) from two rocprofv3 --pmc
 # runs, profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
 # inside the timed run, so the measured figure is carried here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.924e9
+TRAFFIC_BYTES_PER_LAUNCH = 1.906e9
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_VF = 107       # Ewald + LJ force+energy (src/gromacs/gmxlib/nrnb.cpp:90-103)
 RC, RLIST_OUTER, RLIST_INNER, EWALD_RTOL, SIGMA = 1.0, 1.1, 1.04, 1e-5, 0.002
@@ -243,11 +243,12 @@ def main():
             sys.exit(f"--split needs {len(BUCKET_EDGES) + 1} values")
         ens.set_split(BUCKET_EDGES, split, args.wpb)
         tuned = {"bucket_edges_atoms": BUCKET_EDGES, "max_pairs_per_item": split, "waves_per_block": args.wpb,
-                 "fixed": True}
+                 "i_cluster_in_lds": ens.launch_config()[1], "fixed": True}
     elif not args.no_autotune:
         t0 = time.perf_counter()
         split, wpb = ens.autotune(BUCKET_EDGES)
-        tuned = {"bucket_edges_atoms": BUCKET_EDGES, "max_pairs_per_item": split, "waves_per_block": wpb}
+        tuned = {"bucket_edges_atoms": BUCKET_EDGES, "max_pairs_per_item": split, "waves_per_block": wpb,
+                 "i_cluster_in_lds": ens.launch_config()[1]}
         log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned}")
     stats = ens.stats()
 

@@ -342,7 +342,10 @@ int enqueueStep(nbnxm_ensemble* e, int mode, float sigma, uint64_t seed, bool co
             NbKernelArgs ak = a;
             ak.sciOrder = e->dPlainOrder.ptr + s.workBegin;
             ak.nsci = s.workEnd - s.workBegin;
-            NBNXM_HIP_CHECK(launchNonbonded(ak, e->opt.accumulation, e->vdwType, computeEnergy, e->launchConfig, e->stream));
+            // Per-system launches choose the i-cluster placement by their own entry count.
+            NbLaunchConfig pc = e->launchConfig;
+            pc.iClusterInLds = -1;
+            NBNXM_HIP_CHECK(launchNonbonded(ak, e->opt.accumulation, e->vdwType, computeEnergy, pc, e->stream));
             if (fixed)
             {
                 NBNXM_HIP_CHECK(launchConvertForcesRange(e->fFixed.ptr, s.slotOffset, s.numSlots, e->fOut.ptr, e->fOutDouble.ptr, e->stream));
@@ -812,6 +815,41 @@ int nbnxm_ensemble_set_split(nbnxm_ensemble_t* e, int numBuckets, const int* buc
     return applySplit(e, edges, std::vector<int>(maxPairs, maxPairs + numBuckets));
 }
 
+int nbnxm_ensemble_set_i_cluster_in_lds(nbnxm_ensemble_t* e, int mode)
+{
+    if (!e || mode < -1 || mode > 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "ensemble_set_i_cluster_in_lds: mode -1, 0 or 1");
+    }
+    if (e->launchConfig.iClusterInLds != mode)
+    {
+        e->launchConfig.iClusterInLds = mode;
+        const int rc = dropGraph(e);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+    }
+    return NBNXM_OK;
+}
+
+int nbnxm_ensemble_get_launch_config(nbnxm_ensemble_t* e, int* wavesPerBlock, int* iClusterInLds)
+{
+    if (!e)
+    {
+        return fail(NBNXM_ERR_INVALID, "ensemble_get_launch_config: null ensemble");
+    }
+    if (wavesPerBlock)
+    {
+        *wavesPerBlock = e->launchConfig.wavesPerBlock;
+    }
+    if (iClusterInLds)
+    {
+        *iClusterInLds = iClusterDataInLds(e->launchConfig, kernelArgs(e).nsci) ? 1 : 0;
+    }
+    return NBNXM_OK;
+}
+
 int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int* chosenMaxPairs, int* chosenWavesPerBlock)
 {
     if (!e || numBuckets < 1 || (numBuckets > 1 && !bucketEdges))
@@ -894,6 +932,25 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
         }
     }
     e->launchConfig.wavesPerBlock = bestW;
+    // The other i-cluster placement of the tuned kernel than the automatic one.
+    const int autoLds = iClusterDataInLds(e->launchConfig, kernelArgs(e).nsci) ? 1 : 0;
+    e->launchConfig.iClusterInLds = 1 - autoLds;
+    {
+        double ms = 0.0;
+        rc = timeBatched(e, reps, &ms, blocks);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+        if (!(ms < bestMs * margin))
+        {
+            e->launchConfig.iClusterInLds = autoLds;
+        }
+        else
+        {
+            bestMs = ms;
+        }
+    }
     rc = dropGraph(e);
     if (rc != NBNXM_OK)
     {

@@ -1438,6 +1438,16 @@ int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock)
     return NBNXM_OK;
 }
 
+int nbnxm_gpu_set_i_cluster_in_lds(nbnxm_gpu_t* nb, int mode)
+{
+    if (!nb || mode < -1 || mode > 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_set_i_cluster_in_lds: mode -1, 0 or 1");
+    }
+    nb->launchConfig.iClusterInLds = mode;
+    return NBNXM_OK;
+}
+
 void nbnxm_gpu_free(nbnxm_gpu_t* nb)
 {
     delete nb;

@@ -137,14 +137,23 @@ __host__ __device__ constexpr size_t typeTableLdsBytes(int numTypes)
     return (sizeof(float2) * numTypes * numTypes + 15) / 16 * 16;
 }
 
-template<bool computeEnergy, int vdwType, int kFlav>
+// Waves per SIMD the register allocation must allow.  With the i-cluster
+// data in LDS the tuned kernels fit 128 VGPRs without spilling (4 waves),
+// except F+E with the type table, which stays at 3.
+template<bool computeEnergy, int vdwType, int kFlav, bool kILds>
 constexpr int minWavesPerSimd()
 {
+    if (kILds)
+    {
+        return (computeEnergy && vdwType == NBNXM_VDW_CUT_TYPE_TABLE) ? 1 : 4;
+    }
     return (kFlav == c_flavRuntime && !computeEnergy && vdwType != NBNXM_VDW_CUT_TYPE_TABLE) ? 3 : 1;
 }
 
-template<bool computeEnergy, int vdwType, int accumulation, int kFlav>
-__global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
+// kILds: the i-cluster coordinates and charges are read from per-wave LDS for
+// every cluster pair instead of being held in 32 registers (tuned class only).
+template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false>
+__global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
 {
     constexpr bool generic = kFlav != c_flavTuned;
     const int lane = threadIdx.x & 63;
@@ -206,21 +215,33 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     const float4 sh = loadF4(asConst(a.shiftVec), s.shift);
     [[maybe_unused]] const FlavourConsts<float> fk(flav);
 
-    // i-atom coordinates and charges of all 8 clusters in registers, and the
-    // type-table row offsets too; combination parameters in per-wave LDS.
+    // i-atom coordinates (shifted) and charges (times epsfac) of all 8
+    // clusters in registers, or with kILds in per-wave LDS; the type-table row
+    // offsets in registers; combination parameters in per-wave LDS.
     __shared__ float2 sLi[4][c_ncps * c_cl];
     float2* stageLi = sLi[threadIdx.x >> 6];
-    float xi[c_ncps], yi[c_ncps], zi[c_ncps], qi[c_ncps];
+    static_assert(!kILds || !generic, "i-cluster data in LDS: tuned class only");
+    constexpr bool iInLds = kILds;
+    __shared__ float4 sI[iInLds ? 4 : 1][iInLds ? c_ncps * c_cl : 1];
+    float4* stageI = sI[iInLds ? (threadIdx.x >> 6) : 0];
+    float xi[iInLds ? 1 : c_ncps], yi[iInLds ? 1 : c_ncps], zi[iInLds ? 1 : c_ncps], qi[iInLds ? 1 : c_ncps];
     [[maybe_unused]] int tOffI[c_ncps]; // type table: row offset numTypes * type_i, in registers
 #pragma unroll
     for (int c = 0; c < c_ncps; c++)
     {
         const int ai = (s.sci * c_ncps + c) * c_cl + ti;
         const float4 v = loadF4(xqc, ai);
-        xi[c] = v.x + sh.x;
-        yi[c] = v.y + sh.y;
-        zi[c] = v.z + sh.z;
-        qi[c] = v.w * epsfac;
+        if constexpr (iInLds)
+        {
+            stageI[c * c_cl + ti] = make_float4(v.x + sh.x, v.y + sh.y, v.z + sh.z, v.w * epsfac);
+        }
+        else
+        {
+            xi[c] = v.x + sh.x;
+            yi[c] = v.y + sh.y;
+            zi[c] = v.z + sh.z;
+            qi[c] = v.w * epsfac;
+        }
         if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
         {
             tOffI[c] = a.numTypes * a.atomTypes[ai];
@@ -291,7 +312,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
 #pragma unroll
                 for (int c = 0; c < c_ncps; c++)
                 {
-                    q2 += qi[c] * qi[c];
+                    const float qc = iInLds ? stageI[c * c_cl + ti].w : qi[iInLds ? 0 : c];
+                    q2 += qc * qc;
                 }
                 eElAcc -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
             }
@@ -383,9 +405,18 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 {
                     continue;
                 }
-                const float dx = xi[c] - xqj.x;
-                const float dy = yi[c] - xqj.y;
-                const float dz = zi[c] - xqj.z;
+                float4 iv;
+                if constexpr (iInLds)
+                {
+                    iv = stageI[c * c_cl + ti];
+                }
+                else
+                {
+                    iv = make_float4(xi[c], yi[c], zi[c], qi[c]);
+                }
+                const float dx = iv.x - xqj.x;
+                const float dy = iv.y - xqj.y;
+                const float dz = iv.z - xqj.z;
                 float r2 = norm2Fixed(dx, dy, dz);
                 // Cluster pairs on the diagonal of the central shift always carry
                 // the Newton mask in their exclusion words, so they take the
@@ -429,7 +460,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     r2 = __builtin_amdgcn_fmed3f(r2, NBNXM_MIN_DISTANCE_SQUARED, rcoulomb_sq);
                     const float inv_r = __builtin_amdgcn_rsqf(r2);
                     const float inv_r2 = inv_r * inv_r;
-                    const float qq = qi[c] * xqj.w;
+                    const float qq = iv.w * xqj.w;
                     float F_invr;
                     if constexpr (generic)
                     {
@@ -705,7 +736,7 @@ constexpr size_t c_maxLdsTypeTableBytes = sizeof(float2) * c_maxLdsTableTypes * 
 // falls back to the run-time class, which reads it from global memory).
 constexpr size_t c_maxLdsEwaldTableBytes = 32768;
 
-template<bool E, int V, int A, int kFlav>
+template<bool E, int V, int A, int kFlav, bool kILds = false>
 hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     const int wpb = cfg.wavesPerBlock;
@@ -722,7 +753,7 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
         return hipErrorInvalidValue;
     }
     const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes;
-    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
+    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
     return hipGetLastError();
 }
 
@@ -730,9 +761,22 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
 template<int kFlav>
 hipError_t launchTunedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
 {
+    const bool iLds = (kFlav == c_flavTuned) && iClusterDataInLds(cfg, a.nsci);
     auto acc = [&](auto e, auto v) -> hipError_t {
         constexpr bool E = decltype(e)::value;
         constexpr int V = decltype(v)::value;
+        if constexpr (kFlav == c_flavTuned)
+        {
+            if (iLds)
+            {
+                switch (accumulation)
+                {
+                    case NBNXM_ACCUM_FLOAT: return launchTunedKernel<E, V, NBNXM_ACCUM_FLOAT, kFlav, true>(a, flav, cfg, stream);
+                    case NBNXM_ACCUM_FIXED64: return launchTunedKernel<E, V, NBNXM_ACCUM_FIXED64, kFlav, true>(a, flav, cfg, stream);
+                    default: return hipErrorInvalidValue;
+                }
+            }
+        }
         switch (accumulation)
         {
             case NBNXM_ACCUM_FLOAT: return launchTunedKernel<E, V, NBNXM_ACCUM_FLOAT, kFlav>(a, flav, cfg, stream);

@@ -50,7 +50,23 @@ enum class KernelFlavor : int
 struct NbLaunchConfig
 {
     int wavesPerBlock = 4;
+    // Tuned kernels: i-cluster data in LDS at 4 waves per SIMD (1), in
+    // registers at 3 (0), or chosen by the entry count (-1).
+    int iClusterInLds = -1;
 };
+
+// Entries from which the automatic choice puts the i-cluster data in LDS
+// (three rounds of 4,096 resident waves on 256 CUs): with several entries per
+// wave slot the fourth wave per SIMD pays for the extra LDS read per cluster
+// pair; with about one entry per slot the shorter register path wins.
+// Measured crossover on water boxes: registers 3 % faster at 7,200 entries,
+// LDS 5-7 % faster from 15,000.
+constexpr int c_iClusterLdsMinEntries = 12288;
+
+inline bool iClusterDataInLds(const NbLaunchConfig& cfg, int nsci)
+{
+    return cfg.iClusterInLds >= 0 ? cfg.iClusterInLds != 0 : nsci >= c_iClusterLdsMinEntries;
+}
 
 // Flavour classes of the tuned list walk: the tuned flavour, the run-time
 // flavour (any combination, read from NbFlavourParams), and combinations

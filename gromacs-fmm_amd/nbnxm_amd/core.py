@@ -312,6 +312,10 @@ class GpuTask:
     def set_waves_per_block(self, w):
         check(self._lib.nbnxm_gpu_set_waves_per_block(self._h, int(w)), "gpu_set_waves_per_block")
 
+    def set_i_cluster_in_lds(self, mode):
+        """1: i-cluster data in LDS (4 waves per SIMD), 0: in registers, -1: by entry count."""
+        check(self._lib.nbnxm_gpu_set_i_cluster_in_lds(self._h, int(mode)), "gpu_set_i_cluster_in_lds")
+
     def fetch(self, want_double=True):
         """cpyback + wait: returns (f_float, f_double or None, e_lj, e_el, fshift)."""
         n3 = 3 * self.atoms.num_atoms

@@ -85,6 +85,17 @@ class Ensemble:
         check(self._lib.nbnxm_ensemble_autotune(self._h, len(chosen), ptr(edges) if edges.size else None, ptr(chosen), ctypes.byref(w)), "ensemble_autotune")
         return chosen.tolist(), w.value
 
+    def set_i_cluster_in_lds(self, mode):
+        """Batched launch: 1 i-cluster data in LDS, 0 in registers, -1 by entry count."""
+        check(self._lib.nbnxm_ensemble_set_i_cluster_in_lds(self._h, int(mode)), "ensemble_set_i_cluster_in_lds")
+
+    def launch_config(self):
+        """(waves per block, i-cluster data in LDS 0/1) of the batched launch."""
+        w = ctypes.c_int(0)
+        lds = ctypes.c_int(0)
+        check(self._lib.nbnxm_ensemble_get_launch_config(self._h, ctypes.byref(w), ctypes.byref(lds)), "ensemble_get_launch_config")
+        return w.value, lds.value
+
     def run(self, steps, mode=MODE_BATCHED, sigma=0.0, seed=1, compute_energy=False, use_graph=False, steps_per_graph=0):
         check(self._lib.nbnxm_ensemble_run(self._h, mode, steps, sigma, seed, int(compute_energy), int(use_graph), steps_per_graph), "ensemble_run")
 

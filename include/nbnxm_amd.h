@@ -336,6 +336,11 @@ int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int*
 int nbnxm_gpu_get_sci_order(nbnxm_gpu_t* nb, int* order, int* sciCount);
 /* Launch shape of the nonbonded kernel: waves (entries) per workgroup, 1..4. */
 int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock);
+/* Where the tuned kernel keeps the i-cluster coordinates and charges: 1 in
+ * LDS (4 waves per SIMD), 0 in registers (3 waves per SIMD), -1 chosen by
+ * the entry count (LDS from 12288 entries).  Launch tuning with no
+ * counterpart in the reference; results are bit-identical either way. */
+int nbnxm_gpu_set_i_cluster_in_lds(nbnxm_gpu_t* nb, int mode);
 /* gpu_free. */
 void nbnxm_gpu_free(nbnxm_gpu_t* nb);
 
@@ -388,6 +393,11 @@ int nbnxm_ensemble_set_split(nbnxm_ensemble_t* e, int numBuckets, const int* buc
 /* Times candidate splits per size bucket and keeps the fastest; writes the
  * chosen maxPairs per bucket and waves per block when the pointers are set. */
 int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int* chosenMaxPairs, int* chosenWavesPerBlock);
+/* The i-cluster placement of the batched launch (as nbnxm_gpu_set_i_cluster_in_lds;
+ * the autotuner sets it).  Per-system launches always choose by entry count. */
+int nbnxm_ensemble_set_i_cluster_in_lds(nbnxm_ensemble_t* e, int mode);
+/* The batched launch's waves per block and i-cluster placement (resolved: 0 or 1). */
+int nbnxm_ensemble_get_launch_config(nbnxm_ensemble_t* e, int* wavesPerBlock, int* iClusterInLds);
 /* Runs numSteps evaluations: x = x_search + N(0, sigma) per atom and step
  * (sigma 0: the search positions), nonbonded forces, FP64 energies and shift
  * forces.  useGraph: capture stepsPerGraph evaluations in a hipGraph and replay

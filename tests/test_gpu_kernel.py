@@ -110,3 +110,29 @@ def test_newton_third_law(water9k):
     f = r["fd"].reshape(-1, 3)[real]
     # Pair forces cancel, so the total force is zero up to fixed-point rounding.
     assert np.abs(f.sum(axis=0)).max() <= 1e-6 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("vdw", [nb.VDW_CUT_COMB_GEOM, nb.VDW_CUT_TYPE_TABLE, nb.VDW_CUT_COMB_LB])
+@pytest.mark.parametrize("energy", [True, False])
+def test_i_cluster_placement_bit_identical(water9k, vdw, energy):
+    """The tuned kernel with the i-cluster coordinates and charges in LDS (4
+    waves per SIMD) and in registers (3 waves per SIMD): the same arithmetic
+    in the same order, so the fixed-point forces, energies and shift forces
+    agree bit for bit; both match the FP32 oracle."""
+    announce(f"i-cluster placement vdw={vdw} energy={energy}")
+    ic = nb.interaction_const(1.0, 1e-5)
+    _, plist, atoms = nb.build(water9k, 1.1, vdw)
+    out = []
+    for mode in (0, 1):
+        task = nb.GpuTask(ic, atoms, 1.1, None, nb.ACCUM_FIXED64)
+        task.set_i_cluster_in_lds(mode)
+        task.init_pairlist(plist)
+        task.launch(compute_energy=energy, compute_virial=True)
+        out.append(task.fetch())
+    (_, fd0, elj0, eel0, fs0), (_, fd1, elj1, eel1, fs1) = out
+    assert np.array_equal(fd0, fd1)
+    assert np.array_equal(fs0, fs1)
+    assert elj0 == elj1 and eel0 == eel1
+    f32, _, _, _ = oracle.kernel(plist, atoms, ic, energy, "f32")
+    real = atoms.atom_indices >= 0
+    assert rel_rms(fd1.reshape(-1, 3)[real], f32[real]) <= 1e-5
