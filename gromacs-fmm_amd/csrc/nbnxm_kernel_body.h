@@ -137,21 +137,47 @@ __host__ __device__ constexpr size_t typeTableLdsBytes(int numTypes)
     return (sizeof(float2) * numTypes * numTypes + 15) / 16 * 16;
 }
 
-// Waves per SIMD the register allocation must allow.  With the i-cluster
-// data in LDS the tuned kernels fit 128 VGPRs without spilling (4 waves),
-// except F+E with the type table, which stays at 3.
+// Waves per SIMD an instantiation reaches without spilling once the
+// i-cluster data is in LDS, where that is more than with the data in
+// registers (0: no LDS variant).  4 (128 VGPRs): every F kernel of the
+// compile-time classes, and F+E with a combination rule for the tuned,
+// reaction-field and tabulated-Ewald classes.  3 (168 VGPRs): F+E of the
+// LJ force-switch and LJ-PME classes, which take 172-184 VGPRs (2 waves)
+// with the data in registers.  The other F+E kernels spill 20-92 B per lane
+// at 128 VGPRs and already run 3 waves with the data in registers.
+template<bool computeEnergy, int vdwType, int kFlav>
+constexpr int iClusterLdsWaves()
+{
+    if (kFlav == c_flavRuntime)
+    {
+        return 0;
+    }
+    if (!computeEnergy)
+    {
+        return 4;
+    }
+    if (kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldLjPme)
+    {
+        return 3;
+    }
+    const bool fits4 = vdwType != NBNXM_VDW_CUT_TYPE_TABLE
+                       && (kFlav == c_flavTuned || kFlav == c_flavRfPotShift || kFlav == c_flavEwaldTabPotShift);
+    return fits4 ? 4 : 0;
+}
+
+// Waves per SIMD the register allocation must allow.
 template<bool computeEnergy, int vdwType, int kFlav, bool kILds>
 constexpr int minWavesPerSimd()
 {
     if (kILds)
     {
-        return (computeEnergy && vdwType == NBNXM_VDW_CUT_TYPE_TABLE) ? 1 : 4;
+        return iClusterLdsWaves<computeEnergy, vdwType, kFlav>();
     }
     return (kFlav == c_flavRuntime && !computeEnergy && vdwType != NBNXM_VDW_CUT_TYPE_TABLE) ? 3 : 1;
 }
 
 // kILds: the i-cluster coordinates and charges are read from per-wave LDS for
-// every cluster pair instead of being held in 32 registers (tuned class only).
+// every cluster pair instead of being held in 32 registers.
 template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false>
 __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
 {
@@ -220,7 +246,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     // offsets in registers; combination parameters in per-wave LDS.
     __shared__ float2 sLi[4][c_ncps * c_cl];
     float2* stageLi = sLi[threadIdx.x >> 6];
-    static_assert(!kILds || !generic, "i-cluster data in LDS: tuned class only");
+    static_assert(!kILds || iClusterLdsWaves<computeEnergy, vdwType, kFlav>() > 0, "no LDS variant for this instantiation");
     constexpr bool iInLds = kILds;
     __shared__ float4 sI[iInLds ? 4 : 1][iInLds ? c_ncps * c_cl : 1];
     float4* stageI = sI[iInLds ? (threadIdx.x >> 6) : 0];
@@ -761,11 +787,11 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
 template<int kFlav>
 hipError_t launchTunedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
 {
-    const bool iLds = (kFlav == c_flavTuned) && iClusterDataInLds(cfg, a.nsci);
+    const bool iLds = iClusterDataInLds(cfg, a.nsci);
     auto acc = [&](auto e, auto v) -> hipError_t {
         constexpr bool E = decltype(e)::value;
         constexpr int V = decltype(v)::value;
-        if constexpr (kFlav == c_flavTuned)
+        if constexpr (iClusterLdsWaves<E, V, kFlav>() > 0)
         {
             if (iLds)
             {

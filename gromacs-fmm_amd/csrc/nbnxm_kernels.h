@@ -50,8 +50,9 @@ enum class KernelFlavor : int
 struct NbLaunchConfig
 {
     int wavesPerBlock = 4;
-    // Tuned kernels: i-cluster data in LDS at 4 waves per SIMD (1), in
-    // registers at 3 (0), or chosen by the entry count (-1).
+    // Compile-time flavour classes: i-cluster data in LDS at 4 waves per
+    // SIMD (1; instantiations that would spill keep registers), in registers
+    // (0), or chosen by the entry count (-1).
     int iClusterInLds = -1;
 };
 

@@ -336,10 +336,13 @@ int nbnxm_gpu_get_masks(nbnxm_gpu_t* nb, unsigned int* imaskInner, unsigned int*
 int nbnxm_gpu_get_sci_order(nbnxm_gpu_t* nb, int* order, int* sciCount);
 /* Launch shape of the nonbonded kernel: waves (entries) per workgroup, 1..4. */
 int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock);
-/* Where the tuned kernel keeps the i-cluster coordinates and charges: 1 in
- * LDS (4 waves per SIMD), 0 in registers (3 waves per SIMD), -1 chosen by
- * the entry count (LDS from 12288 entries).  Launch tuning with no
- * counterpart in the reference; results are bit-identical either way. */
+/* Where the list walk keeps the i-cluster coordinates and charges: 1 in LDS
+ * at 4 waves per SIMD (every F kernel of the compile-time flavour classes;
+ * F+E of the tuned, reaction-field and tabulated-Ewald classes with a
+ * combination rule; the other kernels keep registers), 0 in registers (3
+ * waves per SIMD), -1 chosen by the entry count (LDS from 12288 entries).
+ * Launch tuning with no counterpart in the reference; fixed-point results
+ * are bit-identical either way. */
 int nbnxm_gpu_set_i_cluster_in_lds(nbnxm_gpu_t* nb, int mode);
 /* gpu_free. */
 void nbnxm_gpu_free(nbnxm_gpu_t* nb);

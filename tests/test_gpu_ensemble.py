@@ -86,16 +86,19 @@ def test_i_cluster_placement_bitwise(systems, ens, mode):
     announce(f"batched i-cluster placement {mode} vs plain bitwise")
     ens.set_i_cluster_in_lds(mode)
     assert ens.launch_config()[1] == (mode if mode >= 0 else 0)
-    ens.set_step(5)
-    ens.run(1, nb.MODE_BATCHED, 0.002, 9, True)
-    fb = [ens.forces(k) for k in range(len(systems))]
-    eb, fsb = ens.energies()
-    ens.set_step(5)
-    ens.run(1, nb.MODE_PLAIN, 0.002, 9, True)
-    for k in range(len(systems)):
-        assert np.array_equal(ens.forces(k), fb[k]), k
-    ep, fsp = ens.energies()
-    assert np.array_equal(ep, eb) and np.array_equal(fsp, fsb)
+    # The type-table F+E kernel keeps registers; the F kernel takes the setting.
+    for energy in (True, False):
+        ens.set_step(5)
+        ens.run(1, nb.MODE_BATCHED, 0.002, 9, energy)
+        fb = [ens.forces(k) for k in range(len(systems))]
+        eb, fsb = ens.energies()
+        ens.set_step(5)
+        ens.run(1, nb.MODE_PLAIN, 0.002, 9, energy)
+        for k in range(len(systems)):
+            assert np.array_equal(ens.forces(k), fb[k]), (energy, k)
+        if energy:
+            ep, fsp = ens.energies()
+            assert np.array_equal(ep, eb) and np.array_equal(fsp, fsb)
 
 
 def test_graph_replays_equal_eager(systems, ens):

@@ -104,6 +104,36 @@ def water3k():
     return s, plist, atoms
 
 
+@pytest.mark.parametrize("name", ["tabulated", "rf", "twin", "forceswitch", "potswitch", "ljpme"])
+def test_i_cluster_placement_flavours(water3k, name):
+    """Each compile-time flavour class with the i-cluster data in LDS and in
+    registers, F and F+E: fixed-point forces, energies and shift forces agree
+    bit for bit (combination rule; LJ-PME with the type table), and match the
+    FP64 oracle."""
+    announce(f"flavour i-cluster placement {name}")
+    s, plist, atoms = water3k
+    if name != "ljpme":
+        _, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_COMB_GEOM)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5, **FLAVOURS[name])
+    real = atoms.atom_indices >= 0
+    for energy in (False, True):
+        out = []
+        for mode in (0, 1):
+            task = nb.GpuTask(ic, atoms, 1.1, None, nb.ACCUM_FIXED64)
+            task.set_i_cluster_in_lds(mode)
+            task.init_pairlist(plist)
+            task.launch(compute_energy=energy, compute_virial=True)
+            out.append(task.fetch())
+            del task
+        (_, fd0, elj0, eel0, fs0), (_, fd1, elj1, eel1, fs1) = out
+        assert np.array_equal(fd0, fd1), energy
+        assert np.array_equal(fs0, fs1) and elj0 == elj1 and eel0 == eel1
+    f64, _, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    assert rel_rms(fd1.reshape(-1, 3)[real], f64[real]) < 5e-6
+    assert elj1 == pytest.approx(elj64, rel=2e-5)
+    assert eel1 == pytest.approx(eel64, rel=2e-5)
+
+
 @pytest.mark.parametrize("name", sorted(FLAVOURS))
 def test_flavour_water3k(water3k, name):
     """3000-atom water with a buffered list: FP32 kernel vs the FP64 oracle,
