@@ -57,12 +57,12 @@ struct NbLaunchConfig
 };
 
 // Entries from which the automatic choice puts the i-cluster data in LDS
-// (three rounds of 4,096 resident waves on 256 CUs): with several entries per
-// wave slot the fourth wave per SIMD pays for the extra LDS read per cluster
-// pair; with about one entry per slot the shorter register path wins.
-// Measured crossover on water boxes: registers 3 % faster at 7,200 entries,
-// LDS 5-7 % faster from 15,000.
-constexpr int c_iClusterLdsMinEntries = 12288;
+// (two rounds of 4,096 resident waves on 256 CUs): with several entries per
+// wave slot the extra wave per SIMD pays for the extra LDS read per cluster
+// pair; with one or two entries per slot the shorter register path wins.
+// Measured on water boxes: registers 3 % faster at 7,200 entries, LDS 3-6 %
+// faster from 8,400.
+constexpr int c_iClusterLdsMinEntries = 8192;
 
 inline bool iClusterDataInLds(const NbLaunchConfig& cfg, int nsci)
 {

@@ -340,7 +340,7 @@ int nbnxm_gpu_set_waves_per_block(nbnxm_gpu_t* nb, int wavesPerBlock);
  * at 4 waves per SIMD (every F kernel of the compile-time flavour classes;
  * F+E of the tuned, reaction-field and tabulated-Ewald classes with a
  * combination rule; the other kernels keep registers), 0 in registers (3
- * waves per SIMD), -1 chosen by the entry count (LDS from 12288 entries).
+ * waves per SIMD), -1 chosen by the entry count (LDS from 8192 entries).
  * Launch tuning with no counterpart in the reference; fixed-point results
  * are bit-identical either way. */
 int nbnxm_gpu_set_i_cluster_in_lds(nbnxm_gpu_t* nb, int mode);
