@@ -85,7 +85,9 @@ def test_i_cluster_placement_bitwise(systems, ens, mode):
     or in registers gives the per-system launches' results bit for bit."""
     announce(f"batched i-cluster placement {mode} vs plain bitwise")
     ens.set_i_cluster_in_lds(mode)
-    assert ens.launch_config()[1] == (mode if mode >= 0 else 0)
+    # Automatic choice: LDS from 8192 work items.
+    expect = mode if mode >= 0 else int(ens.stats()["work_items"] >= 8192)
+    assert ens.launch_config()[1] == expect
     # The type-table F+E kernel keeps registers; the F kernel takes the setting.
     for energy in (True, False):
         ens.set_step(5)
