@@ -563,7 +563,8 @@ typedef struct nbnxm_verlet_params
     double rlistInner;   /* pruned radius in [cut-off, rlistOuter]; rlistOuter = no dynamic pruning */
     int nstlistPrune;    /* steps between prunes of the whole list (even, >= 2) */
     int accumulation;    /* enum nbnxm_accumulation */
-    int gpuSearch;       /* nonzero: grid and list built by the device search (nbnxm_gpu_search_*) */
+    int gpuSearch;       /* nonzero: grid and list built by the device search (nbnxm_gpu_search_*);
+                          * searches after the first keep coordinates and list on the device */
 } nbnxm_verlet_params_t;
 
 /* Sets up the system: per-atom types (numTypes types, nbfp (6C6, 12C12)
