@@ -107,6 +107,45 @@ def water96k_setup(nb, vdw):
     return s, search, plist, atoms
 
 
+def placement_table(launches):
+    """One JSON line per (box, flavour, F / F+E, i-cluster placement): water
+    boxes of 96,000, 384,000 and 3,000,000 atoms with unsplit lists, the
+    i-cluster data of the list walk in registers (0) and in LDS (1), wall
+    clock over `launches` launches after warm-up, both placements twice in
+    alternation (the minimum is reported)."""
+    import nbnxm_amd as nb
+
+    names = ["ewald_analytical", "reaction_field", "ewald_tabulated", "ewald_forceswitch", "ewald_twin", "ewald_ljpme"]
+    for nmol, seed in ((32000, 2), (128000, 3), (1000000, 5)):
+        s = nb.systems.water_box(nmol, None, seed=seed)
+        lists = {}
+        for name in names:
+            vdw = nb.VDW_CUT_TYPE_TABLE if name == "ewald_ljpme" else nb.VDW_CUT_COMB_GEOM
+            if vdw not in lists:
+                lists[vdw] = nb.build(s, RLIST_OUTER, vdw)[1:]
+            plist, atoms = lists[vdw]
+            kw = FLAVOUR_OPTIONS[name]
+            ic = nb.interaction_const(RC, EWALD_RTOL) if not kw else nb.interaction_const_ex(RC, ewald_rtol=EWALD_RTOL, **kw)
+            for energy in (False, True):
+                ms = {0: [], 1: []}
+                for mode in (0, 1, 0, 1):
+                    task = nb.GpuTask(ic, atoms, RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
+                    task.set_i_cluster_in_lds(mode)
+                    task.init_pairlist(plist)
+                    for _ in range(3):
+                        task.launch(energy, False)
+                    task.fetch()
+                    t0 = time.perf_counter()
+                    for _ in range(launches):
+                        task.launch(energy, False)
+                    task.fetch()
+                    ms[mode].append((time.perf_counter() - t0) / launches * 1e3)
+                    del task
+                print(json.dumps({"placement": name, "atoms": s.n, "entries": int(len(plist.sci)), "energies": energy,
+                                  "ms_registers": min(ms[0]), "ms_lds": min(ms[1]),
+                                  "lj": "type table" if vdw == nb.VDW_CUT_TYPE_TABLE else "geometric"}), flush=True)
+
+
 def flavour_table(launches):
     """One JSON line per interaction flavour and F / F+E: one evaluation of the
     96,000-atom water box (rc 1.0 nm, list 1.1 nm, fixed point) per launch,
@@ -190,6 +229,8 @@ def main():
                     help="only run 5 plain device copies of MIB MiB (counter calibration for HBM bytes) and exit")
     ap.add_argument("--flavours", type=int, default=0, metavar="LAUNCHES",
                     help="only print the per-flavour kernel table for the 96k-atom water box and exit")
+    ap.add_argument("--placement", type=int, default=0, metavar="LAUNCHES",
+                    help="only print the i-cluster placement table (registers vs LDS) on 96k-3M-atom water boxes and exit")
     ap.add_argument("--md-step", type=int, default=0, metavar="STEPS",
                     help="only time the nonbonded part of an MD step (96k-atom water) and exit")
     args = ap.parse_args()
@@ -198,6 +239,9 @@ def main():
         return
     if args.md_step:
         md_step_table(args.md_step)
+        return
+    if args.placement:
+        placement_table(args.placement)
         return
     if args.calibrate_copy:
         import torch
