@@ -488,6 +488,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     const float inv_r2 = inv_r * inv_r;
                     const float qq = iv.w * xqj.w;
                     float F_invr;
+                    // Energy terms of the tuned paths, accumulated after them.
+                    [[maybe_unused]] float eLj6 = 0.0f, eLjArg = 0.0f, eIb = 1.0f, eElArg = 0.0f;
                     if constexpr (generic)
                     {
                         const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
@@ -514,10 +516,10 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                         F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(z2) * beta3);
                         if constexpr (computeEnergy)
                         {
-                            eLJ12 = __builtin_fmaf(inv_r6, __builtin_fmaf(c12, inv_r6, -2.0f * c6), eLJ12);
-                            eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
-                            eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
-                            eElG += qq * (inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift);
+                            eLj6 = inv_r6;
+                            eLjArg = __builtin_fmaf(c12, inv_r6, -2.0f * c6);
+                            eIb = int_bit;
+                            eElArg = inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift;
                         }
                     }
                     else
@@ -542,10 +544,9 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                             const float u = __builtin_fmaf(R, inv_r, twoBetaSqrtPi);
                             const float tl = __builtin_fmaf(c12, inv_r6, -c6);
                             F_invr = __builtin_fmaf(inv_r6, tl, w * u) * inv_r2;
-                            eLJ12 = __builtin_fmaf(inv_r6, tl - c6, eLJ12);
-                            eSumC6 += c6;
-                            eSumC12 += c12;
-                            eElG = __builtin_fmaf(qq, __builtin_fmaf(e * R, inv_r, -ewaldShift), eElG);
+                            eLj6 = inv_r6;
+                            eLjArg = tl - c6;
+                            eElArg = __builtin_fmaf(e * R, inv_r, -ewaldShift);
                         }
                         else
                         {
@@ -553,6 +554,13 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                             const float coul = qq * __builtin_fmaf(pmecorrF(beta2 * r2), beta3, inv_r2 * inv_r);
                             F_invr = __builtin_fmaf(inv_r6 * tl, inv_r2, coul);
                         }
+                    }
+                    if constexpr (computeEnergy && !generic)
+                    {
+                        eLJ12 = __builtin_fmaf(eLj6, eLjArg, eLJ12);
+                        eSumC6 = __builtin_fmaf(eIb, c6, eSumC6);
+                        eSumC12 = __builtin_fmaf(eIb, c12, eSumC12);
+                        eElG = __builtin_fmaf(qq, eElArg, eElG);
                     }
                     fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
                     fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
