@@ -296,17 +296,21 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
         {
             if constexpr (std::is_same_v<Real, float>)
             {
+                // Both paths give V/qq; one accumulation after them keeps the
+                // accumulator in place (no copies at the join).
+                float elArg;
                 if (exclFree)
                 {
                     // erfc(beta r) = exp(-beta^2 r^2) R(t) instead of 1 - erff.
                     const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(k.betaT, r2 * inv_r, 1.0f));
                     const float e = __builtin_amdgcn_exp2f(r2 * k.negBeta2Log2e);
-                    eEl = __builtin_fmaf(qq, __builtin_fmaf(e * erfcxPolyT(t), inv_r, -k.sh_ewald), eEl);
+                    elArg = __builtin_fmaf(e * erfcxPolyT(t), inv_r, -k.sh_ewald);
                 }
                 else
                 {
-                    eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+                    elArg = inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald;
                 }
+                eEl = __builtin_fmaf(qq, elArg, eEl);
             }
             else
             {
@@ -316,6 +320,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
     }
     else if constexpr (std::is_same_v<Real, float>)
     {
+        [[maybe_unused]] float elArg = 0.0f;
         if (exclFree)
         {
             // No excluded lane (int_bit = 1, wave-uniform; the F+E kernels of the
@@ -329,7 +334,7 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
             F_invr = __builtin_fmaf(qq * e, __builtin_fmaf(R, inv_r, k.twoBetaSqrtPi) * inv_r2, F_invr);
             if constexpr (computeEnergy)
             {
-                eEl = __builtin_fmaf(qq, __builtin_fmaf(e * R, inv_r, -k.sh_ewald), eEl);
+                elArg = __builtin_fmaf(e * R, inv_r, -k.sh_ewald);
             }
         }
         else
@@ -337,8 +342,12 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
             F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(k.beta2 * r2) * k.beta3);
             if constexpr (computeEnergy)
             {
-                eEl += qq * (inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald);
+                elArg = inv_r * (int_bit - erff(r2 * inv_r * k.beta)) - int_bit * k.sh_ewald;
             }
+        }
+        if constexpr (computeEnergy)
+        {
+            eEl = __builtin_fmaf(qq, elArg, eEl);
         }
     }
     else
