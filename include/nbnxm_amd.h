@@ -399,7 +399,9 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
 /* The i-cluster placement of the batched launch (as nbnxm_gpu_set_i_cluster_in_lds;
  * the autotuner sets it).  Per-system launches always choose by entry count. */
 int nbnxm_ensemble_set_i_cluster_in_lds(nbnxm_ensemble_t* e, int mode);
-/* The batched launch's waves per block and i-cluster placement (resolved: 0 or 1). */
+/* The batched launch's waves per block and i-cluster placement setting,
+ * resolved for its entry count (0 or 1; kernels without an LDS variant keep
+ * the data in registers whatever the setting). */
 int nbnxm_ensemble_get_launch_config(nbnxm_ensemble_t* e, int* wavesPerBlock, int* iClusterInLds);
 /* Runs numSteps evaluations: x = x_search + N(0, sigma) per atom and step
  * (sigma 0: the search positions), nonbonded forces, FP64 energies and shift
