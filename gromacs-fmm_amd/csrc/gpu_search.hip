@@ -160,15 +160,27 @@ __global__ void wrapAndKeyKernel(const double* x, int numAtomsTotal, const int* 
     vals[a] = a;
 }
 
-// Step 2: atoms per column (order-independent counts).
+// Step 2: atoms per column.  The keys are sorted by column, so a wave holds
+// a few runs of equal columns: the first lane of each run adds the run's
+// length (one atomic per run and wave instead of one per atom, which
+// serialised on the counter of each column).  Active lanes are lanes
+// 0 .. n-1 of the wave (only the last wave of the grid is partial).
 __global__ void columnCountKernel(const unsigned long long* sortedKeys, int numAtomsTotal, int* colCount)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= numAtomsTotal)
+    const int lane = threadIdx.x & 63;
+    const bool active = i < numAtomsTotal;
+    const int col = active ? static_cast<int>(sortedKeys[i] >> 32) : -1;
+    const int prevCol = __shfl_up(col, 1, 64);
+    const bool head = active && (lane == 0 || col != prevCol);
+    const unsigned long long heads = __builtin_amdgcn_ballot_w64(head);
+    const int numActive = __popcll(__builtin_amdgcn_ballot_w64(active));
+    if (head)
     {
-        return;
+        const unsigned long long later = heads & ~((2ull << lane) - 1ull);
+        const int next = later ? (__ffsll(static_cast<long long>(later)) - 1) : numActive;
+        atomicAdd(colCount + col, next - lane);
     }
-    atomicAdd(colCount + static_cast<int>(sortedKeys[i] >> 32), 1);
 }
 
 __global__ void cellsPerColumnKernel(const int* colCount, int numCols, int* colCells)
