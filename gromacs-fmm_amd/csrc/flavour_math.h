@@ -363,17 +363,17 @@ __device__ __forceinline__ Real pairForce(const NbFlavourParams& flav,
     return F_invr;
 }
 
-// Per-type grid factor sqrt(6 C6_ii) of the geometric LJ-PME rule.
+// LJ-PME grid C6 of a type pair (index numTypes * type_i + type_j).
 template<typename Real>
-__device__ __forceinline__ Real gridC6(const NbFlavourParams& flav, int type)
+__device__ __forceinline__ Real ljPmeGridC6(const NbFlavourParams& flav, int pairIndex)
 {
     if constexpr (std::is_same_v<Real, float>)
     {
-        return flav.gridC6[type];
+        return flav.ljPmeGrid[pairIndex];
     }
     else
     {
-        return flav.gridC6D[type];
+        return flav.ljPmeGridD[pairIndex];
     }
 }
 

@@ -28,6 +28,7 @@
 
 #include "capi_common.h"
 #include "gpu_search.h"
+#include "nbnxm_kernels.h"
 #include "nbnxm_amd.h"
 #include "pairsearch.h"
 
@@ -1008,7 +1009,7 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     NBNXM_HIP_CHECK(s->colCells.resize(ncol + 1));
     NBNXM_HIP_CHECK(s->colCellStart.resize(ncol + 1));
     NBNXM_HIP_CHECK(s->colAtomStart.resize(ncol + 1));
-    NBNXM_HIP_CHECK(hipMemsetAsync(s->colCount.ptr, 0, sizeof(int) * (ncol + 1), st));
+    NBNXM_HIP_CHECK(launchFillI32(s->colCount.ptr, ncol + 1, 0, st));
     hipLaunchKernelGGL(columnCountKernel, dim3((N + 255) / 256), dim3(256), 0, st, s->keysSorted.ptr, N, s->colCount.ptr);
     NBNXM_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(cellsPerColumnKernel, dim3((ncol + 1 + 255) / 256), dim3(256), 0, st, s->colCount.ptr, ncol + 1, s->colCells.ptr);
@@ -1078,7 +1079,7 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     P.maxPairsPerEntry = maxPairsPerEntry;
     NBNXM_HIP_CHECK(s->counts.resize(3 * static_cast<size_t>(numTasks)));
     NBNXM_HIP_CHECK(s->error.resize(1));
-    NBNXM_HIP_CHECK(hipMemsetAsync(s->error.ptr, 0, sizeof(int), st));
+    NBNXM_HIP_CHECK(launchFillI32(s->error.ptr, 1, 0, st));
     P.counts = s->counts.ptr;
     P.error = s->error.ptr;
     hipLaunchKernelGGL(searchKernel<false>, dim3(numTasks), dim3(64), 0, st, P);

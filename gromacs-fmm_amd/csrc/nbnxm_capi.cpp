@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -89,6 +90,10 @@ int validateFlavour(const nbnxm_interaction_const_t* ic)
     if (ic->vdwModifier < NBNXM_VDW_MOD_POTSHIFT || ic->vdwModifier > NBNXM_VDW_MOD_POTSWITCH)
     {
         return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: unknown LJ modifier");
+    }
+    if (ic->ljPme < NBNXM_LJPME_NONE || ic->ljPme > NBNXM_LJPME_LB)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "interaction constants: unknown LJ-PME grid rule");
     }
     if (ic->ljPme && ic->vdwModifier != NBNXM_VDW_MOD_POTSHIFT)
     {
@@ -189,8 +194,8 @@ struct nbnxm_gpu
     DeviceBuffer<float2> paramsB;
     DeviceBuffer<double> paramsD;
     NbFlavourParams flavour;       // used unless the flavour is the tuned default
-    DeviceBuffer<float> gridC6;    // LJ-PME: sqrt(6 C6_ii) per type
-    DeviceBuffer<double> gridC6D;
+    DeviceBuffer<float> ljPmeGrid;  // LJ-PME: grid C6 per type pair
+    DeviceBuffer<double> ljPmeGridD;
     DeviceBuffer<float> ewaldTable; // tabulated Ewald: F(r), one padding entry
     DeviceBuffer<double> ewaldTableD;
 
@@ -239,8 +244,35 @@ struct nbnxm_gpu
     bool lastComputeVirial = false;
     hipEvent_t adoptEvent = nullptr; // orders a device list's producer before this task
 
+    // gpu_get_f: every launch also writes its (accumulated) forces as floats
+    // into fOut, with the virial partials taken before (fixed-point modes).
+    bool externalForces = false;
+
+    // Event timing (gpu_get_timings, GpuTimers of the reference): start/stop
+    // events of each timed launch or transfer since the last finish, accounted
+    // into `timings` when a finish call finds the task complete.
+    struct TimingRecord
+    {
+        int kind;
+        hipEvent_t start;
+        hipEvent_t stop;
+    };
+    bool doTime = false;
+    nbnxm_gpu_timings_t timings{};
+    std::vector<hipEvent_t> eventPool;
+    std::vector<TimingRecord> pendingTimings;
+
     ~nbnxm_gpu()
     {
+        for (const TimingRecord& r : pendingTimings)
+        {
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        for (hipEvent_t e : eventPool)
+        {
+            (void)hipEventDestroy(e);
+        }
         if (hEnergies)
         {
             (void)hipHostFree(hEnergies);
@@ -273,6 +305,108 @@ hipStream_t nbnxm_amd::gpuTaskStream(const nbnxm_gpu_t* nb)
 namespace
 {
 
+// Kinds of timed work: the nonbonded kernel at c_timeNonbonded + 2*fresh +
+// energies (ktime[fresh][energies]), then the two prunes and the transfers.
+constexpr int c_timeNonbonded = 0;
+constexpr int c_timePrune = 4;
+constexpr int c_timeRollingPrune = 5;
+constexpr int c_timeXqH2d = 6;
+constexpr int c_timeD2h = 7;
+constexpr int c_timeListH2d = 8;
+
+hipError_t pooledEvent(nbnxm_gpu* nb, hipEvent_t* e)
+{
+    if (!nb->eventPool.empty())
+    {
+        *e = nb->eventPool.back();
+        nb->eventPool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(e);
+}
+
+// Opens a timed region on the task's stream (nothing when timing is off).
+int timingBegin(nbnxm_gpu* nb, int kind)
+{
+    if (!nb->doTime)
+    {
+        return NBNXM_OK;
+    }
+    nbnxm_gpu::TimingRecord r{ kind, nullptr, nullptr };
+    NBNXM_HIP_CHECK(pooledEvent(nb, &r.start));
+    NBNXM_HIP_CHECK(pooledEvent(nb, &r.stop));
+    nb->pendingTimings.push_back(r);
+    NBNXM_HIP_CHECK(hipEventRecord(r.start, nb->stream));
+    return NBNXM_OK;
+}
+
+// Closes the region timingBegin opened last.
+int timingEnd(nbnxm_gpu* nb)
+{
+    if (!nb->doTime || nb->pendingTimings.empty())
+    {
+        return NBNXM_OK;
+    }
+    NBNXM_HIP_CHECK(hipEventRecord(nb->pendingTimings.back().stop, nb->stream));
+    return NBNXM_OK;
+}
+
+// gpu_accumulate_timings / countPruneKernelTime (gpu_common.h:90-250): the
+// regions recorded since the last finish, once the stream has completed them.
+int accountTimings(nbnxm_gpu* nb)
+{
+    bool anyNonbonded = false;
+    for (const nbnxm_gpu::TimingRecord& r : nb->pendingTimings)
+    {
+        float ms = 0.0f;
+        NBNXM_HIP_CHECK(hipEventElapsedTime(&ms, r.start, r.stop));
+        nbnxm_gpu_timings_t& t = nb->timings;
+        if (r.kind < c_timePrune)
+        {
+            nbnxm_kernel_timing_t& k = t.ktime[(r.kind - c_timeNonbonded) >> 1][(r.kind - c_timeNonbonded) & 1];
+            k.t += ms;
+            k.c++;
+            anyNonbonded = true;
+        }
+        else if (r.kind == c_timePrune)
+        {
+            t.pruneTime.t += ms;
+            t.pruneTime.c++;
+        }
+        else if (r.kind == c_timeRollingPrune)
+        {
+            t.dynamicPruneTime.t += ms;
+            t.dynamicPruneTime.c++;
+        }
+        else if (r.kind == c_timeXqH2d)
+        {
+            t.nb_h2d_t += ms;
+        }
+        else if (r.kind == c_timeD2h)
+        {
+            t.nb_d2h_t += ms;
+        }
+        else
+        {
+            t.pl_h2d_t += ms;
+            t.pl_h2d_c++;
+        }
+        nb->eventPool.push_back(r.start);
+        nb->eventPool.push_back(r.stop);
+    }
+    nb->pendingTimings.clear();
+    if (anyNonbonded)
+    {
+        nb->timings.nb_c++;
+    }
+    return NBNXM_OK;
+}
+
+} // namespace
+
+namespace
+{
+
 int uploadParams(nbnxm_gpu* nb)
 {
     const float rc = static_cast<float>(nb->ic.rcoulomb);
@@ -288,6 +422,47 @@ int uploadParams(nbnxm_gpu* nb)
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsD.ptr, pd, sizeof(pd), hipMemcpyHostToDevice, nb->stream));
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     return NBNXM_OK;
+}
+
+// The interaction constants of a task: the flavour of its launches, the
+// tabulated Ewald correction when selected (init_ewald_coulomb_force_table,
+// nbnxm_gpu_data_mgmt.cpp:83-95) and the parameter blocks; the LJ-PME grid
+// tables (per type pair, from the atom data) are kept.  The stream is idle
+// when this returns.
+int setFlavour(nbnxm_gpu* nb, const nbnxm_interaction_const_t& ic)
+{
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    NbFlavourParams flav = flavourParams(ic);
+    flav.ljPmeGrid = nb->ljPmeGrid.ptr;
+    flav.ljPmeGridD = nb->ljPmeGridD.ptr;
+    if (ic.tabulatedEwald)
+    {
+        // The extra last entry repeats the end point so that r a rounding step
+        // above rc still reads inside the table.
+        double scale = 0;
+        const int np = nbnxm_ewald_correction_table(ic.ewaldcoeff_q, ic.rcoulomb, nullptr, nullptr, 0, &scale);
+        if (np < 2)
+        {
+            return np < 0 ? np : fail(NBNXM_ERR_INVALID, "gpu_init: Ewald table too short");
+        }
+        std::vector<float> t(np + 1);
+        std::vector<double> td(np + 1);
+        nbnxm_ewald_correction_table(ic.ewaldcoeff_q, ic.rcoulomb, t.data(), td.data(), np, &scale);
+        t[np] = t[np - 1];
+        td[np] = td[np - 1];
+        NBNXM_HIP_CHECK(nb->ewaldTable.resize(np + 1));
+        NBNXM_HIP_CHECK(nb->ewaldTableD.resize(np + 1));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTable.ptr, t.data(), sizeof(float) * (np + 1), hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTableD.ptr, td.data(), sizeof(double) * (np + 1), hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+        flav.ewaldTableScale = scale;
+        flav.ewaldTable = nb->ewaldTable.ptr;
+        flav.ewaldTableD = nb->ewaldTableD.ptr;
+        flav.ewaldTableSize = np + 1;
+    }
+    nb->ic = ic;
+    nb->flavour = flav;
+    return uploadParams(nb);
 }
 
 } // namespace
@@ -575,6 +750,10 @@ int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* o, nbnxm_interacti
     {
         return fail(NBNXM_ERR_INVALID, "init_interaction_const_ex: switched LJ needs 0 <= rvdw_switch < rvdw");
     }
+    if (o->ljPme < NBNXM_LJPME_NONE || o->ljPme > NBNXM_LJPME_LB)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "init_interaction_const_ex: unknown LJ-PME grid rule");
+    }
     // LJ-PME exists with the potential shift only (src/gromacs/nbnxm/simd_kernel.h:227-228).
     if (o->ljPme && o->vdwModifier != NBNXM_VDW_MOD_POTSHIFT)
     {
@@ -593,7 +772,7 @@ int nbnxm_init_interaction_const_ex(const nbnxm_ic_options_t* o, nbnxm_interacti
     c.rvdw = rvdw;
     c.eeltype = o->eeltype;
     c.vdwModifier = o->vdwModifier;
-    c.ljPme = o->ljPme ? 1 : 0;
+    c.ljPme = o->ljPme;
     c.rvdw_switch = (o->vdwModifier == NBNXM_VDW_MOD_POTSHIFT) ? 0.0 : o->rvdw_switch;
     c.epsfac = (o->epsilonR != 0) ? one4PiEps0() / o->epsilonR : 0.0;
 
@@ -843,37 +1022,10 @@ int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hEnergies), 2 * sizeof(double), hipHostMallocDefault));
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hFshift), 3 * NBNXM_NUM_SHIFTS * sizeof(double), hipHostMallocDefault));
     NBNXM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&nb->hVirial), 9 * sizeof(double), hipHostMallocDefault));
-    nb->ic = *ic;
-    nb->flavour = flavourParams(*ic);
     nb->listParams = *listParams;
-    if (ic->tabulatedEwald)
-    {
-        // init_ewald_coulomb_force_table (nbnxm_gpu_data_mgmt.cpp:83-95); the
-        // extra last entry repeats the end point so that r a rounding step
-        // above rc still reads inside the table.
-        double scale = 0;
-        const int np = nbnxm_ewald_correction_table(ic->ewaldcoeff_q, ic->rcoulomb, nullptr, nullptr, 0, &scale);
-        if (np < 2)
-        {
-            return np < 0 ? np : fail(NBNXM_ERR_INVALID, "gpu_init: Ewald table too short");
-        }
-        std::vector<float> t(np + 1);
-        std::vector<double> td(np + 1);
-        nbnxm_ewald_correction_table(ic->ewaldcoeff_q, ic->rcoulomb, t.data(), td.data(), np, &scale);
-        t[np] = t[np - 1];
-        td[np] = td[np - 1];
-        NBNXM_HIP_CHECK(nb->ewaldTable.resize(np + 1));
-        NBNXM_HIP_CHECK(nb->ewaldTableD.resize(np + 1));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTable.ptr, t.data(), sizeof(float) * (np + 1), hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ewaldTableD.ptr, td.data(), sizeof(double) * (np + 1), hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
-        nb->flavour.ewaldTableScale = scale;
-        nb->flavour.ewaldTable = nb->ewaldTable.ptr;
-        nb->flavour.ewaldTableD = nb->ewaldTableD.ptr;
-        nb->flavour.ewaldTableSize = np + 1;
-    }
     nb->accumulation = accumulation;
-    rc = uploadParams(nb.get());
+    nb->doTime = std::getenv("GMX_ENABLE_GPU_TIMING") != nullptr;
+    rc = setFlavour(nb.get(), *ic);
     if (rc != NBNXM_OK)
     {
         return rc;
@@ -918,9 +1070,25 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
         }
     }
     const int n = nbat->numAtoms;
+    if (n != nb->numAtoms)
+    {
+        // The list, the buffer-op maps and the pending forces index the old
+        // grid: until gpu_init_pairlist / gpu_init_x_to_nbat_x run again the
+        // calls that read them return NBNXM_ERR_STATE.
+        nb->haveList = false;
+        nb->haveFreshList = false;
+        nb->haveSortedOrder = false;
+        nb->numRealAtoms = -1;
+        nb->haveLaunched = false;
+    }
     nb->numAtoms = n;
     nb->vdwType = nbat->vdwType;
     nb->numTypes = nbat->numTypes;
+    rc = timingBegin(nb, c_timeListH2d);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
     NBNXM_HIP_CHECK(nb->xq.resize(n));
     NBNXM_HIP_CHECK(nb->fFixed.resize(3 * static_cast<size_t>(n)));
     NBNXM_HIP_CHECK(nb->fFloat.resize(3 * static_cast<size_t>(n)));
@@ -950,24 +1118,20 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
     }
     if (nb->ic.ljPme)
     {
-        // Geometric grid rule: sqrt of the diagonal 6 C6 (atomdata.cpp:299-309).
-        const int nt = nbat->numTypes;
-        std::vector<float> g(nt);
-        std::vector<double> gd(nt);
-        for (int t = 0; t < nt; t++)
-        {
-            const float c6 = nbat->nbfp[2 * (static_cast<size_t>(t) * nt + t)];
-            g[t] = std::sqrt(c6);
-            gd[t] = std::sqrt(static_cast<double>(c6));
-        }
-        NBNXM_HIP_CHECK(nb->gridC6.resize(nt));
-        NBNXM_HIP_CHECK(nb->gridC6D.resize(nt));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->gridC6.ptr, g.data(), sizeof(float) * nt, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->gridC6D.ptr, gd.data(), sizeof(double) * nt, hipMemcpyHostToDevice, nb->stream));
+        // Grid C6 of every type pair by the grid rule, from the nbfp diagonal
+        // (atomdata.cpp:296-330).
+        const size_t nt2 = static_cast<size_t>(nbat->numTypes) * nbat->numTypes;
+        std::vector<float> g(nt2);
+        std::vector<double> gd(nt2);
+        ljPmeGridTables(nb->ic.ljPme, nbat->numTypes, nbat->nbfp, g.data(), gd.data());
+        NBNXM_HIP_CHECK(nb->ljPmeGrid.resize(nt2));
+        NBNXM_HIP_CHECK(nb->ljPmeGridD.resize(nt2));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ljPmeGrid.ptr, g.data(), sizeof(float) * nt2, hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->ljPmeGridD.ptr, gd.data(), sizeof(double) * nt2, hipMemcpyHostToDevice, nb->stream));
         // The copies read host vectors that go out of scope: finish them here.
         NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
-        nb->flavour.gridC6 = nb->gridC6.ptr;
-        nb->flavour.gridC6D = nb->gridC6D.ptr;
+        nb->flavour.ljPmeGrid = nb->ljPmeGrid.ptr;
+        nb->flavour.ljPmeGridD = nb->ljPmeGridD.ptr;
     }
     NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
@@ -978,6 +1142,11 @@ int nbnxm_gpu_init_atomdata(nbnxm_gpu_t* nb, const nbnxm_atomdata_t* nbat)
     NBNXM_HIP_CHECK(nb->virialPartials.resize(9 * static_cast<size_t>(nb->virialBlocks)));
     NBNXM_HIP_CHECK(nb->virial.resize(9));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->slotRange.ptr, range, sizeof(range), hipMemcpyHostToDevice, nb->stream));
+    rc = timingEnd(nb);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
     // Host buffers of the caller may be reused after return.
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     return NBNXM_OK;
@@ -1015,7 +1184,13 @@ int nbnxm_gpu_init_pairlist(nbnxm_gpu_t* nb, const nbnxm_pairlist_gpu_t* list)
     {
         return fail(NBNXM_ERR_INVALID, "gpu_init_pairlist: excl[0] must be the all-interaction mask");
     }
-    return setPairlist(nb, list->sci, list->sci, list->nsci, list->cjPacked, list->ncjPacked, list->excl, list->nexcl, hipMemcpyHostToDevice);
+    rc = timingBegin(nb, c_timeListH2d);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    rc = setPairlist(nb, list->sci, list->sci, list->nsci, list->cjPacked, list->ncjPacked, list->excl, list->nexcl, hipMemcpyHostToDevice);
+    return rc != NBNXM_OK ? rc : timingEnd(nb);
 }
 
 int nbnxm_gpu_copy_xq_to_gpu(nbnxm_gpu_t* nb, const float* xq, int numAtoms)
@@ -1030,7 +1205,17 @@ int nbnxm_gpu_copy_xq_to_gpu(nbnxm_gpu_t* nb, const float* xq, int numAtoms)
     }
     if (numAtoms > 0)
     {
+        int rc = timingBegin(nb, c_timeXqH2d);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->xq.ptr, xq, sizeof(float4) * numAtoms, hipMemcpyHostToDevice, nb->stream));
+        rc = timingEnd(nb);
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
         NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     }
     return NBNXM_OK;
@@ -1075,7 +1260,17 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
         }
         NBNXM_HIP_CHECK(launchFillI32(nb->sciHistogram.ptr, NBNXM_SCI_HISTOGRAM_SIZE, 0, nb->stream));
     }
+    int trc = timingBegin(nb, fresh ? c_timePrune : c_timeRollingPrune);
+    if (trc != NBNXM_OK)
+    {
+        return trc;
+    }
     NBNXM_HIP_CHECK(launchPrune(pa, fresh, nb->stream));
+    trc = timingEnd(nb);
+    if (trc != NBNXM_OK)
+    {
+        return trc;
+    }
     if (fresh)
     {
         // Order entries by descending pruned pair count (the role of
@@ -1118,6 +1313,7 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
         return fail(NBNXM_ERR_STATE, "gpu_launch_kernel: shift vectors and pair list must be uploaded first");
     }
     // A fresh list is pruned first (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:505-513).
+    const bool freshList = nb->haveFreshList;
     if (nb->haveFreshList)
     {
         const int rc = launchPruneImpl(nb, 1);
@@ -1147,6 +1343,11 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     a.computeFshift = computeVirial ? 1 : 0;
     // The tuned list walk stages the type table in LDS up to 65 types (64 + dummy).
     const bool tableFitsLds = nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE || nb->numTypes <= c_maxLdsTableTypes;
+    const int trc = timingBegin(nb, c_timeNonbonded + 2 * (freshList ? 1 : 0) + (computeEnergy ? 1 : 0));
+    if (trc != NBNXM_OK)
+    {
+        return trc;
+    }
     if (isDefaultFlavour(nb->ic) && (tableFitsLds || nb->accumulation == NBNXM_ACCUM_FP64))
     {
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
@@ -1162,6 +1363,22 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
         // and type tables beyond the LDS staging: the standalone flavour kernel.
         NBNXM_HIP_CHECK(launchNonbondedFlavour(a, nb->flavour, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->stream));
     }
+    const int trc2 = timingEnd(nb);
+    if (trc2 != NBNXM_OK)
+    {
+        return trc2;
+    }
+    if (nb->externalForces && nb->accumulation != NBNXM_ACCUM_FLOAT)
+    {
+        // gpu_get_f consumers read float forces: the virial partials from the
+        // accumulator, then its float image (the accumulator is kept for
+        // further launches until gpu_clear_outputs, as the reference's f).
+        if (computeVirial)
+        {
+            NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, nb->fFixed.ptr, nullptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
+        }
+        NBNXM_HIP_CHECK(launchConvertForces(nb->fFixed.ptr, nb->numAtoms, nb->fOut.ptr, nb->fOutDouble.ptr, nb->stream, false));
+    }
     nb->lastComputeEnergy = computeEnergy != 0;
     nb->lastComputeVirial = computeVirial != 0;
     nb->forcesConsumed = false;
@@ -1176,6 +1393,12 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         return fail(NBNXM_ERR_INVALID, "gpu_launch_cpyback: null handle");
     }
     const size_t n3 = 3 * static_cast<size_t>(nb->numAtoms);
+    const int trc = timingBegin(nb, c_timeD2h);
+    if (trc != NBNXM_OK)
+    {
+        return trc;
+    }
+    const bool external = nb->externalForces && nb->accumulation != NBNXM_ACCUM_FLOAT;
     if (nb->forcesConsumed)
     {
         // add_nbat_f_to_f already reduced the forces and the virial partials;
@@ -1183,6 +1406,18 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         if (f || fDouble)
         {
             return fail(NBNXM_ERR_STATE, "gpu_launch_cpyback: the forces of this launch were taken by gpu_add_nbat_f_to_f");
+        }
+    }
+    else if (external)
+    {
+        // The launch already wrote the float forces (and the virial partials).
+        if (f && n3)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(f, nb->fOut.ptr, sizeof(float) * n3, hipMemcpyDeviceToHost, nb->stream));
+        }
+        if (fDouble && n3)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(fDouble, nb->fOutDouble.ptr, sizeof(double) * n3, hipMemcpyDeviceToHost, nb->stream));
         }
     }
     else
@@ -1228,7 +1463,7 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble)
         NBNXM_HIP_CHECK(launchVirialFinish(nb->virialPartials.ptr, 1, nb->virialBlocks, nb->shiftVec.ptr, nb->fshift.ptr, nb->virial.ptr, nb->stream));
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->hVirial, nb->virial.ptr, 9 * sizeof(double), hipMemcpyDeviceToHost, nb->stream));
     }
-    return NBNXM_OK;
+    return timingEnd(nb);
 }
 
 int nbnxm_gpu_init_x_to_nbat_x(nbnxm_gpu_t* nb, int numAtoms, int numGridAtoms, const int* atomIndices)
@@ -1305,6 +1540,14 @@ int nbnxm_gpu_add_nbat_f_to_f(nbnxm_gpu_t* nb, float* d_f, const float* d_fToAdd
         return fail(NBNXM_ERR_STATE, "gpu_add_nbat_f_to_f: no nonbonded launch whose forces are still pending");
     }
     const bool fixed = nb->accumulation != NBNXM_ACCUM_FLOAT;
+    if (fixed && nb->externalForces)
+    {
+        // The launch wrote float forces and the virial partials; the
+        // accumulator stays until gpu_clear_outputs.
+        NBNXM_HIP_CHECK(launchReduceForces(nullptr, nb->fOut.ptr, nb->cellIndex.ptr, nb->numRealAtoms, d_fToAdd, d_f, accumulate != 0, nb->stream));
+        nb->forcesConsumed = true;
+        return NBNXM_OK;
+    }
     if (nb->lastComputeVirial)
     {
         NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, fixed ? nb->fFixed.ptr : nullptr, fixed ? nullptr : nb->fFloat.ptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
@@ -1332,13 +1575,20 @@ int nbnxm_gpu_get_virial(nbnxm_gpu_t* nb, double* virial)
     return NBNXM_OK;
 }
 
-int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift)
+} // extern "C"
+
+namespace
 {
-    if (!nb)
+
+// The part of gpu_try_finish_task after the stream has completed
+// (gpu_common.h:296-325): timings, then the staged outputs.
+int finishTask(nbnxm_gpu* nb, double* e_lj, double* e_el, double* fshift)
+{
+    const int rc = accountTimings(nb);
+    if (rc != NBNXM_OK)
     {
-        return fail(NBNXM_ERR_INVALID, "gpu_wait_finish_task: null handle");
+        return rc;
     }
-    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
     // Accumulate into the caller's outputs like gpu_reduce_staged_outputs
     // (src/gromacs/nbnxm/gpu_common.h:138-160).
     if (nb->lastComputeEnergy)
@@ -1359,6 +1609,150 @@ int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, doub
             fshift[k] += nb->hFshift[k];
         }
     }
+    return NBNXM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_wait_finish_task: null handle");
+    }
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    return finishTask(nb, e_lj, e_el, fshift);
+}
+
+int nbnxm_gpu_try_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_try_finish_task: null handle");
+    }
+    const hipError_t q = hipStreamQuery(nb->stream);
+    if (q == hipErrorNotReady)
+    {
+        // Not an error: take it off the thread's last-error state so the next
+        // launch check does not report it.
+        (void)hipGetLastError();
+        return 0;
+    }
+    NBNXM_HIP_CHECK(q);
+    const int rc = finishTask(nb, e_lj, e_el, fshift);
+    return rc != NBNXM_OK ? rc : 1;
+}
+
+int nbnxm_gpu_pme_loadbal_update_param(nbnxm_gpu_t* nb, const nbnxm_interaction_const_t* ic, const nbnxm_pairlist_params_t* listParams)
+{
+    if (!nb || !ic)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_pme_loadbal_update_param: null pointer");
+    }
+    int rc = validateFlavour(ic);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    if (ic->ljPme != nb->ic.ljPme)
+    {
+        return fail(NBNXM_ERR_UNSUPPORTED, "gpu_pme_loadbal_update_param: the LJ-PME grid rule is fixed at gpu_init");
+    }
+    const nbnxm_pairlist_params_t lp = listParams ? *listParams : nb->listParams;
+    if (!(lp.rlistOuter >= ic->rcoulomb) || !(lp.rlistInner >= ic->rcoulomb) || lp.rlistInner > lp.rlistOuter
+        || lp.numRollingPruningParts < 1)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_pme_loadbal_update_param: need rc <= rlistInner <= rlistOuter and numRollingPruningParts >= 1");
+    }
+    // set_cutoff_parameters + nbnxn_gpu_pick_ewald_kernel_type +
+    // init_ewald_coulomb_force_table (nbnxm_gpu_data_mgmt.cpp:178-200,511-527).
+    rc = setFlavour(nb, *ic);
+    if (rc != NBNXM_OK)
+    {
+        return rc;
+    }
+    nb->listParams = lp;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_set_timing(nbnxm_gpu_t* nb, int enable)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_set_timing: null handle");
+    }
+    nb->doTime = enable != 0;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_get_timings(const nbnxm_gpu_t* nb, nbnxm_gpu_timings_t* timings)
+{
+    if (!nb || !timings)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_timings: null pointer");
+    }
+    if (!nb->doTime)
+    {
+        return 0;
+    }
+    *timings = nb->timings;
+    return 1;
+}
+
+int nbnxm_gpu_reset_timings(nbnxm_gpu_t* nb)
+{
+    if (!nb)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_reset_timings: null handle");
+    }
+    // init_timings (nbnxm_gpu_data_mgmt.cpp:253-272); regions still in flight
+    // are dropped with the old counts.
+    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    for (const nbnxm_gpu::TimingRecord& r : nb->pendingTimings)
+    {
+        nb->eventPool.push_back(r.start);
+        nb->eventPool.push_back(r.stop);
+    }
+    nb->pendingTimings.clear();
+    nb->timings = nbnxm_gpu_timings_t{};
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_get_f(nbnxm_gpu_t* nb, float** d_f)
+{
+    if (!nb || !d_f)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_f: null pointer");
+    }
+    if (nb->accumulation == NBNXM_ACCUM_FLOAT)
+    {
+        *d_f = nb->fFloat.ptr;
+        return NBNXM_OK;
+    }
+    if (!nb->externalForces)
+    {
+        // Forces of a launch made before the switch are still only in the
+        // accumulator: take its virial partials and give it its float image now.
+        if (nb->haveLaunched && !nb->forcesConsumed && nb->lastComputeVirial)
+        {
+            NBNXM_HIP_CHECK(launchVirialPartials(nb->xq.ptr, nb->fFixed.ptr, nullptr, nb->slotRange.ptr, 1, nb->virialBlocks, nb->virialPartials.ptr, nb->stream));
+        }
+        NBNXM_HIP_CHECK(launchConvertForces(nb->fFixed.ptr, nb->numAtoms, nb->fOut.ptr, nb->fOutDouble.ptr, nb->stream, false));
+        nb->externalForces = true;
+    }
+    *d_f = nb->fOut.ptr;
+    return NBNXM_OK;
+}
+
+int nbnxm_gpu_get_stream(nbnxm_gpu_t* nb, void** stream)
+{
+    if (!nb || !stream)
+    {
+        return fail(NBNXM_ERR_INVALID, "gpu_get_stream: null pointer");
+    }
+    *stream = nb->stream;
     return NBNXM_OK;
 }
 

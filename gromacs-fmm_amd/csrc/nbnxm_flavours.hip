@@ -2,7 +2,7 @@
 // (Ewald analytical + LJ cut-off with potential shift, rvdw = rcoulomb):
 // tabulated Ewald, reaction field / plain cut-off Coulomb, the twin cut-off LJ
 // check, LJ force switch, LJ potential switch and the LJ-PME grid correction
-// with the geometric grid rule.  These are the kernels the reference generates from one
+// with the geometric or Lorentz-Berthelot grid rule.  These are the kernels the reference generates from one
 // body with preprocessor flavours (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel.cuh,
 // selected in nbnxm_cuda.cu:171-254); here one body takes the flavour as
 // wave-uniform kernel arguments, so a flavour costs scalar branches, not a
@@ -18,7 +18,9 @@
 #include "flavour_math.h"
 #include "nbnxm_kernels.h"
 
+#include <cmath>
 #include <type_traits>
+#include <vector>
 
 namespace nbnxm_amd
 {
@@ -161,7 +163,6 @@ __global__ __launch_bounds__(256) void nbnxmFlavourKernel(const NbKernelArgs a, 
             const int typej = needTypes ? a.atomTypes[aj] : 0;
             const float2 combj = useTable ? make_float2(0.f, 0.f) : a.ljComb[aj];
             const Real qj = static_cast<Real>(xqj.w);
-            const Real c6gridJ = flav.ljPme ? gridC6<Real>(flav, typej) : Real(0);
             Real fcj[3] = { Real(0), Real(0), Real(0) };
             for (int c = 0; c < c_ncps; c++)
             {
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void nbnxmFlavourKernel(const NbKernelArgs a, 
                         c6 = epsilon * sigma6;
                         c12 = c6 * sigma6;
                     }
-                    const Real c6grid = flav.ljPme ? gridC6<Real>(flav, typei[c]) * c6gridJ : Real(0);
+                    const Real c6grid = flav.ljPme ? ljPmeGridC6<Real>(flav, a.numTypes * typei[c] + typej) : Real(0);
                     const Real F_invr = pairForce<Real, computeEnergy>(flav, k, r2, int_bit, qi[c] * qj, c6, c12, c6grid, eLJ, eEl);
                     fcj[0] -= dx * F_invr;
                     fcj[1] -= dy * F_invr;
@@ -294,6 +295,57 @@ hipError_t launchFlavourAcc(const NbKernelArgs& a, const NbFlavourParams& flav, 
 
 } // namespace
 
+void ljPmeGridTables(int rule, int numTypes, const float* nbfp, float* grid, double* gridD)
+{
+    const int nt = numTypes;
+    // nbfp_comb per type (atomdata.cpp:299-330): geometric sqrt(6C6_ii);
+    // Lorentz-Berthelot 0.5 sixthroot(12C12/6C6) and sqrt(6C6^2/12C12), zero
+    // for a type without LJ; in the reference's real (FP32) arithmetic.
+    std::vector<float> a(nt), b(nt);
+    std::vector<double> ad(nt), bd(nt);
+    for (int t = 0; t < nt; t++)
+    {
+        const float c6 = nbfp[2 * (static_cast<size_t>(t) * nt + t)];
+        const float c12 = nbfp[2 * (static_cast<size_t>(t) * nt + t) + 1];
+        if (rule == NBNXM_LJPME_LB)
+        {
+            const bool lj = c6 > 0 && c12 > 0;
+            a[t] = lj ? static_cast<float>(0.5 * std::sqrt(std::cbrt(c12 / c6))) : 0.0f;
+            b[t] = lj ? std::sqrt(c6 * c6 / c12) : 0.0f;
+            ad[t] = lj ? 0.5 * std::sqrt(std::cbrt(static_cast<double>(c12) / c6)) : 0.0;
+            bd[t] = lj ? std::sqrt(static_cast<double>(c6) * c6 / c12) : 0.0;
+        }
+        else
+        {
+            a[t] = std::sqrt(c6);
+            ad[t] = std::sqrt(static_cast<double>(c6));
+        }
+    }
+    for (int i = 0; i < nt; i++)
+    {
+        for (int j = 0; j < nt; j++)
+        {
+            const size_t k = static_cast<size_t>(i) * nt + j;
+            if (rule == NBNXM_LJPME_LB)
+            {
+                // sigma = s_i + s_j, epsilon = e_i e_j, c6grid = epsilon sigma2^3 (:317-324).
+                const float sigma = a[i] + a[j];
+                const float epsilon = b[i] * b[j];
+                const float sigma2 = sigma * sigma;
+                grid[k] = epsilon * sigma2 * sigma2 * sigma2;
+                const double sd = ad[i] + ad[j];
+                const double s2 = sd * sd;
+                gridD[k] = bd[i] * bd[j] * s2 * s2 * s2;
+            }
+            else
+            {
+                grid[k] = a[i] * a[j];
+                gridD[k] = ad[i] * ad[j];
+            }
+        }
+    }
+}
+
 NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
 {
     NbFlavourParams p;
@@ -359,7 +411,7 @@ NbFlavourParams flavourParams(const nbnxm_interaction_const_t& ic)
 
 hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, hipStream_t stream)
 {
-    if (vdwType < NBNXM_VDW_CUT_TYPE_TABLE || vdwType > NBNXM_VDW_CUT_COMB_LB || (flav.ljPme && (!flav.gridC6 || !flav.gridC6D || !a.atomTypes || !a.nbfp))
+    if (vdwType < NBNXM_VDW_CUT_TYPE_TABLE || vdwType > NBNXM_VDW_CUT_COMB_LB || (flav.ljPme && (!flav.ljPmeGrid || !flav.ljPmeGridD || !a.atomTypes || !a.nbfp))
         || (flav.tabulatedEwald && (!flav.ewaldTable || !flav.ewaldTableD || !(flav.ewaldTableScale > 0))))
     {
         return hipErrorInvalidValue;

@@ -193,10 +193,13 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     // vector load's vmcnt wait would.
     // The tabulated Ewald class stages its FP32 correction table after it, so
     // that the two table reads per pair are LDS reads too.
+    // LJ-PME (flavour classes with the type table) stages the grid C6 of every
+    // type pair after it, read per pair with the type-table index.
     constexpr bool ldsEwaldTable = FlavourTraits<kFlav>::table;
     HIP_DYNAMIC_SHARED(float4, sDyn);
     [[maybe_unused]] float2* sNbfp = reinterpret_cast<float2*>(sDyn);
     [[maybe_unused]] float* sEwald = reinterpret_cast<float*>(reinterpret_cast<char*>(sDyn) + typeTableLdsBytes(vdwType == NBNXM_VDW_CUT_TYPE_TABLE ? a.numTypes : 0));
+    [[maybe_unused]] float* sGrid = sEwald;
     if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE || ldsEwaldTable)
     {
         if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
@@ -205,6 +208,16 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
             for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
             {
                 sNbfp[i] = a.nbfp[i];
+            }
+            if constexpr (generic && !ldsEwaldTable)
+            {
+                if (flavLjPme<kFlav>(flav))
+                {
+                    for (int i = threadIdx.x; i < numEntries; i += blockDim.x)
+                    {
+                        sGrid[i] = flav.ljPmeGrid[i];
+                    }
+                }
             }
         }
         if constexpr (ldsEwaldTable)
@@ -277,17 +290,6 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     {
         const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
         stageLi[lane] = loadF2(asConst(a.ljComb), ai);
-    }
-
-    // LJ-PME grid factors of the i-atoms (flavour classes, with the type table).
-    [[maybe_unused]] float gci[c_ncps];
-    if constexpr (generic)
-    {
-#pragma unroll
-        for (int c = 0; c < c_ncps; c++)
-        {
-            gci[c] = flavLjPme<kFlav>(flav) ? flav.gridC6[a.atomTypes[(s.sci * c_ncps + c) * c_cl + ti]] : 0.0f;
-        }
     }
 
     float fciX[c_ncps], fciY[c_ncps], fciZ[c_ncps];
@@ -416,11 +418,6 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
             const float2 ljStaged = stageLj[jm * c_cl + tj];
             const int typej = __float_as_int(ljStaged.x);
             const float2 ljcj = ljStaged;
-            [[maybe_unused]] float gcj = 0.0f;
-            if constexpr (generic && vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
-            {
-                gcj = flavLjPme<kFlav>(flav) ? flav.gridC6[typej] : 0.0f;
-            }
             float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
             unsigned long long anyInRange = 0ull;
 #pragma unroll
@@ -444,15 +441,15 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 const float dy = iv.y - xqj.y;
                 const float dz = iv.z - xqj.z;
                 float r2 = norm2Fixed(dx, dy, dz);
-                // Cluster pairs on the diagonal of the central shift always carry
-                // the Newton mask in their exclusion words, so they take the
-                // interaction-bit path, the only one that needs the j > i test.
+                // On the diagonal of the central shift only pairs with j > i
+                // interact (nonSelfInteraction, :423,508), whatever the list's
+                // exclusion words say: lanes with tj <= ti leave the cut-off.
                 // Wave-uniform: exclHere from the group's OR of its exclusion
-                // words, diag from list indices.
+                // words, the diagonal test from list indices.
                 const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
-                if (exclHere && centralShift && (firstCluster + c == cj))
+                if (centralShift && (firstCluster + c == cj))
                 {
-                    r2 = upperLane ? r2 : 1.0e30f; // lanes with tj <= ti leave the cut-off
+                    r2 = upperLane ? r2 : 1.0e30f;
                 }
                 const bool inRange = r2 < rcoulomb_sq;
                 anyInRange |= __builtin_amdgcn_ballot_w64(inRange);
@@ -493,7 +490,12 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     if constexpr (generic)
                     {
                         const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
-                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, gci[c] * gcj, eLJg, eElG,
+                        float c6grid = 0.0f;
+                        if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE && !ldsEwaldTable)
+                        {
+                            c6grid = flavLjPme<kFlav>(flav) ? sGrid[tOffI[c] + typej] : 0.0f;
+                        }
+                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, c6grid, eLJg, eElG,
                                                                         ldsEwaldTable ? sEwald : nullptr, computeEnergy && !(hasExcl && exclHere));
                     }
                     else if (hasExcl && exclHere)
@@ -782,11 +784,14 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
     // The type table kernel stages the whole table in LDS.
     const size_t typeBytes = (V == NBNXM_VDW_CUT_TYPE_TABLE) ? sizeof(float2) * a.numTypes * a.numTypes : 0;
     const size_t ewaldBytes = FlavourTraits<kFlav>::table ? sizeof(float) * flav.ewaldTableSize : 0;
-    if (typeBytes > c_maxLdsTypeTableBytes || ewaldBytes > c_maxLdsEwaldTableBytes)
+    // The LJ-PME grid table follows the type table (never with the Ewald table).
+    const bool gridInLds = kFlav != c_flavTuned && !FlavourTraits<kFlav>::table && V == NBNXM_VDW_CUT_TYPE_TABLE && flav.ljPme;
+    const size_t gridBytes = gridInLds ? sizeof(float) * a.numTypes * a.numTypes : 0;
+    if (typeBytes > c_maxLdsTypeTableBytes || ewaldBytes > c_maxLdsEwaldTableBytes || (gridInLds && !flav.ljPmeGrid))
     {
         return hipErrorInvalidValue;
     }
-    const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes;
+    const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes + gridBytes;
     hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
     return hipGetLastError();
 }

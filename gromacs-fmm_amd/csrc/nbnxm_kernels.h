@@ -90,20 +90,22 @@ enum FlavourClass : int
 };
 
 // Flavour of a non-tuned launch: the interaction constants of one launch
-// (nbnxm_interaction_const_t, wave-uniform), plus the per-type LJ-PME grid
-// factors sqrt(6 C6_ii) on the device in FP32 and FP64.
+// (nbnxm_interaction_const_t, wave-uniform), plus the LJ-PME grid C6 of every
+// type pair (numTypes^2, row-major by the i type) on the device in FP32 and
+// FP64, formed on the host by the grid rule exactly as the reference kernels
+// form it per pair (ljPmeGridTables).
 struct NbFlavourParams
 {
     int eeltype = NBNXM_ELEC_EWALD;
     int vdwModifier = NBNXM_VDW_MOD_POTSHIFT;
-    int ljPme = 0;
+    int ljPme = 0; // NBNXM_LJPME_*
     int vdwCutoffCheck = 0; // rvdw < rcoulomb
     double rcoulomb = 0, rvdw = 0, epsfac = 0, beta = 0, sh_ewald = 0, k_rf = 0, c_rf = 0;
     double cpot_rep = 0, cpot_disp = 0, rvdw_switch = 0;
     double disp_c2 = 0, disp_c3 = 0, rep_c2 = 0, rep_c3 = 0, sw_c3 = 0, sw_c4 = 0, sw_c5 = 0;
     double ewaldcoeff_lj = 0, sh_lj_ewald = 0;
-    const float* gridC6 = nullptr;
-    const double* gridC6D = nullptr;
+    const float* ljPmeGrid = nullptr;
+    const double* ljPmeGridD = nullptr;
     // Tabulated Ewald correction: F(r) table with one padding entry.
     int tabulatedEwald = 0;
     double ewaldTableScale = 0;
@@ -157,6 +159,16 @@ hipError_t launchNonbondedFlavour(const NbKernelArgs& a, const NbFlavourParams& 
 // Types (with the dummy type) of the largest LJ table the tuned kernel stages in LDS.
 constexpr int c_maxLdsTableTypes = 65;
 
+// LJ-PME grid C6 per type pair (numTypes^2, row-major) from the nbfp diagonal
+// (6 C6, 12 C12): the per-type combination parameters of nbfp_comb
+// (src/gromacs/nbnxm/atomdata.cpp:296-330) combined as the reference kernels
+// combine them per pair, geometric sqrt(6C6_ii) sqrt(6C6_jj)
+// (calculate_lj_ewald_c6grid) or Lorentz-Berthelot eps (s_i + s_j)^6
+// (calculate_lj_ewald_comb_LB_F_E, cuda/nbnxm_cuda_kernel_utils.cuh:290-337),
+// in FP32 in the reference's operation order; the FP64 table does the same in
+// double from the FP32 nbfp values.
+void ljPmeGridTables(int rule, int numTypes, const float* nbfp, float* grid, double* gridD);
+
 
 // flav == nullptr: the tuned flavour (per-system parameters from sysParams*);
 // otherwise the FP32 instantiation of flav's flavour class (single parameter
@@ -188,9 +200,9 @@ struct PruneArgs
 
 hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t stream);
 
-// Converts the fixed-point forces to float (optionally double) and clears the
-// accumulator for the next evaluation.
-hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream);
+// Converts the fixed-point forces to float (optionally double) and, with
+// clear, clears the accumulator for the next evaluation.
+hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream, bool clear = true);
 // Converts the fixed-point forces of slots [first, first+count).
 hipError_t launchConvertForcesRange(unsigned long long* fFixed, int first, int count, float* fOut, double* fOutDouble, hipStream_t stream);
 

@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
     }
 }
 
-__global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* fOut, double* fOutDouble)
+__global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* fOut, double* fOutDouble, int clear)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n3)
@@ -335,7 +335,10 @@ __global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* f
     {
         fOutDouble[i] = d;
     }
-    fFixed[i] = 0ull;
+    if (clear)
+    {
+        fFixed[i] = 0ull;
+    }
 }
 
 // One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
@@ -726,8 +729,8 @@ hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType,
     }
     if (flav)
     {
-        // The flavour-class instantiations read LJ-PME grid factors by type: type table only.
-        if (accumulation == NBNXM_ACCUM_FP64 || (flav->ljPme && (vdwType != NBNXM_VDW_CUT_TYPE_TABLE || !flav->gridC6))
+        // The flavour-class instantiations read the LJ-PME grid C6 by type pair: type table only.
+        if (accumulation == NBNXM_ACCUM_FP64 || (flav->ljPme && (vdwType != NBNXM_VDW_CUT_TYPE_TABLE || !flav->ljPmeGrid))
             || (flav->tabulatedEwald && (!flav->ewaldTable || !(flav->ewaldTableScale > 0))))
         {
             return hipErrorInvalidValue;
@@ -761,14 +764,14 @@ hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t strea
     return hipGetLastError();
 }
 
-hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream)
+hipError_t launchConvertForces(unsigned long long* fFixed, int numAtoms, float* fOut, double* fOutDouble, hipStream_t stream, bool clear)
 {
     const int n3 = 3 * numAtoms;
     if (n3 == 0)
     {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(convertForcesKernel, dim3((n3 + 255) / 256), dim3(256), 0, stream, fFixed, n3, fOut, fOutDouble);
+    hipLaunchKernelGGL(convertForcesKernel, dim3((n3 + 255) / 256), dim3(256), 0, stream, fFixed, n3, fOut, fOutDouble, clear ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -69,6 +69,9 @@ class InteractionConst(ctypes.Structure):
 
 ELEC_EWALD = 0
 ELEC_RF = 1
+LJPME_NONE = 0
+LJPME_GEOM = 1
+LJPME_LB = 2
 VDW_MOD_POTSHIFT = 0
 VDW_MOD_FORCESWITCH = 1
 VDW_MOD_POTSWITCH = 2
@@ -97,6 +100,17 @@ class Atomdata(ctypes.Structure):
                 ("numTypes", ctypes.c_int), ("type", ctypes.c_void_p),
                 ("nbfp", ctypes.c_void_p), ("vdwType", ctypes.c_int),
                 ("ljComb", ctypes.c_void_p), ("shiftVec", ctypes.c_void_p)]
+
+
+class KernelTiming(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("c", ctypes.c_int)]
+
+
+class GpuTimings(ctypes.Structure):
+    _fields_ = [("ktime", (KernelTiming * 2) * 2), ("pruneTime", KernelTiming),
+                ("dynamicPruneTime", KernelTiming), ("nb_h2d_t", ctypes.c_double),
+                ("nb_d2h_t", ctypes.c_double), ("nb_c", ctypes.c_int),
+                ("pl_h2d_t", ctypes.c_double), ("pl_h2d_c", ctypes.c_int)]
 
 
 # numpy dtypes with the same layout as the C structs
@@ -146,6 +160,13 @@ SIGNATURES = [
     ("nbnxm_gpu_launch_kernel_pruneonly", _c_int, [_vp, _c_int]),
     ("nbnxm_gpu_launch_cpyback", _c_int, [_vp, _vp, _vp]),
     ("nbnxm_gpu_wait_finish_task", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_try_finish_task", _c_int, [_vp, _vp, _vp, _vp]),
+    ("nbnxm_gpu_pme_loadbal_update_param", _c_int, [_vp, _PTR(InteractionConst), _PTR(PairlistParams)]),
+    ("nbnxm_gpu_set_timing", _c_int, [_vp, _c_int]),
+    ("nbnxm_gpu_get_timings", _c_int, [_vp, _PTR(GpuTimings)]),
+    ("nbnxm_gpu_reset_timings", _c_int, [_vp]),
+    ("nbnxm_gpu_get_f", _c_int, [_vp, _PTR(_vp)]),
+    ("nbnxm_gpu_get_stream", _c_int, [_vp, _PTR(_vp)]),
     ("nbnxm_gpu_get_virial", _c_int, [_vp, _vp]),
     ("nbnxm_gpu_init_x_to_nbat_x", _c_int, [_vp, _c_int, _c_int, _vp]),
     ("nbnxm_gpu_x_to_nbat_x", _c_int, [_vp, _vp]),

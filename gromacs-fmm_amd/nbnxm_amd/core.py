@@ -27,14 +27,16 @@ def interaction_const_ex(rcoulomb, coulomb="ewald", vdw_modifier="potshift", lj_
                          rvdw_switch=0.0, ewald_rtol=1e-5, ewald_rtol_lj=1e-3, epsilon_r=1.0, epsilon_rf=0.0,
                          plain_cutoff=False, tabulated=False):
     """Any kernel flavour: coulomb "ewald" | "rf"; vdw_modifier "potshift" |
-    "forceswitch" | "potswitch"; lj_pme adds the geometric LJ-PME grid
-    correction; rvdw < rcoulomb gives the twin cut-off; tabulated selects the
-    tabulated Ewald correction."""
+    "forceswitch" | "potswitch"; lj_pme adds the LJ-PME grid correction with
+    the geometric (True or "geom") or Lorentz-Berthelot ("lb") grid rule;
+    rvdw < rcoulomb gives the twin cut-off; tabulated selects the tabulated
+    Ewald correction."""
     o = _lib.IcOptions()
     o.eeltype = {"ewald": _lib.ELEC_EWALD, "rf": _lib.ELEC_RF}[coulomb]
     o.vdwModifier = {"potshift": _lib.VDW_MOD_POTSHIFT, "forceswitch": _lib.VDW_MOD_FORCESWITCH,
                      "potswitch": _lib.VDW_MOD_POTSWITCH}[vdw_modifier]
-    o.ljPme = int(bool(lj_pme))
+    o.ljPme = {False: _lib.LJPME_NONE, True: _lib.LJPME_GEOM, "geom": _lib.LJPME_GEOM,
+               "lb": _lib.LJPME_LB}[lj_pme]
     o.rcoulomb = rcoulomb
     o.rvdw = rvdw
     o.rvdw_switch = rvdw_switch
@@ -340,6 +342,63 @@ class GpuTask:
         check(self._lib.nbnxm_gpu_add_nbat_f_to_f(self._h, ctypes.c_void_p(d_f),
                                                   None if d_f_to_add is None else ctypes.c_void_p(d_f_to_add),
                                                   int(accumulate)), "gpu_add_nbat_f_to_f")
+
+    def try_finish(self):
+        """gpu_try_finish_task in Check mode: None while the task runs, else
+        (e_lj, e_el, fshift) added as wait_finish_task adds them."""
+        elj = ctypes.c_double(0.0)
+        eel = ctypes.c_double(0.0)
+        fshift = np.zeros(3 * NUM_SHIFTS, dtype=np.float64)
+        rc = self._lib.nbnxm_gpu_try_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift))
+        if rc < 0:
+            check(rc, "gpu_try_finish_task")
+        return None if rc == 0 else (elj.value, eel.value, fshift)
+
+    def wait(self):
+        """gpu_wait_finish_task: (e_lj, e_el, fshift) of the last launch."""
+        elj = ctypes.c_double(0.0)
+        eel = ctypes.c_double(0.0)
+        fshift = np.zeros(3 * NUM_SHIFTS, dtype=np.float64)
+        check(self._lib.nbnxm_gpu_wait_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift)), "gpu_wait_finish_task")
+        return elj.value, eel.value, fshift
+
+    def pme_loadbal_update_param(self, ic, rlist_outer=None, rlist_inner=None, rolling_parts=1, dynamic_pruning=True):
+        """gpu_pme_loadbal_update_param: new interaction constants (and list radii)."""
+        lp = None
+        if rlist_outer is not None:
+            lp = PairlistParams()
+            lp.rlistOuter = rlist_outer
+            lp.rlistInner = rlist_outer if rlist_inner is None else rlist_inner
+            lp.useDynamicPruning = 1 if dynamic_pruning else 0
+            lp.numRollingPruningParts = rolling_parts
+        check(self._lib.nbnxm_gpu_pme_loadbal_update_param(self._h, ctypes.byref(ic), None if lp is None else ctypes.byref(lp)),
+              "gpu_pme_loadbal_update_param")
+
+    def set_timing(self, enable=True):
+        check(self._lib.nbnxm_gpu_set_timing(self._h, int(enable)), "gpu_set_timing")
+
+    def timings(self):
+        """gpu_get_timings: the GpuTimings struct, or None when timing is off."""
+        t = _lib.GpuTimings()
+        rc = self._lib.nbnxm_gpu_get_timings(self._h, ctypes.byref(t))
+        if rc < 0:
+            check(rc, "gpu_get_timings")
+        return t if rc == 1 else None
+
+    def reset_timings(self):
+        check(self._lib.nbnxm_gpu_reset_timings(self._h), "gpu_reset_timings")
+
+    def get_f(self):
+        """gpu_get_f: device address (int) of the nbat float force buffer."""
+        p = ctypes.c_void_p()
+        check(self._lib.nbnxm_gpu_get_f(self._h, ctypes.byref(p)), "gpu_get_f")
+        return p.value
+
+    def stream(self):
+        """The task's hipStream_t as an int."""
+        p = ctypes.c_void_p()
+        check(self._lib.nbnxm_gpu_get_stream(self._h, ctypes.byref(p)), "gpu_get_stream")
+        return p.value
 
     def virial(self):
         """Single-sum virial (3, 3) of the last launch made with compute_virial."""

@@ -113,6 +113,14 @@ enum nbnxm_elec_type
     NBNXM_ELEC_RF = 1     /* reaction field; with k_rf = 0 the plain cut-off */
 };
 
+/* LJ-PME grid combination rule (interaction_const_t::ljpme_comb_rule,
+ * LongRangeVdW; the GPU kernels VdwType::EwaldGeom / EwaldLB,
+ * src/gromacs/nbnxm/nbnxm_gpu_data_mgmt.cpp:327-346). */
+#define NBNXM_LJPME_NONE 0
+#define NBNXM_LJPME_GEOM 1 /* grid C6 = sqrt(6C6_ii 6C6_jj) */
+#define NBNXM_LJPME_LB 2   /* grid C6 from per-type sigma and epsilon, Lorentz-Berthelot
+                            * (calculate_lj_ewald_comb_LB_F_E, cuda/nbnxm_cuda_kernel_utils.cuh:290-337) */
+
 /* LJ interaction modifier (InteractionModifiers of the kernel). */
 enum nbnxm_vdw_modifier
 {
@@ -135,7 +143,7 @@ typedef struct nbnxm_interaction_const
     /* Kernel flavour; nbnxm_init_interaction_const sets Ewald, PotShift, no LJ-PME. */
     int eeltype;              /* enum nbnxm_elec_type */
     int vdwModifier;          /* enum nbnxm_vdw_modifier */
-    int ljPme;                /* nonzero: LJ-PME grid correction, geometric grid combination */
+    int ljPme;                /* NBNXM_LJPME_*: LJ-PME grid correction and its grid combination rule */
     double rvdw_switch;
     double ewaldcoeff_lj;     /* calc_ewaldcoeff_lj, src/gromacs/ewald/ewald_utils.cpp:84-112 */
     double sh_lj_ewald;       /* LJ-PME potential shift (interaction_const.cpp:140-143) */
@@ -155,7 +163,7 @@ typedef struct nbnxm_ic_options
 {
     int eeltype;        /* enum nbnxm_elec_type */
     int vdwModifier;    /* enum nbnxm_vdw_modifier */
-    int ljPme;          /* LJ-PME with the geometric grid rule */
+    int ljPme;          /* NBNXM_LJPME_*: LJ-PME with the geometric or Lorentz-Berthelot grid rule */
     double rcoulomb;
     double rvdw;        /* 0 -> rcoulomb */
     double rvdw_switch; /* ForceSwitch / PotSwitch */
@@ -272,7 +280,11 @@ typedef struct nbnxm_gpu nbnxm_gpu_t;
  * runs the tuned kernels; every other flavour (reaction field, plain cut-off,
  * twin cut-off, force / potential switch, LJ-PME) runs the generic flavour
  * kernel (nbnxm_cuda.cu:171-254 selects among the same set).  LJ-PME needs
- * nbat->type and nbat->nbfp for every vdwType (grid C6 = sqrt(6 C6_ii 6 C6_jj)). */
+ * nbat->type and nbat->nbfp for every vdwType: the grid C6 of each type pair
+ * comes from the nbfp diagonal by ic->ljPme's rule (atomdata.cpp:296-330).
+ * Timing (gpu_get_timings) starts enabled when the environment sets
+ * GMX_ENABLE_GPU_TIMING, as decideGpuTimingsUsage does for CUDA
+ * (src/gromacs/gpu_utils/gpu_utils.cpp:66-76). */
 int nbnxm_gpu_init(const nbnxm_interaction_const_t* ic,
                    const nbnxm_pairlist_params_t* listParams,
                    const nbnxm_atomdata_t* nbat,
@@ -300,6 +312,73 @@ int nbnxm_gpu_launch_cpyback(nbnxm_gpu_t* nb, float* f, double* fDouble);
 /* gpu_wait_finish_task (src/gromacs/nbnxm/gpu_common.h:345): waits and adds the
  * energies and shift forces (3*45) of the last launch to the outputs. */
 int nbnxm_gpu_wait_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift);
+/* gpu_try_finish_task with GpuTaskCompletion::Check
+ * (src/gromacs/nbnxm/nbnxm_gpu.h:199, gpu_common.h:253-333): returns 0 at
+ * once while work queued on the task's stream is still running, otherwise
+ * does what nbnxm_gpu_wait_finish_task does (energies and shift forces added
+ * to the outputs, timings accounted) and returns 1; negative on error. */
+int nbnxm_gpu_try_finish_task(nbnxm_gpu_t* nb, double* e_lj, double* e_el, double* fshift);
+/* gpu_pme_loadbal_update_param (src/gromacs/nbnxm/gpu_data_mgmt.h:92,
+ * nbnxm_gpu_data_mgmt.cpp:511-527), called by PME load balancing
+ * (src/gromacs/ewald/pme_load_balancing.cpp:856): new cut-off, Ewald
+ * coefficient, potential shifts and the flavour they select (twin cut-off
+ * when rcoulomb grows past rvdw; analytical or tabulated Ewald with the
+ * correction table regenerated), keeping the list and the atom data.
+ * listParams (may be NULL: unchanged) are the new list radii, as
+ * set_cutoff_parameters reads them from the pair-list sets; the caller builds
+ * the next list at the new radius.  The LJ-PME rule cannot change here. */
+int nbnxm_gpu_pme_loadbal_update_param(nbnxm_gpu_t* nb, const nbnxm_interaction_const_t* ic, const nbnxm_pairlist_params_t* listParams);
+
+/* gmx_kernel_timing_data_t (src/gromacs/timing/include/gromacs/timing/gpu_timing.h:48-52). */
+typedef struct nbnxm_kernel_timing
+{
+    double t; /* accumulated time, ms */
+    int c;    /* number of calls */
+} nbnxm_kernel_timing_t;
+
+/* gmx_wallclock_gpu_nbnxn_t (gpu_timing.h:80-94), filled from HIP events on
+ * the task's stream and accounted when a finish call completes the task. */
+typedef struct nbnxm_gpu_timings
+{
+    nbnxm_kernel_timing_t ktime[2][2];    /* [fresh list][energies]: nonbonded launches */
+    nbnxm_kernel_timing_t pruneTime;      /* first-pass prune of a fresh list */
+    nbnxm_kernel_timing_t dynamicPruneTime; /* rolling prune */
+    double nb_h2d_t;                      /* xq uploads */
+    double nb_d2h_t;                      /* force, energy and shift-force downloads */
+    int nb_c;                             /* finished tasks with a nonbonded launch */
+    double pl_h2d_t;                      /* pair-list and atom-data uploads */
+    int pl_h2d_c;
+} nbnxm_gpu_timings_t;
+
+/* Turns event timing of the task's launches and transfers on (1) or off (0);
+ * no counterpart (the reference decides once from the environment). */
+int nbnxm_gpu_set_timing(nbnxm_gpu_t* nb, int enable);
+/* gpu_get_timings (gpu_data_mgmt.h:109, nbnxm_gpu_data_mgmt.cpp:809-812):
+ * returns 1 and copies the accumulated timings when timing is on, 0 (and
+ * leaves *timings alone) when it is off, as the reference returns null. */
+int nbnxm_gpu_get_timings(const nbnxm_gpu_t* nb, nbnxm_gpu_timings_t* timings);
+/* gpu_reset_timings (gpu_data_mgmt.h:114, nbnxm_gpu_data_mgmt.cpp:815-821). */
+int nbnxm_gpu_reset_timings(nbnxm_gpu_t* nb);
+/* gpu_get_f (gpu_data_mgmt.h:133): the device nbat force buffer, 3 floats per
+ * grid slot, that GpuForceReduction registers (registerNbnxmForce,
+ * src/gromacs/mdlib/sim_util.cpp:1167,1248) and reduces into the atom-order
+ * force buffer through the grid's cell indices.  With fixed-point
+ * accumulation the integer accumulator cannot be registered, so once this is
+ * called every nbnxm_gpu_launch_kernel ends by writing the accumulated forces
+ * there as floats (converted exactly from fixed point; the virial partials
+ * are taken before) on the task's stream; with FLOAT accumulation it is the
+ * accumulator itself.  Either way the forces then accumulate over launches
+ * until nbnxm_gpu_clear_outputs, as the reference's nbat force buffer does
+ * (gpu_clear_outputs, nbnxm_gpu_data_mgmt.cpp:792); nbnxm_gpu_launch_cpyback
+ * and nbnxm_gpu_add_nbat_f_to_f read the same buffer and no longer clear it.
+ * A consumer orders its reads after the launch on the task's stream
+ * (nbnxm_gpu_get_stream).  The address is valid until the next
+ * nbnxm_gpu_init_atomdata, after which the reference registers it again
+ * (sim_util.cpp:1155-1167 at every search step). */
+int nbnxm_gpu_get_f(nbnxm_gpu_t* nb, float** d_f);
+/* The task's HIP stream (the reference's local nonbonded DeviceStream,
+ * nbnxm_cuda_types.h:106), as a hipStream_t. */
+int nbnxm_gpu_get_stream(nbnxm_gpu_t* nb, void** stream);
 /* Adds the single-sum virial (9, row-major) of the last launch made with
  * computeVirial to virial, after waiting.  Replaces the host calc_vir over the
  * nbat forces and shift forces (src/gromacs/mdlib/calcvir.cpp:89-133, as

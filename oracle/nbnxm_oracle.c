@@ -357,7 +357,8 @@ static float pair_f32(const flav32_t* p,
     }
     if (p->ljPme)
     {
-        /* calculate_lj_ewald_comb_geom_F[_E] (nbnxm_cuda_kernel_utils.cuh:233-288) */
+        /* calculate_lj_ewald_comb_geom_F[_E] (nbnxm_cuda_kernel_utils.cuh:233-288) and
+         * calculate_lj_ewald_comb_LB_F_E (:290-337): the same terms, c6grid by the rule */
         const float inv_r6_nm = inv_r2 * inv_r2 * inv_r2;
         const float cr2 = p->lje_coeff2 * r2;
         const float expmcr2 = expf(-cr2);
@@ -544,6 +545,10 @@ static int check_flavour(const nbnxm_atomdata_t* nbat, const nbnxm_interaction_c
     {
         return NBNXM_ERR_INVALID;
     }
+    if (ic->ljPme < NBNXM_LJPME_NONE || ic->ljPme > NBNXM_LJPME_LB)
+    {
+        return NBNXM_ERR_INVALID;
+    }
     /* LJ-PME reads the type table for the grid C6 (NULL nbat: brute force, table given). */
     if (ic->ljPme && nbat && (!nbat->type || !nbat->nbfp))
     {
@@ -552,19 +557,67 @@ static int check_flavour(const nbnxm_atomdata_t* nbat, const nbnxm_interaction_c
     return NBNXM_OK;
 }
 
-/* Per-type sqrt(6C6_ii) of the LJ-PME geometric grid rule (atomdata.cpp:299-309). */
-static float* grid_c6_f32(const nbnxm_atomdata_t* nbat)
+/* Per-type LJ-PME grid parameters, nbfp_comb of atomdata.cpp:296-330 in the
+ * reference's FP32 ("real") arithmetic, two floats per type: geometric
+ * (sqrt(6C6_ii), 0); Lorentz-Berthelot (0.5 sixthroot(12C12/6C6),
+ * sqrt(6C6^2/12C12)), with sixthroot(x) = sqrt(cbrt(x))
+ * (api/legacy/include/gromacs/math/functions.h:241-244), zero without LJ. */
+static float* grid_params_f32(int rule, const nbnxm_atomdata_t* nbat)
 {
     const int nt = nbat->numTypes;
-    float* g = (float*)malloc(sizeof(float) * (size_t)(nt > 0 ? nt : 1));
+    float* g = (float*)malloc(sizeof(float) * 2 * (size_t)(nt > 0 ? nt : 1));
     if (g)
     {
         for (int t = 0; t < nt; t++)
         {
-            g[t] = sqrtf(nbat->nbfp[2 * (t * nt + t)]);
+            const float c6 = nbat->nbfp[2 * (t * nt + t)];
+            const float c12 = nbat->nbfp[2 * (t * nt + t) + 1];
+            if (rule == NBNXM_LJPME_LB)
+            {
+                const int lj = c6 > 0.0F && c12 > 0.0F;
+                g[2 * t] = lj ? (float)(0.5 * sqrtf(cbrtf(c12 / c6))) : 0.0F;
+                g[2 * t + 1] = lj ? sqrtf(c6 * c6 / c12) : 0.0F;
+            }
+            else
+            {
+                g[2 * t] = sqrtf(c6);
+                g[2 * t + 1] = 0.0F;
+            }
         }
     }
     return g;
+}
+
+/* Grid C6 of a pair in FP32 as the CUDA kernels combine it: geometric
+ * calculate_lj_ewald_c6grid, Lorentz-Berthelot sigma = s_i + s_j,
+ * epsilon = e_i e_j, epsilon sigma2 sigma2 sigma2
+ * (nbnxm_cuda_kernel_utils.cuh:307-324). */
+static float grid_c6_pair_f32(int rule, const float* g, int ti, int tj)
+{
+    if (rule == NBNXM_LJPME_LB)
+    {
+        const float sigma = g[2 * ti] + g[2 * tj];
+        const float epsilon = g[2 * ti + 1] * g[2 * tj + 1];
+        const float sigma2 = sigma * sigma;
+        return epsilon * sigma2 * sigma2 * sigma2;
+    }
+    return g[2 * ti] * g[2 * tj];
+}
+
+/* The same grid C6 in FP64 from the (6C6, 12C12) diagonals of the two types. */
+static double grid_c6_pair_f64(int rule, double c6i, double c12i, double c6j, double c12j)
+{
+    if (rule == NBNXM_LJPME_LB)
+    {
+        const double si = (c6i > 0 && c12i > 0) ? 0.5 * sqrt(cbrt(c12i / c6i)) : 0.0;
+        const double ei = (c6i > 0 && c12i > 0) ? sqrt(c6i * c6i / c12i) : 0.0;
+        const double sj = (c6j > 0 && c12j > 0) ? 0.5 * sqrt(cbrt(c12j / c6j)) : 0.0;
+        const double ej = (c6j > 0 && c12j > 0) ? sqrt(c6j * c6j / c12j) : 0.0;
+        const double sigma = si + sj;
+        const double sigma2 = sigma * sigma;
+        return ei * ej * sigma2 * sigma2 * sigma2;
+    }
+    return sqrt(c6i) * sqrt(c6j);
 }
 
 /* ------------------------------------------------------------------------
@@ -592,7 +645,7 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
         return NBNXM_ERR_INVALID;
     }
     const flav32_t p = flav32_from_ic(ic, &tab);
-    float* gridC6 = p.ljPme ? grid_c6_f32(nbat) : NULL;
+    float* gridC6 = p.ljPme ? grid_params_f32(p.ljPme, nbat) : NULL;
     if (p.ljPme && !gridC6)
     {
         ewald_table_free(&tab);
@@ -702,7 +755,7 @@ int oracle_kernel_f32(const nbnxm_pairlist_gpu_t* list,
                                 c6 = epsilon * sigma6;
                                 c12 = c6 * sigma6;
                             }
-                            const float c6grid = p.ljPme ? gridC6[nbat->type[ai]] * gridC6[nbat->type[aj]] : 0.0F;
+                            const float c6grid = p.ljPme ? grid_c6_pair_f32(p.ljPme, gridC6, nbat->type[ai], nbat->type[aj]) : 0.0F;
                             const float F_invr = pair_f32(&p, computeEnergy, r2, int_bit, qi * qj, c6, c12,
                                                           lbForceOnlyKernel, sigma, epsilon, c6grid, &E_lj, &E_el);
                             const float tx = dx * F_invr;
@@ -861,8 +914,9 @@ int oracle_kernel_f64(const nbnxm_pairlist_gpu_t* list,
                             double c6grid = 0.0;
                             if (ic->ljPme)
                             {
-                                c6grid = sqrt((double)nbat->nbfp[2 * (nbat->type[ai] * (ntypes + 1))]
-                                              * (double)nbat->nbfp[2 * (nbat->type[aj] * (ntypes + 1))]);
+                                const float* di = nbat->nbfp + 2 * (nbat->type[ai] * (ntypes + 1));
+                                const float* dj = nbat->nbfp + 2 * (nbat->type[aj] * (ntypes + 1));
+                                c6grid = grid_c6_pair_f64(ic->ljPme, di[0], di[1], dj[0], dj[1]);
                             }
                             double elj = 0.0, eel = 0.0;
                             const double F_invr = pair_f64(ic, &tab, r2, int_bit, qi * qj, c6, c12, c6grid, &elj, &eel);
@@ -995,7 +1049,9 @@ static int bruteforce_f64(int n,
             const double qq = epsfac * q[i] * q[j];
             const double c6 = nbfp[2 * (numTypes * type[i] + type[j])];
             const double c12 = nbfp[2 * (numTypes * type[i] + type[j]) + 1];
-            const double c6grid = ic->ljPme ? sqrt(nbfp[2 * (numTypes * type[i] + type[i])] * nbfp[2 * (numTypes * type[j] + type[j])]) : 0.0;
+            const double* di = nbfp + 2 * (numTypes * type[i] + type[i]);
+            const double* dj = nbfp + 2 * (numTypes * type[j] + type[j]);
+            const double c6grid = ic->ljPme ? grid_c6_pair_f64(ic->ljPme, di[0], di[1], dj[0], dj[1]) : 0.0;
             for (int sz = -1; sz <= 1; sz++)
             {
                 for (int sy = -1; sy <= 1; sy++)

@@ -1,5 +1,5 @@
 """GPU parity of every interaction flavour (reaction field, plain cut-off, twin
-cut-off, LJ force / potential switch, LJ-PME) against the reference's 18
+cut-off, LJ force / potential switch, LJ-PME with both grid rules) against the reference's 18
 refdata cases and the CPU oracle.  Calls go through the C ABI."""
 import numpy as np
 import pytest
@@ -172,6 +172,53 @@ def test_ljpme_with_combination_kernel(vdw):
     assert rel_rms(f[real], f64[real]) < 5e-6
     assert elj == pytest.approx(elj64, rel=2e-5)
     assert eel == pytest.approx(eel64, rel=2e-5)
+
+
+@pytest.fixture(scope="module")
+def mixed_lb():
+    """About 1,500 atoms: 5 chains of 50 atoms with 6 Lorentz-Berthelot LJ
+    types, charges and 1-2/1-3 exclusions, in water (8 LJ types in all)."""
+    return nb.systems.hetero_system(n_chains=5, chain_len=50, total_atoms=1500, n_types=6, seed=7)
+
+
+LJPME_LB_CASES = {
+    "lb": dict(lj_pme="lb", ewald_rtol_lj=1e-3),            # compile-time LJ-PME class
+    "lb_twin": dict(lj_pme="lb", ewald_rtol_lj=1e-3, rvdw=0.85),  # run-time class
+    "geom": dict(lj_pme="geom", ewald_rtol_lj=1e-3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(LJPME_LB_CASES))
+@pytest.mark.parametrize("vdw", [nb.VDW_CUT_TYPE_TABLE, nb.VDW_CUT_COMB_LB])
+@pytest.mark.parametrize("accumulation", ACCUMULATIONS)
+def test_ljpme_grid_rules_mixed_types(mixed_lb, case, vdw, accumulation):
+    """LJ-PME with the Lorentz-Berthelot grid rule (VdwType::EwaldLB,
+    nbnxm_gpu_data_mgmt.cpp:335-346; calculate_lj_ewald_comb_LB_F_E,
+    nbnxm_cuda_kernel_utils.cuh:290-337) and the geometric one on a system
+    with 8 LJ types, F and F+E, against the FP64 oracle: FP32 modes within
+    5e-6 force relative RMS and 2e-5 in the energies, the FP64 mode to 1e-10."""
+    announce(f"ljpme grid rule {case} vdw={vdw} accumulation={accumulation}")
+    s = mixed_lb
+    search, plist, atoms = nb.build(s, 1.1, vdw)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5, **LJPME_LB_CASES[case])
+    f64, fs64, elj64, eel64 = oracle.kernel(plist, atoms, ic, True, "f64")
+    real = atoms.atom_indices >= 0
+    task, f, elj, eel, fshift = _run(ic, atoms, plist, 1.1, accumulation)
+    if accumulation == nb.ACCUM_FP64:
+        assert rel_rms(f[real], f64[real]) < 1e-10
+        assert elj == pytest.approx(elj64, rel=1e-10)
+        assert eel == pytest.approx(eel64, rel=1e-10)
+    else:
+        assert rel_rms(f[real], f64[real]) < 5e-6
+        assert elj == pytest.approx(elj64, rel=2e-5)
+        assert eel == pytest.approx(eel64, rel=2e-5)
+    assert np.abs(fshift - fs64).max() <= 1e-4 * max(1.0, np.abs(fs64).max())
+    # Force-only pass equals the F+E forces.
+    task.clear_outputs()
+    task.launch(compute_energy=False, compute_virial=False)
+    f2, fd2, _, _, _ = task.fetch()
+    f2 = (fd2 if fd2 is not None else f2).reshape(-1, 3)
+    assert rel_rms(f2[real], f[real]) < 1e-6
 
 
 def test_flavour_is_kernel_ewald(water3k):

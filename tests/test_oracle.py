@@ -327,6 +327,68 @@ def test_lj_pme_closed_form():
     assert elj - self_e == pytest.approx(c6 * (1 - g(beta * r)) / r ** 6, rel=1e-10)
 
 
+def test_lj_pme_lb_closed_form():
+    """Lorentz-Berthelot grid rule: for two types the grid C6 the oracle forms
+    from the nbfp diagonals (nbfp_comb, atomdata.cpp:311-324, combined as
+    nbnxm_cuda_kernel_utils.cuh:307-324) equals the textbook
+    C6_ij = 4 sqrt(eps_i eps_j) ((sigma_i + sigma_j)/2)^6, so with the same
+    real-space C6 the dispersion is -C6_ij g(beta r)/r^6 + constants; the
+    geometric grid rule gives a different value for unequal sigma."""
+    rc, rtol = 1.0, 1e-3
+    beta = nb.ewald_coeff_lj(rc, rtol)
+    g = lambda x: math.exp(-x * x) * (1 + x * x + x ** 4 / 2)
+    sig_eps = np.array([[0.28, 0.3], [0.39, 0.8]])
+    c6c12 = nb.systems.combine_params(sig_eps, "lb")
+    c6ij = 4.0 * math.sqrt(0.3 * 0.8) * ((0.28 + 0.39) / 2) ** 6
+    assert c6c12[0, 1, 0] == pytest.approx(c6ij, rel=1e-14)
+    c6ii = [4.0 * e * sg ** 6 for sg, e in sig_eps]
+    ic = nb.interaction_const_ex(rc, lj_pme="lb", ewald_rtol_lj=rtol)
+    assert ic.ljPme == 2
+    ic.epsfac = 0.0
+    self_e = sum(c6ii) * beta ** 6 / 12.0
+    for r in (0.3, 0.6, 0.95):
+        s = nb.systems.System(box=np.array([3.0, 3.0, 3.0]), x=np.array([[0.5, 0.5, 0.5], [0.5 + r, 0.5, 0.5]]),
+                              q=np.zeros(2), types=np.array([0, 1], dtype=np.int32), c6c12=c6c12,
+                              excl_ranges=np.array([0, 1, 2], dtype=np.int32), excl_elements=np.array([0, 1], dtype=np.int32))
+        f, elj, _, _ = oracle.bruteforce(s, ic)
+        c12 = c6c12[0, 1, 1]
+        expect = (-c6ij * g(beta * r) / r ** 6 + c6ij / rc ** 6 + c6ij * ic.sh_lj_ewald
+                  + c12 * (1 / r ** 12 - 1 / rc ** 12))
+        assert elj - self_e == pytest.approx(expect, rel=1e-10)
+        # dV/dr of the same closed form (the C12 term is the plain LJ repulsion).
+        dg = lambda x: -x ** 5 * math.exp(-x * x)  # d/dx g(x)
+        dvdr = 6 * c6ij * g(beta * r) / r ** 7 - c6ij * beta * dg(beta * r) / r ** 6 - 12 * c12 / r ** 13
+        assert f[1, 0] == pytest.approx(-dvdr, rel=1e-9)
+        icg = nb.interaction_const_ex(rc, lj_pme="geom", ewald_rtol_lj=rtol)
+        icg.epsfac = 0.0
+        _, eljg, _, _ = oracle.bruteforce(s, icg)
+        assert abs(eljg - elj) > 1e-6 * abs(elj)
+
+
+def _mixed_lb_system(seed=7):
+    """About 1,500 atoms: 5 chains of 50 atoms with 6 Lorentz-Berthelot LJ
+    types and charges, 1-2/1-3 exclusions, in water."""
+    return nb.systems.hetero_system(n_chains=5, chain_len=50, total_atoms=1500, n_types=6, seed=seed)
+
+
+@pytest.mark.parametrize("rule", ["lb", "geom"])
+def test_lj_pme_mixed_types_list_equals_bruteforce(rule):
+    """Both grid rules on several LJ types (type table kernel): the FP64 list
+    restatement equals the brute force, the FP32 one follows it."""
+    s = wrapped_float_system(_mixed_lb_system())
+    ic = nb.interaction_const_ex(1.0, lj_pme=rule, ewald_rtol_lj=1e-3)
+    search, plist, atoms = nb.build(s, 1.1, nb.VDW_CUT_TYPE_TABLE, 16)
+    f, fs, elj, eel = oracle.kernel(plist, atoms, ic, True, "f64")
+    fb, eljb, eelb, npairs = oracle.bruteforce(s, ic)
+    assert rel_rms(atoms.to_original_order(f, s.n), fb) < 1e-6
+    assert eel == pytest.approx(eelb, rel=1e-7)
+    assert elj == pytest.approx(eljb, rel=1e-6)
+    f32, _, elj32, eel32 = oracle.kernel(plist, atoms, ic, True, "f32")
+    real = atoms.atom_indices >= 0
+    assert rel_rms(f32[real], f[real]) < 5e-6
+    assert elj32 == pytest.approx(elj, rel=1e-4)
+
+
 def test_ewald_table_against_exact():
     """The tabulated Ewald correction interpolates -d/dr erf(beta r)/r with the
     FP32 spline error eps * beta^3 of its spacing (forcetable.cpp:178-181,
