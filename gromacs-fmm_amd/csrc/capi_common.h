@@ -30,6 +30,10 @@ int hipFail(hipError_t e, const char* what);
 // their own work against it).
 hipStream_t gpuTaskStream(const nbnxm_gpu_t* nb);
 
+// Per-system parameter blocks of the tuned and FP64 kernels
+// (NbKernelArgs::sysParamsA/B/D) from the interaction constants.
+void systemParamBlocks(const nbnxm_interaction_const_t& ic, float4* pa, float2* pb, double pd[8]);
+
 // A single-system grid and list that live on the device (the output of a
 // device-input search build), for a GPU task to adopt without a host trip.
 // Cluster, group and mask indices are the single-system ones.

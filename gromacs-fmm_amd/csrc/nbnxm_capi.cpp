@@ -302,6 +302,18 @@ hipStream_t nbnxm_amd::gpuTaskStream(const nbnxm_gpu_t* nb)
     return nb ? nb->stream : nullptr;
 }
 
+void nbnxm_amd::systemParamBlocks(const nbnxm_interaction_const_t& ic, float4* pa, float2* pb, double pd[8])
+{
+    const float rc = static_cast<float>(ic.rcoulomb);
+    *pa = make_float4(rc * rc, static_cast<float>(ic.epsfac), static_cast<float>(ic.ewaldcoeff_q), static_cast<float>(ic.sh_ewald));
+    *pb = make_float2(static_cast<float>(ic.repulsion_shift_cpot), static_cast<float>(ic.dispersion_shift_cpot));
+    const double d[8] = { ic.rcoulomb * ic.rcoulomb, ic.epsfac, ic.ewaldcoeff_q, ic.sh_ewald, ic.repulsion_shift_cpot, ic.dispersion_shift_cpot, 0.0, 0.0 };
+    for (int k = 0; k < 8; k++)
+    {
+        pd[k] = d[k];
+    }
+}
+
 namespace
 {
 
@@ -409,15 +421,14 @@ namespace
 
 int uploadParams(nbnxm_gpu* nb)
 {
-    const float rc = static_cast<float>(nb->ic.rcoulomb);
-    const float4 pa = make_float4(rc * rc, static_cast<float>(nb->ic.epsfac), static_cast<float>(nb->ic.ewaldcoeff_q), static_cast<float>(nb->ic.sh_ewald));
-    const float2 pb = make_float2(static_cast<float>(nb->ic.repulsion_shift_cpot), static_cast<float>(nb->ic.dispersion_shift_cpot));
+    float4 pa;
+    float2 pb;
+    double pd[8];
+    systemParamBlocks(nb->ic, &pa, &pb, pd);
     NBNXM_HIP_CHECK(nb->paramsA.resize(1));
     NBNXM_HIP_CHECK(nb->paramsB.resize(1));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsA.ptr, &pa, sizeof(pa), hipMemcpyHostToDevice, nb->stream));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsB.ptr, &pb, sizeof(pb), hipMemcpyHostToDevice, nb->stream));
-    const double pd[8] = { nb->ic.rcoulomb * nb->ic.rcoulomb, nb->ic.epsfac, nb->ic.ewaldcoeff_q, nb->ic.sh_ewald,
-                           nb->ic.repulsion_shift_cpot, nb->ic.dispersion_shift_cpot, 0.0, 0.0 };
     NBNXM_HIP_CHECK(nb->paramsD.resize(8));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->paramsD.ptr, pd, sizeof(pd), hipMemcpyHostToDevice, nb->stream));
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));

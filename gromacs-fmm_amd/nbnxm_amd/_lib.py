@@ -113,6 +113,16 @@ class GpuTimings(ctypes.Structure):
                 ("pl_h2d_t", ctypes.c_double), ("pl_h2d_c", ctypes.c_int)]
 
 
+class DeviceInputs(ctypes.Structure):
+    _fields_ = [("numAtoms", ctypes.c_int), ("xq", ctypes.c_void_p), ("numTypes", ctypes.c_int),
+                ("type", ctypes.c_void_p), ("nbfp", ctypes.c_void_p), ("vdwType", ctypes.c_int),
+                ("ljComb", ctypes.c_void_p), ("shiftVec", ctypes.c_void_p), ("nsci", ctypes.c_int),
+                ("sci", ctypes.c_void_p), ("cjPacked", ctypes.c_void_p), ("excl", ctypes.c_void_p)]
+
+
+NOALLOC_MAX_ENTRIES = 65536
+
+
 # numpy dtypes with the same layout as the C structs
 SCI_DTYPE = np.dtype([("sci", np.int32), ("shift", np.int32),
                       ("cjPackedBegin", np.int32), ("cjPackedEnd", np.int32)])
@@ -167,6 +177,8 @@ SIGNATURES = [
     ("nbnxm_gpu_reset_timings", _c_int, [_vp]),
     ("nbnxm_gpu_get_f", _c_int, [_vp, _PTR(_vp)]),
     ("nbnxm_gpu_get_stream", _c_int, [_vp, _PTR(_vp)]),
+    ("nbnxm_gpu_launch_kernel_noalloc", _c_int, [_PTR(InteractionConst), _PTR(DeviceInputs), _c_int, _c_int, _c_int,
+                                                 _vp, _vp, _vp, _vp, _vp]),
     ("nbnxm_gpu_get_virial", _c_int, [_vp, _vp]),
     ("nbnxm_gpu_init_x_to_nbat_x", _c_int, [_vp, _c_int, _c_int, _vp]),
     ("nbnxm_gpu_x_to_nbat_x", _c_int, [_vp, _vp]),

@@ -379,6 +379,57 @@ int nbnxm_gpu_get_f(nbnxm_gpu_t* nb, float** d_f);
 /* The task's HIP stream (the reference's local nonbonded DeviceStream,
  * nbnxm_cuda_types.h:106), as a hipStream_t. */
 int nbnxm_gpu_get_stream(nbnxm_gpu_t* nb, void** stream);
+/* Device inputs of nbnxm_gpu_launch_kernel_noalloc: what gpu_init_atomdata,
+ * gpu_upload_shiftvec and gpu_init_pairlist would upload, already in DEVICE
+ * memory owned by the caller, in the layouts of nbnxm_atomdata_t and
+ * nbnxm_pairlist_gpu_t. */
+typedef struct nbnxm_device_inputs
+{
+    int numAtoms;                      /* grid slots, multiple of 8 */
+    const float* xq;                   /* 4*numAtoms */
+    int numTypes;                      /* type table: including the dummy type */
+    const int* type;                   /* numAtoms, type table kernel, else NULL */
+    const float* nbfp;                 /* 2*numTypes^2, type table kernel, else NULL */
+    int vdwType;                       /* enum nbnxm_vdw_type */
+    const float* ljComb;               /* 2*numAtoms, combination rules, else NULL */
+    const float* shiftVec;             /* 4*NBNXM_NUM_SHIFTS: x, y, z, 0 per shift */
+    int nsci;                          /* entries, at most NBNXM_NOALLOC_MAX_ENTRIES */
+    const nbnxm_sci_t* sci;
+    const nbnxm_cj_packed_t* cjPacked; /* masks as evaluated (a pruned list is not pruned again) */
+    const nbnxm_excl_t* excl;          /* excl[0] all ones */
+} nbnxm_device_inputs_t;
+
+/* Largest number of pair-list entries of one nbnxm_gpu_launch_kernel_noalloc
+ * call: its scratch holds 64 bytes per entry (4 MiB in all, per GPU). */
+#define NBNXM_NOALLOC_MAX_ENTRIES 65536
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the nonbonded evaluation of nbnxm_gpu_launch_kernel on a
+ * list that needs no further pruning, followed by the energy and shift-force
+ * reduction of nbnxm_gpu_launch_cpyback, on caller-owned device buffers,
+ * asynchronously on `stream` (a hipStream_t, NULL = the null stream).  It
+ * allocates nothing: its scratch (parameter blocks and the FP64 partials of
+ * every entry) is a fixed library array on each GPU, so it accepts at most
+ * NBNXM_NOALLOC_MAX_ENTRIES entries (NBNXM_ERR_INVALID beyond).  Calls that
+ * overlap on one GPU share that array and race: order them on one stream.
+ * The tuned flavour only (analytical Ewald + LJ potential shift, rvdw =
+ * rcoulomb; NBNXM_ERR_UNSUPPORTED otherwise), any accumulation mode.  Forces
+ * are ADDED to d_fFixed (FIXED64 / FP64: int64, 2^-32 units, 3 per slot) or
+ * d_fFloat (FLOAT); d_energies (2: E_lj, E_el) and d_fshift (3*45) are
+ * written, and are only needed with computeEnergy / computeFshift.  Forces,
+ * energies and shift forces equal, bit for bit, those of a task created by
+ * nbnxm_gpu_init with the same constants, list and atom data. */
+int nbnxm_gpu_launch_kernel_noalloc(const nbnxm_interaction_const_t* ic,
+                                    const nbnxm_device_inputs_t* in,
+                                    int accumulation,
+                                    int computeEnergy,
+                                    int computeFshift,
+                                    unsigned long long* d_fFixed,
+                                    float* d_fFloat,
+                                    double* d_energies,
+                                    double* d_fshift,
+                                    void* stream);
 /* Adds the single-sum virial (9, row-major) of the last launch made with
  * computeVirial to virial, after waiting.  Replaces the host calc_vir over the
  * nbat forces and shift forces (src/gromacs/mdlib/calcvir.cpp:89-133, as
