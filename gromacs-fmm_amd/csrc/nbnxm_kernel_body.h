@@ -363,9 +363,9 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     }
 
     // Newton / self masking on the diagonal of the central shift (:423): on a
-    // cluster pair with ci == cj only lanes with tj > ti interact.
-    // Lanes with tj > ti: the pairs of a diagonal cluster pair that interact.
-    const bool upperLane = tj > ti;
+    // cluster pair with ci == cj only lanes with tj > ti interact; their lane
+    // mask (a constant of the layout) as a wave-uniform value.
+    const unsigned long long upperLanes = __builtin_amdgcn_ballot_w64(tj > ti);
     const bool lowerLane = ti < 4;
     const bool xLane = ti < 2;
     const bool yLane = (ti >> 1) == 1;
@@ -443,16 +443,15 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 float r2 = norm2Fixed(dx, dy, dz);
                 // On the diagonal of the central shift only pairs with j > i
                 // interact (nonSelfInteraction, :423,508), whatever the list's
-                // exclusion words say: lanes with tj <= ti leave the cut-off.
-                // Wave-uniform: exclHere from the group's OR of its exclusion
-                // words, the diagonal test from list indices.
+                // exclusion words say: the lanes with tj <= ti leave the
+                // in-range mask.  Wave-uniform: exclHere from the group's OR of
+                // its exclusion words, the lane mask from list indices, so the
+                // rule costs a scalar AND of masks rather than a select per lane.
                 const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
-                if (centralShift && (firstCluster + c == cj))
-                {
-                    r2 = upperLane ? r2 : 1.0e30f;
-                }
-                const bool inRange = r2 < rcoulomb_sq;
-                anyInRange |= __builtin_amdgcn_ballot_w64(inRange);
+                const unsigned long long keepLanes = (centralShift && (firstCluster + c == cj)) ? upperLanes : ~0ull;
+                const unsigned long long inRangeLanes = __builtin_amdgcn_ballot_w64(r2 < rcoulomb_sq) & keepLanes;
+                const bool inRange = __builtin_amdgcn_inverse_ballot_w64(inRangeLanes);
+                anyInRange |= inRangeLanes;
                 if (inRange)
                 {
                     float c6, c12;

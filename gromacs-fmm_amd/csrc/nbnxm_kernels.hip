@@ -208,20 +208,32 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernelFp64(const NbKernelAr
     }
 }
 
-// Prune (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343), one
-// wave per entry of the current rolling part, split 1.
+// Prune (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343), split
+// 1: one workgroup of c_pruneWavesPerEntry waves per entry of the current
+// rolling part, wave k of it checking packed groups k, k + 4, ... (an entry of
+// a large box has hundreds of groups; one wave per entry would leave most
+// SIMDs idle on lists of a few thousand entries and on rolling parts).  The
+// list mechanics of the nonbonded kernel: list words through the constant
+// address space (scalar loads), groups with nothing to check skipped by a
+// scalar test, and the 32 j-atoms of the wave's next group fetched into
+// registers while the current one is checked, then staged in per-wave LDS, so
+// no distance waits for a global load.  The distances and ballots are those
+// of the reference.
+constexpr int c_pruneWavesPerEntry = 4;
+
 template<bool haveFreshList>
-__global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
+__global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(const PruneArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7;
     const int tj = lane >> 3;
-    const int wavesPerBlock = blockDim.x >> 6;
-    const int w = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+    const int waveInEntry = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = blockIdx.x;
 
-    // Each wave keeps its own copy of the part index, as each block does in the reference (:119-127).
-    const int part = a.rollingPart[w];
-    if (lane == 0)
+    // The part index per block, as the reference keeps it (:119-127).
+    const int part = __builtin_amdgcn_readfirstlane(a.rollingPart[w]);
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
         a.rollingPart[w] = (part + 1) % a.numParts;
     }
@@ -231,9 +243,9 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
         return;
     }
     const int work = w * a.numParts + part;
-    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? a.sciOrder[work] : work);
-    const nbnxm_sci_t s = a.sci[entry];
-    const float4 sh = a.shiftVec[s.shift];
+    const int entry = __builtin_amdgcn_readfirstlane(a.sciOrder ? asConst(a.sciOrder)[work] : work);
+    const nbnxm_sci_t s = loadSci(asConst(static_cast<const nbnxm_sci_t*>(a.sci)), entry);
+    const float4 sh = loadF4(asConst(static_cast<const float4*>(a.shiftVec)), s.shift);
     float xi[c_ncps], yi[c_ncps], zi[c_ncps];
 #pragma unroll
     for (int c = 0; c < c_ncps; c++)
@@ -243,25 +255,72 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
         yi[c] = v.y + sh.y;
         zi[c] = v.z + sh.z;
     }
+    // The list words of a group are read (scalar) before its wave writes
+    // them, and each group is read and written by one wave only.
+    const ConstPtr<nbnxm_cj_packed_t> cjList = asConst(static_cast<const nbnxm_cj_packed_t*>(a.cjPacked));
+    const ConstPtr<unsigned int> outerList = asConst(static_cast<const unsigned int*>(a.imaskOuter));
+    // Bits still to check: every bit of a fresh list, else those of the outer
+    // mask that the inner mask has dropped (:229-245).
+    auto checkBits = [&](int g) -> unsigned int {
+        const unsigned int inner = cjList[g].imei[0].imask;
+        return haveFreshList ? inner : (inner ^ outerList[g]);
+    };
+    auto nextToCheck = [&](int g) {
+        while (g < s.cjPackedEnd && checkBits(g) == 0u)
+        {
+            g += c_pruneWavesPerEntry;
+        }
+        return g;
+    };
+    __shared__ float4 sXq[c_pruneWavesPerEntry][c_jg * c_cl];
+    float4* stageXq = sXq[waveInEntry];
+    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto prefetch = [&](int g) {
+        const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
+        const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
+        const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
+        const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
+        const int k = (lane >> 3) & 3;
+        const int cjl = (k == 1) ? g1 : ((k == 2) ? g2 : ((k == 3) ? g3 : g0));
+        if (lane < 32)
+        {
+            pf = a.xq[cjl * c_cl + (lane & 7)];
+        }
+    };
+    auto stage = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 32)
+        {
+            stageXq[lane] = pf;
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+
     int prunedPairCount = 0;
-    for (int jp = s.cjPackedBegin; jp < s.cjPackedEnd; jp++)
+    int jp = nextToCheck(s.cjPackedBegin + waveInEntry);
+    if (jp < s.cjPackedEnd)
     {
-        unsigned int imaskFull, imaskCheck, imaskNew;
+        prefetch(jp);
+        stage();
+    }
+    while (jp < s.cjPackedEnd)
+    {
+        unsigned int imaskFull, imaskNew;
+        const unsigned int imaskCheck = checkBits(jp);
         if (haveFreshList)
         {
-            imaskFull = a.cjPacked[jp].imei[0].imask;
-            imaskCheck = imaskFull;
-            imaskNew = 0;
+            imaskFull = imaskCheck;
+            imaskNew = 0u;
         }
         else
         {
-            imaskFull = a.imaskOuter[jp];
-            imaskNew = a.cjPacked[jp].imei[0].imask;
-            imaskCheck = imaskNew ^ imaskFull;
+            imaskFull = outerList[jp];
+            imaskNew = cjList[jp].imei[0].imask;
         }
-        if (imaskCheck == 0)
+        const int jn = nextToCheck(jp + c_pruneWavesPerEntry);
+        if (jn < s.cjPackedEnd)
         {
-            continue;
+            prefetch(jn);
         }
 #pragma unroll
         for (int jm = 0; jm < c_jg; jm++)
@@ -270,8 +329,7 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
             {
                 continue;
             }
-            const int cj = a.cjPacked[jp].cj[jm];
-            const float4 xj = a.xq[cj * c_cl + tj];
+            const float4 xj = stageXq[jm * c_cl + tj];
 #pragma unroll
             for (int c = 0; c < c_ncps; c++)
             {
@@ -281,11 +339,11 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
                     continue;
                 }
                 const float r2 = norm2Fixed(xi[c] - xj.x, yi[c] - xj.y, zi[c] - xj.z);
-                if (haveFreshList && __ballot(r2 < a.rlistOuter_sq) == 0)
+                if (haveFreshList && __builtin_amdgcn_ballot_w64(r2 < a.rlistOuter_sq) == 0ull)
                 {
                     imaskFull &= ~mask_ji;
                 }
-                if (__ballot(r2 < a.rlistInner_sq) != 0)
+                if (__builtin_amdgcn_ballot_w64(r2 < a.rlistInner_sq) != 0ull)
                 {
                     imaskNew |= mask_ji;
                 }
@@ -303,17 +361,37 @@ __global__ __launch_bounds__(256) void nbnxmPruneKernel(const PruneArgs a)
         {
             prunedPairCount += __popc(imaskNew);
         }
-    }
-    if (haveFreshList && lane == 0)
-    {
-        const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
-        if (a.sciHistogram)
+        if (jn < s.cjPackedEnd)
         {
-            atomicAdd(a.sciHistogram + index, 1);
+            stage();
         }
-        if (a.sciCount)
+        jp = jn;
+    }
+    if (haveFreshList)
+    {
+        // The entry's pruned cluster pairs over its waves, in wave order.
+        __shared__ int sCount[c_pruneWavesPerEntry];
+        if (lane == 0)
         {
-            a.sciCount[work] = index;
+            sCount[waveInEntry] = prunedPairCount;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            int total = 0;
+            for (int k = 0; k < c_pruneWavesPerEntry; k++)
+            {
+                total += sCount[k];
+            }
+            const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - total - 1, 0);
+            if (a.sciHistogram)
+            {
+                atomicAdd(a.sciHistogram + index, 1);
+            }
+            if (a.sciCount)
+            {
+                a.sciCount[work] = index;
+            }
         }
     }
 }
@@ -746,20 +824,20 @@ hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType,
 
 hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t stream)
 {
-    const int wpb = 4;
-    const int maxInPart = (a.nsci + a.numParts - 1) / a.numParts;
-    const int blocks = (maxInPart + wpb - 1) / wpb;
+    // One workgroup per entry of the largest part (a.rollingPart holds at
+    // least that many part counters).
+    const int blocks = (a.nsci + a.numParts - 1) / a.numParts;
     if (blocks == 0)
     {
         return hipSuccess;
     }
     if (haveFreshList)
     {
-        hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * wpb), 0, stream, a);
+        hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * c_pruneWavesPerEntry), 0, stream, a);
     }
     else
     {
-        hipLaunchKernelGGL(nbnxmPruneKernel<false>, dim3(blocks), dim3(64 * wpb), 0, stream, a);
+        hipLaunchKernelGGL(nbnxmPruneKernel<false>, dim3(blocks), dim3(64 * c_pruneWavesPerEntry), 0, stream, a);
     }
     return hipGetLastError();
 }
