@@ -182,24 +182,23 @@ def md_step_table(steps):
     box: search every 100 steps on the host or on the device, rolling prune,
     x -> xq, kernel (energies every 100 steps), force reduction; coordinates
     held fixed, so only the nonbonded work is timed."""
-    import torch
     import nbnxm_amd as nb
 
     s = nb.systems.config_water_96k()
     params = np.stack([6.0 * s.c6c12[..., 0], 12.0 * s.c6c12[..., 1]], axis=-1)
     ic = nb.interaction_const(RC, EWALD_RTOL)
-    x = torch.tensor(s.x, dtype=torch.float32, device="cuda").contiguous()
-    f = torch.zeros_like(x)
-    stream = torch.cuda.current_stream().cuda_stream
+    d_x = nb.DeviceArray(s.x.astype(np.float32))
+    d_f = nb.DeviceArray.zeros(d_x.shape)
+    stream = 0
     for gpu_search in (False, True):
         ver = nb.NonbondedVerlet(s.types, params, s.q, ic, nstlist=100, rlist_outer=RLIST_OUTER, rlist_inner=1.02,
                                  nstlist_prune=10, excl_ranges=s.excl_ranges, excl_elements=s.excl_elements,
                                  gpu_search=gpu_search)
-        ver.step(0, s.box, x.data_ptr(), f.data_ptr(), stream)
+        ver.step(0, s.box, d_x.ptr, d_f.ptr, stream)
         search_ms, other = [], []
         for step in range(1, steps + 1):
             t0 = time.perf_counter()
-            ver.step(step, s.box, x.data_ptr(), f.data_ptr(), stream, compute_energy=(step % 100 == 0))
+            ver.step(step, s.box, d_x.ptr, d_f.ptr, stream, compute_energy=(step % 100 == 0))
             dt = (time.perf_counter() - t0) * 1e3
             (search_ms if step % 100 == 0 else other).append(dt)
         st = ver.stats()

@@ -12,18 +12,6 @@ for p in (ROOT, os.path.join(ROOT, "gromacs-fmm_amd")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
-    # GPU runs: torch's GPU runtime is initialised before the library's,
-    # whatever order the test modules run in (torch reported no GPU when the
-    # library had initialised its runtime first, and the tests that drive an
-    # integrator on torch tensors skipped).
-    markexpr = getattr(config.option, "markexpr", "") or ""
-    if "gpu" in markexpr and "not gpu" not in markexpr:
-        try:
-            import torch
-
-            torch.cuda.is_available()
-        except ImportError:
-            pass
 
 
 @pytest.fixture(scope="session")

@@ -3,6 +3,7 @@ parameter update, the non-blocking finish, event timings, the nbat force buffer
 of gpu_get_f, the state after a new atom count, and the j > i rule on the
 diagonal of a hand-built list."""
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -265,3 +266,18 @@ def test_diagonal_without_newton_mask(water3k):
             assert rel_rms(f[real], f64[real]) < tol
             assert elj == pytest.approx(elj64, rel=2e-5)
             assert eel == pytest.approx(eel64, rel=2e-5)
+
+
+def test_one_hip_runtime_in_process():
+    """The library, and the device arrays the tests and the benchmark hand it,
+    run on one HIP runtime: the libamdhip64 of /opt/rocm the library is linked
+    against (torch's wheel carries another under the same SONAME; nothing
+    that drives the library imports torch)."""
+    announce("one HIP runtime in the process")
+    d = nb.DeviceArray(np.arange(16, dtype=np.float32))
+    assert np.array_equal(d.download(), np.arange(16, dtype=np.float32))
+    with open("/proc/self/maps") as fh:
+        paths = {line.split()[-1] for line in fh if "libamdhip64" in line}
+    real = {os.path.realpath(p) for p in paths}
+    assert len(real) == 1, real
+    assert next(iter(real)).startswith(os.path.realpath("/opt/rocm")), real

@@ -1,7 +1,9 @@
 """The nonbonded part of an MD loop on the GPU (nbnxm_verlet_*): search every
 nstlist steps, dynamic rolling prune, device buffer ops, inside a leapfrog
-integrator on device arrays.  Checked against the CPU oracle at the current
-positions and by energy conservation.  Calls go through the C ABI."""
+integrator whose coordinates and forces live in device arrays (the step reads
+x and writes f there; the update itself runs on the host).  Checked against
+the CPU oracle at the current positions and by energy conservation.  Calls go
+through the C ABI."""
 import dataclasses
 import math
 import types
@@ -58,39 +60,40 @@ def oracle_atoms(x_atoms, q, sci, cj, excl, ai, box):
 
 
 class Leapfrog:
-    """v(t+dt/2) = v(t-dt/2) + f(t)/m dt; x(t+dt) = x(t) + v(t+dt/2) dt on the GPU."""
+    """v(t+dt/2) = v(t-dt/2) + f(t)/m dt; x(t+dt) = x(t) + v(t+dt/2) dt in
+    FP32, with x and f held in device arrays for the step.  x is read back
+    before each update: a search step puts the atoms in the box in place."""
 
-    def __init__(self, torch, system, temperature, dt, seed=3):
-        self.torch = torch
+    def __init__(self, system, temperature, dt, seed=3):
         rng = np.random.default_rng(seed)
         v = rng.normal(0.0, math.sqrt(KB * temperature / MASS), size=system.x.shape)
         v -= v.mean(axis=0)
-        self.x = torch.tensor(system.x, dtype=torch.float32, device="cuda").contiguous()
-        self.v = torch.tensor(v, dtype=torch.float32, device="cuda").contiguous()
-        self.f = torch.zeros_like(self.x)
+        self.x = system.x.astype(np.float32)
+        self.v = v.astype(np.float32)
+        self.d_x = nb.DeviceArray(self.x)
+        self.d_f = nb.DeviceArray.zeros(self.x.shape)
         self.dt = dt
 
     def kinetic(self, v):
-        return 0.5 * MASS * float((v.double() ** 2).sum())
+        return 0.5 * MASS * float((v.astype(np.float64) ** 2).sum())
+
+    def forces(self):
+        return self.d_f.download()
+
+    def positions(self):
+        return self.d_x.download()
 
     def advance(self):
-        v_old = self.v.clone()
-        self.v += self.f * (self.dt / MASS)
-        self.x += self.v * self.dt
+        v_old = self.v.copy()
+        self.v += self.forces() * np.float32(self.dt / MASS)
+        self.x = self.positions() + self.v * np.float32(self.dt)
+        self.d_x.upload(self.x)
         return 0.5 * (self.kinetic(v_old) + self.kinetic(self.v))
-
-
-@pytest.fixture(scope="module")
-def torch():
-    import torch as t
-    if not t.cuda.is_available():
-        pytest.skip("no GPU")
-    return t
 
 
 @pytest.mark.parametrize("accumulation,gpu_search", [(nb.ACCUM_FIXED64, False), (nb.ACCUM_FLOAT, False),
                                                      (nb.ACCUM_FIXED64, True)])
-def test_md_list_and_pruning_cover_all_pairs(torch, accumulation, gpu_search):
+def test_md_list_and_pruning_cover_all_pairs(accumulation, gpu_search):
     """60 leapfrog steps with nstlist 20 and rolling pruning of the inner list:
     before each search, the list in use holds every pair within the cut-off
     and the forces and energies match the list-free FP64 sum at the current
@@ -100,19 +103,18 @@ def test_md_list_and_pruning_cover_all_pairs(torch, accumulation, gpu_search):
     ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
     ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
                              rlist_inner=1.05, nstlist_prune=4, accumulation=accumulation, gpu_search=gpu_search)
-    md = Leapfrog(torch, system, 120.0, 0.004)
-    stream = torch.cuda.current_stream().cuda_stream
+    md = Leapfrog(system, 120.0, 0.004)
     checked = 0
     for step in range(60):
         check = step % 20 == 19
-        ver.step(step, system.box, md.x.data_ptr(), md.f.data_ptr(), stream, compute_energy=check)
+        ver.step(step, system.box, md.d_x.ptr, md.d_f.ptr, 0, compute_energy=check)
         if check:
-            x_now = md.x.cpu().numpy().astype(np.float64)
+            x_now = md.positions().astype(np.float64)
             cur = nb.systems.System(box=system.box, x=x_now, q=system.q, types=system.types,
                                     c6c12=system.c6c12, excl_ranges=system.excl_ranges,
                                     excl_elements=system.excl_elements)
             fb, eljb, eelb, npairs = oracle.bruteforce(cur, ic)
-            f = md.f.cpu().numpy()
+            f = md.forces()
             assert rel_rms(f, fb) < 5e-6
             elj, eel = ver.energies()
             assert elj == pytest.approx(eljb, rel=2e-5)
@@ -130,7 +132,7 @@ def test_md_list_and_pruning_cover_all_pairs(torch, accumulation, gpu_search):
     assert st["prunes"] == 27
 
 
-def test_md_energy_conservation(torch):
+def test_md_energy_conservation():
     """NVE leapfrog with the force-switched LJ fluid: the total energy (kinetic
     averaged over the half steps) stays within 0.2 % of the kinetic energy
     over 600 steps with dynamic pruning."""
@@ -140,11 +142,10 @@ def test_md_energy_conservation(torch):
     ic.epsfac = 0.0
     ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
                              rlist_inner=1.05, nstlist_prune=4)
-    md = Leapfrog(torch, system, 100.0, 0.004)
-    stream = torch.cuda.current_stream().cuda_stream
+    md = Leapfrog(system, 100.0, 0.004)
     energies = []
     for step in range(600):
-        ver.step(step, system.box, md.x.data_ptr(), md.f.data_ptr(), stream, compute_energy=True)
+        ver.step(step, system.box, md.d_x.ptr, md.d_f.ptr, 0, compute_energy=True)
         elj, _ = ver.energies()
         ke = md.advance()
         if step >= 100:
@@ -156,7 +157,7 @@ def test_md_energy_conservation(torch):
 
 
 @pytest.mark.parametrize("gpu_search", [False, True])
-def test_box_change_triggers_search(torch, gpu_search):
+def test_box_change_triggers_search(gpu_search):
     """A box change between searches (pressure coupling scales box and
     coordinates) starts a new list at once; forces and energies then match
     the list-free FP64 sum in the new box."""
@@ -165,27 +166,28 @@ def test_box_change_triggers_search(torch, gpu_search):
     ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
     ver = nb.NonbondedVerlet(system.types, params, system.q, ic, nstlist=20, rlist_outer=1.15,
                              rlist_inner=1.05, nstlist_prune=4, gpu_search=gpu_search)
-    x = torch.tensor(system.x, dtype=torch.float32, device="cuda").contiguous()
-    f = torch.zeros_like(x)
-    stream = torch.cuda.current_stream().cuda_stream
+    x = system.x.astype(np.float32)
+    d_x = nb.DeviceArray(x)
+    d_f = nb.DeviceArray.zeros(x.shape)
     for step in range(3):
-        ver.step(step, system.box, x.data_ptr(), f.data_ptr(), stream)
+        ver.step(step, system.box, d_x.ptr, d_f.ptr, 0)
     assert ver.stats()["searches"] == 1
     scale = 1.01
     box = system.box * scale
-    x *= scale
-    ver.step(3, box, x.data_ptr(), f.data_ptr(), stream, compute_energy=True)
+    x = d_x.download() * np.float32(scale)
+    d_x.upload(x)
+    ver.step(3, box, d_x.ptr, d_f.ptr, 0, compute_energy=True)
     assert ver.stats()["searches"] == 2
-    cur = nb.systems.System(box=box, x=x.cpu().numpy().astype(np.float64), q=system.q, types=system.types,
+    cur = nb.systems.System(box=box, x=d_x.download().astype(np.float64), q=system.q, types=system.types,
                             c6c12=system.c6c12, excl_ranges=system.excl_ranges, excl_elements=system.excl_elements)
     fb, eljb, eelb, _ = oracle.bruteforce(cur, ic)
-    assert rel_rms(f.cpu().numpy(), fb) < 5e-6
+    assert rel_rms(d_f.download(), fb) < 5e-6
     elj, eel = ver.energies()
     assert elj == pytest.approx(eljb, rel=2e-5)
     assert eel == pytest.approx(eelb, rel=2e-5, abs=1e-3)
 
 
-def test_device_search_with_exclusions(torch):
+def test_device_search_with_exclusions():
     """A 3000-atom water box whose intramolecular exclusions go to the device
     search: coordinates jittered every step, the list rebuilt on the device
     every 5 steps, and after each rebuild the forces and energies match the
@@ -197,22 +199,23 @@ def test_device_search_with_exclusions(torch):
     params = np.asarray(w.c6c12, dtype=np.float64) * np.array([6.0, 12.0])
     ver = nb.NonbondedVerlet(w.types, params, w.q, ic, nstlist=5, rlist_outer=1.1, excl_ranges=w.excl_ranges,
                              excl_elements=w.excl_elements, gpu_search=True)
-    x = torch.tensor(w.x, dtype=torch.float32, device="cuda").contiguous()
-    f = torch.zeros_like(x)
-    gen = torch.Generator(device="cuda").manual_seed(11)
-    stream = torch.cuda.current_stream().cuda_stream
+    x = w.x.astype(np.float32)
+    d_x = nb.DeviceArray(x)
+    d_f = nb.DeviceArray.zeros(x.shape)
+    rng = np.random.default_rng(11)
     checked = 0
     for step in range(16):
         check = step in (5, 10, 15)
-        ver.step(step, w.box, x.data_ptr(), f.data_ptr(), stream, compute_energy=check)
+        ver.step(step, w.box, d_x.ptr, d_f.ptr, 0, compute_energy=check)
         if check:
-            cur = dataclasses.replace(w, x=x.cpu().numpy().astype(np.float64))
+            cur = dataclasses.replace(w, x=d_x.download().astype(np.float64))
             fb, eljb, eelb, npairs = oracle.bruteforce(cur, ic)
-            assert rel_rms(f.cpu().numpy(), fb) < 5e-6
+            assert rel_rms(d_f.download(), fb) < 5e-6
             elj, eel = ver.energies()
             assert elj == pytest.approx(eljb, rel=2e-5)
             assert eel == pytest.approx(eelb, rel=2e-5)
             checked += 1
-        x += 0.002 * torch.randn(x.shape, generator=gen, device="cuda")
+        x = d_x.download() + (0.002 * rng.standard_normal(x.shape)).astype(np.float32)
+        d_x.upload(x)
     assert checked == 3
     assert ver.stats()["searches"] == 4
