@@ -556,6 +556,10 @@ int setPairlist(nbnxm_gpu* nb,
         NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
     }
     NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, rollingSize, 0, nb->stream));
+    // Outer masks start empty: the prune kernels write those of the groups
+    // with cluster pairs, so a group uploaded without any keeps 0 rather than
+    // what an earlier list left there.
+    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->imaskOuter.ptr), ncj, 0, nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->partials.ptr), 2 * 8 * static_cast<size_t>(std::max(nsci, 1)), 0, nb->stream));
 
     // Radix-sort scratch for the sci ordering.
@@ -1244,6 +1248,24 @@ int nbnxm_gpu_clear_outputs(nbnxm_gpu_t* nb, int computeVirial)
     return NBNXM_OK;
 }
 
+// After the prune of a fresh list (by the prune kernel or the combined one):
+// entries ordered by descending pruned pair count (the role of
+// nbnxnKernelBucketSciSort, src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_sci_sort.cuh:66-87);
+// a stable radix sort on the histogram bin keeps the order deterministic.
+static int sortEntriesOfFreshList(nbnxm_gpu_t* nb)
+{
+    if (nb->nsci > 0)
+    {
+        size_t tempBytes = nb->sortTemp.size;
+        NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nb->sortTemp.ptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, nb->nsci, 0, 14, nb->stream));
+    }
+    nb->haveSortedOrder = true;
+    nb->haveFreshList = false;
+    // Rolling pruning restarts from part 0 after a fresh list.
+    NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, nb->rollingPart.size, 0, nb->stream));
+    return NBNXM_OK;
+}
+
 static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
 {
     PruneArgs pa;
@@ -1257,8 +1279,10 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
     pa.rollingPart = nb->rollingPart.ptr;
     pa.sciCount = nb->sciCount.ptr;
     pa.sciHistogram = nb->sciHistogram.ptr;
+    // Without dynamic pruning the list is pruned to rlistOuter only, as the
+    // reference's combined kernel does.
     const float ro = static_cast<float>(nb->listParams.rlistOuter);
-    const float ri = static_cast<float>(nb->listParams.rlistInner);
+    const float ri = static_cast<float>(nb->listParams.useDynamicPruning ? nb->listParams.rlistInner : nb->listParams.rlistOuter);
     pa.rlistOuter_sq = ro * ro;
     pa.rlistInner_sq = ri * ri;
     pa.numParts = numParts;
@@ -1284,18 +1308,7 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
     }
     if (fresh)
     {
-        // Order entries by descending pruned pair count (the role of
-        // nbnxnKernelBucketSciSort, src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_sci_sort.cuh:66-87);
-        // a stable radix sort on the histogram bin keeps the order deterministic.
-        if (nb->nsci > 0)
-        {
-            size_t tempBytes = nb->sortTemp.size;
-            NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nb->sortTemp.ptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, nb->nsci, 0, 14, nb->stream));
-        }
-        nb->haveSortedOrder = true;
-        nb->haveFreshList = false;
-        // Rolling pruning restarts from part 0 after a fresh list.
-        NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, nb->rollingPart.size, 0, nb->stream));
+        return sortEntriesOfFreshList(nb);
     }
     return NBNXM_OK;
 }
@@ -1323,9 +1336,23 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     {
         return fail(NBNXM_ERR_STATE, "gpu_launch_kernel: shift vectors and pair list must be uploaded first");
     }
-    // A fresh list is pruned first (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:505-513).
+    // Which kernel runs: the tuned list walk (default flavour), the list walk
+    // with a flavour class, or the standalone flavour kernel (FP64 mode of the
+    // other flavours, LJ-PME with a combination-rule kernel, type tables
+    // beyond the LDS staging).
+    // The tuned list walk stages the type table in LDS up to 65 types (64 + dummy).
+    const bool tableFitsLds = nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE || nb->numTypes <= c_maxLdsTableTypes;
+    const bool tunedWalk = isDefaultFlavour(nb->ic) && (tableFitsLds || nb->accumulation == NBNXM_ACCUM_FP64);
+    const bool flavouredWalk = !tunedWalk && nb->accumulation != NBNXM_ACCUM_FP64 && tableFitsLds
+                               && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE);
+    // A fresh list: with dynamic pruning it is pruned first by the prune
+    // kernel (src/gromacs/nbnxm/cuda/nbnxm_cuda.cu:505-513); without, the
+    // FP32 list walks prune it to rlistOuter themselves (the _prune kernels,
+    // :565-578), and the entries are sorted after the launch.
     const bool freshList = nb->haveFreshList;
-    if (nb->haveFreshList)
+    const bool combinedPrune = freshList && !nb->listParams.useDynamicPruning && (tunedWalk || flavouredWalk)
+                               && nb->accumulation != NBNXM_ACCUM_FP64;
+    if (freshList && !combinedPrune)
     {
         const int rc = launchPruneImpl(nb, 1);
         if (rc != NBNXM_OK)
@@ -1352,27 +1379,43 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
     a.fFixed = nb->fFixed.ptr;
     a.entryPartials = nb->partials.ptr;
     a.computeFshift = computeVirial ? 1 : 0;
-    // The tuned list walk stages the type table in LDS up to 65 types (64 + dummy).
-    const bool tableFitsLds = nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE || nb->numTypes <= c_maxLdsTableTypes;
+    if (combinedPrune)
+    {
+        const float ro = static_cast<float>(nb->listParams.rlistOuter);
+        a.sciOrder = nullptr;
+        a.pruneCjPacked = nb->cjPacked.ptr;
+        a.pruneImaskOuter = nb->imaskOuter.ptr;
+        a.pruneSciCount = nb->sciCount.ptr;
+        a.pruneSciHistogram = nb->sciHistogram.ptr;
+        a.pruneRlist_sq = ro * ro;
+        NBNXM_HIP_CHECK(launchFillI32(nb->sciHistogram.ptr, NBNXM_SCI_HISTOGRAM_SIZE, 0, nb->stream));
+    }
     const int trc = timingBegin(nb, c_timeNonbonded + 2 * (freshList ? 1 : 0) + (computeEnergy ? 1 : 0));
     if (trc != NBNXM_OK)
     {
         return trc;
     }
-    if (isDefaultFlavour(nb->ic) && (tableFitsLds || nb->accumulation == NBNXM_ACCUM_FP64))
+    if (tunedWalk)
     {
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream));
     }
-    else if (nb->accumulation != NBNXM_ACCUM_FP64 && tableFitsLds && !(nb->ic.ljPme && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE))
+    else if (flavouredWalk)
     {
         // Other FP32 flavours: the tuned list walk with the pair interaction of the flavour's class.
         NBNXM_HIP_CHECK(launchNonbonded(a, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->launchConfig, nb->stream, &nb->flavour));
     }
     else
     {
-        // FP64 mode of the other flavours, LJ-PME with a combination-rule kernel
-        // and type tables beyond the LDS staging: the standalone flavour kernel.
         NBNXM_HIP_CHECK(launchNonbondedFlavour(a, nb->flavour, nb->accumulation, nb->vdwType, computeEnergy != 0, nb->stream));
+    }
+    if (combinedPrune)
+    {
+        // The entry sort of the pruned list, inside the kernel's timed region as in the reference.
+        const int src = sortEntriesOfFreshList(nb);
+        if (src != NBNXM_OK)
+        {
+            return src;
+        }
     }
     const int trc2 = timingEnd(nb);
     if (trc2 != NBNXM_OK)

@@ -178,9 +178,16 @@ constexpr int minWavesPerSimd()
 
 // kILds: the i-cluster coordinates and charges are read from per-wave LDS for
 // every cluster pair instead of being held in 32 registers.
-template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false>
+// kPrune: the combined prune and interaction kernel of a fresh list without
+// dynamic pruning (the _prune kernels, nbnxm_cuda_kernel.cuh:145-157,494-501,
+// 680-685,722-741): each cluster pair of the list with no atom pair inside
+// rlist leaves the group's mask, which is written back (and to the outer mask),
+// and the entry's pruned cluster-pair count goes to the histogram bins of the
+// entry sort.
+template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false, bool kPrune = false>
 __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
 {
+    static_assert(!(kPrune && kILds), "the combined prune kernel keeps the i-clusters in registers");
     constexpr bool generic = kFlav != c_flavTuned;
     const int lane = threadIdx.x & 63;
     const int ti = lane & 7;
@@ -391,6 +398,12 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     int pendCj[c_jg];
     unsigned int pendMask = 0u;
 
+    // Combined prune: the current group's mask less its cluster pairs with no
+    // pair inside rlist, and the entry's count of kept cluster pairs.
+    [[maybe_unused]] const float pruneRlist_sq = a.pruneRlist_sq;
+    [[maybe_unused]] unsigned int prunedMask = 0u;
+    [[maybe_unused]] int prunedPairCount = 0;
+
     // One packed group of 4 j-clusters against the 8 i-clusters, instantiated
     // with and without exclusion handling.  In a group with exclusions, only
     // cluster pairs that hold an excluded lane take the interaction-bit path;
@@ -441,6 +454,14 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 const float dy = iv.y - xqj.y;
                 const float dz = iv.z - xqj.z;
                 float r2 = norm2Fixed(dx, dy, dz);
+                if constexpr (kPrune)
+                {
+                    // No pair of the 64 inside rlist: the cluster pair leaves the list (:494-501).
+                    if (__builtin_amdgcn_ballot_w64(r2 < pruneRlist_sq) == 0ull)
+                    {
+                        prunedMask &= ~(1u << bit);
+                    }
+                }
                 // On the diagonal of the central shift only pairs with j > i
                 // interact (nonSelfInteraction, :423,508), whatever the list's
                 // exclusion words say: the lanes with tj <= ti leave the
@@ -698,6 +719,10 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         {
             prefetch(jn);
         }
+        if constexpr (kPrune)
+        {
+            prunedMask = imask;
+        }
         if (exclInd != 0)
         {
             group(cj0, cj1, cj2, cj3, imask, wexcl, std::true_type{});
@@ -706,6 +731,17 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         {
             group(cj0, cj1, cj2, cj3, imask, 0xFFFFFFFFu, std::false_type{});
         }
+        if constexpr (kPrune)
+        {
+            // The group's list words were read (scalar) before this store, and
+            // no other wave reads them.
+            if (lane == 0)
+            {
+                a.pruneCjPacked[jp].imei[0].imask = prunedMask;
+                a.pruneImaskOuter[jp] = prunedMask;
+            }
+            prunedPairCount += __popc(prunedMask);
+        }
         if (jn < s.cjPackedEnd)
         {
             stage();
@@ -713,6 +749,16 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         jp = jn;
     }
     flushPending();
+    if constexpr (kPrune)
+    {
+        // The entry's histogram bin for the entry sort (:722-741); one wave per entry.
+        if (lane == 0)
+        {
+            const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
+            atomicAdd(a.pruneSciHistogram + index, 1);
+            a.pruneSciCount[work] = index;
+        }
+    }
 
     // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
     double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
@@ -771,7 +817,7 @@ constexpr size_t c_maxLdsTypeTableBytes = sizeof(float2) * c_maxLdsTableTypes * 
 // falls back to the run-time class, which reads it from global memory).
 constexpr size_t c_maxLdsEwaldTableBytes = 32768;
 
-template<bool E, int V, int A, int kFlav, bool kILds = false>
+template<bool E, int V, int A, int kFlav, bool kILds = false, bool kPrune = false>
 hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     const int wpb = cfg.wavesPerBlock;
@@ -791,7 +837,7 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
         return hipErrorInvalidValue;
     }
     const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes + gridBytes;
-    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
+    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds, kPrune>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
     return hipGetLastError();
 }
 
@@ -800,9 +846,25 @@ template<int kFlav>
 hipError_t launchTunedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav, int accumulation, int vdwType, bool computeEnergy, const NbLaunchConfig& cfg, hipStream_t stream)
 {
     const bool iLds = iClusterDataInLds(cfg, a.nsci);
+    // A fresh list without dynamic pruning: the combined prune kernel (its
+    // outputs set, entries in list order).
+    const bool prune = a.pruneCjPacked != nullptr;
+    if (prune && (!a.pruneImaskOuter || !a.pruneSciCount || !a.pruneSciHistogram || a.sciOrder))
+    {
+        return hipErrorInvalidValue;
+    }
     auto acc = [&](auto e, auto v) -> hipError_t {
         constexpr bool E = decltype(e)::value;
         constexpr int V = decltype(v)::value;
+        if (prune)
+        {
+            switch (accumulation)
+            {
+                case NBNXM_ACCUM_FLOAT: return launchTunedKernel<E, V, NBNXM_ACCUM_FLOAT, kFlav, false, true>(a, flav, cfg, stream);
+                case NBNXM_ACCUM_FIXED64: return launchTunedKernel<E, V, NBNXM_ACCUM_FIXED64, kFlav, false, true>(a, flav, cfg, stream);
+                default: return hipErrorInvalidValue;
+            }
+        }
         if constexpr (iClusterLdsWaves<E, V, kFlav>() > 0)
         {
             if (iLds)

@@ -38,6 +38,15 @@ struct NbKernelArgs
     unsigned long long* fFixed = nullptr;     // NBNXM_ACCUM_FIXED64 / FP64: 3 per atom
     double* entryPartials = nullptr;          // 8 doubles per entry: Elj, Eel, fsx, fsy, fsz
     int computeFshift = 0;
+
+    // Combined prune of a fresh list (set only for that launch; entries in
+    // list order): masks written back in place and to the outer masks, the
+    // histogram bin of each entry for the entry sort.
+    nbnxm_cj_packed_t* pruneCjPacked = nullptr;
+    unsigned int* pruneImaskOuter = nullptr;
+    int* pruneSciCount = nullptr;
+    int* pruneSciHistogram = nullptr;
+    float pruneRlist_sq = 0.0f;
 };
 
 enum class KernelFlavor : int

@@ -817,6 +817,10 @@ hipError_t launchNonbonded(const NbKernelArgs& a, int accumulation, int vdwType,
     }
     if (accumulation == NBNXM_ACCUM_FP64)
     {
+        if (a.pruneCjPacked)
+        {
+            return hipErrorInvalidValue; // the FP64 mode prunes with the prune kernel
+        }
         return launchFp64Vdw(a, vdwType, computeEnergy, cfg, stream);
     }
     return launchTunedFlavour<c_flavTuned>(a, NbFlavourParams{}, accumulation, vdwType, computeEnergy, cfg, stream);
