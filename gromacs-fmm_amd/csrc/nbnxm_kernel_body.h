@@ -471,9 +471,18 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
                 const unsigned long long keepLanes = (centralShift && (firstCluster + c == cj)) ? upperLanes : ~0ull;
                 const unsigned long long inRangeLanes = __builtin_amdgcn_ballot_w64(r2 < rcoulomb_sq) & keepLanes;
-                const bool inRange = __builtin_amdgcn_inverse_ballot_w64(inRangeLanes);
                 anyInRange |= inRangeLanes;
-                if (inRange)
+                // A wave-uniform test skips the pair work when no lane is in
+                // range; inside it the lanes out of range keep F_invr = 0, so
+                // the force updates after the pair math run on the whole wave
+                // and accumulate in place (no copies of the accumulators for
+                // the lanes a divergent region would leave untouched).
+                if (inRangeLanes == 0ull)
+                {
+                    continue;
+                }
+                float F_invr = 0.0f;
+                if (__builtin_amdgcn_inverse_ballot_w64(inRangeLanes))
                 {
                     float c6, c12;
                     float sigma = 0.0f, epsilon = 0.0f;
@@ -504,9 +513,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     const float inv_r = __builtin_amdgcn_rsqf(r2);
                     const float inv_r2 = inv_r * inv_r;
                     const float qq = iv.w * xqj.w;
-                    float F_invr;
                     // Energy terms of the tuned paths, accumulated after them.
-                    [[maybe_unused]] float eLj6 = 0.0f, eLjArg = 0.0f, eIb = 1.0f, eElArg = 0.0f;
+                    [[maybe_unused]] float eLj6 = 0.0f, eLjArg = 0.0f, eElArg = 0.0f;
                     if constexpr (generic)
                     {
                         const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
@@ -540,8 +548,9 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                         {
                             eLj6 = inv_r6;
                             eLjArg = __builtin_fmaf(c12, inv_r6, -2.0f * c6);
-                            eIb = int_bit;
                             eElArg = inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift;
+                            eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
+                            eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
                         }
                     }
                     else
@@ -569,6 +578,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                             eLj6 = inv_r6;
                             eLjArg = tl - c6;
                             eElArg = __builtin_fmaf(e * R, inv_r, -ewaldShift);
+                            eSumC6 += c6;
+                            eSumC12 += c12;
                         }
                         else
                         {
@@ -580,17 +591,15 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     if constexpr (computeEnergy && !generic)
                     {
                         eLJ12 = __builtin_fmaf(eLj6, eLjArg, eLJ12);
-                        eSumC6 = __builtin_fmaf(eIb, c6, eSumC6);
-                        eSumC12 = __builtin_fmaf(eIb, c12, eSumC12);
                         eElG = __builtin_fmaf(qq, eElArg, eElG);
                     }
-                    fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
-                    fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
-                    fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
-                    fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
-                    fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
-                    fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
                 }
+                fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
+                fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
+                fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
+                fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
+                fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
+                fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
             }
             // j-force: sum over the 8 i-lanes, skipped when no pair was inside
             // the cut-off; lanes ti = 0, 1, 2 keep x, y, z for the deferred
