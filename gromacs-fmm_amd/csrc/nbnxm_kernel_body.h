@@ -417,6 +417,19 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         // Bit b set: some lane of cluster pair b has an excluded (or Newton
         // masked) pair.
         const unsigned int exclBits = hasExcl ? waveOr(~wexcl) : 0u;
+        // Bit b set: cluster pair b lies on the diagonal of the central shift
+        // (i-cluster firstCluster + c is j-cluster cj), at most one per j-cluster.
+        unsigned int diagBits = 0u;
+        if (centralShift)
+        {
+            const int cjs[c_jg] = { cj0, cj1, cj2, cj3 };
+#pragma unroll
+            for (int jm = 0; jm < c_jg; jm++)
+            {
+                const unsigned int d = static_cast<unsigned int>(cjs[jm] - firstCluster);
+                diagBits |= (d < static_cast<unsigned int>(c_ncps)) ? (1u << (jm * c_ncps + d)) : 0u;
+            }
+        }
         float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
         [[maybe_unused]] float eLJg = 0.0f;
 #pragma unroll
@@ -469,7 +482,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 // its exclusion words, the lane mask from list indices, so the
                 // rule costs a scalar AND of masks rather than a select per lane.
                 const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
-                const unsigned long long keepLanes = (centralShift && (firstCluster + c == cj)) ? upperLanes : ~0ull;
+                const unsigned long long keepLanes = ((diagBits >> bit) & 1u) ? upperLanes : ~0ull;
                 const unsigned long long inRangeLanes = __builtin_amdgcn_ballot_w64(r2 < rcoulomb_sq) & keepLanes;
                 anyInRange |= inRangeLanes;
                 // A wave-uniform test skips the pair work when no lane is in
