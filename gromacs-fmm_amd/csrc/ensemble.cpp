@@ -441,12 +441,12 @@ int runSteps(nbnxm_ensemble* e, int mode, int numSteps, float sigma, uint64_t se
 
 // Mean step time of `reps` batched steps after one warm-up step; the minimum
 // over `blocks` such timings (the least disturbed by clock changes).
-int timeBatched(nbnxm_ensemble* e, int reps, double* ms, int blocks = 1)
+int timeBatched(nbnxm_ensemble* e, int reps, double* ms, int blocks = 1, bool computeEnergy = false)
 {
     HipEvent a, b;
     NBNXM_HIP_CHECK(hipEventCreate(&a.e));
     NBNXM_HIP_CHECK(hipEventCreate(&b.e));
-    int rc = runSteps(e, NBNXM_MODE_BATCHED, 1, 0.0f, 1, false, false, 0);
+    int rc = runSteps(e, NBNXM_MODE_BATCHED, 1, 0.0f, 1, computeEnergy, false, 0);
     if (rc != NBNXM_OK)
     {
         return rc;
@@ -455,7 +455,7 @@ int timeBatched(nbnxm_ensemble* e, int reps, double* ms, int blocks = 1)
     for (int k = 0; k < blocks; k++)
     {
         NBNXM_HIP_CHECK(hipEventRecord(a.e, e->stream));
-        rc = runSteps(e, NBNXM_MODE_BATCHED, reps, 0.0f, 1, false, false, 0);
+        rc = runSteps(e, NBNXM_MODE_BATCHED, reps, 0.0f, 1, computeEnergy, false, 0);
         if (rc != NBNXM_OK)
         {
             return rc;
@@ -850,7 +850,7 @@ int nbnxm_ensemble_get_launch_config(nbnxm_ensemble_t* e, int* wavesPerBlock, in
     return NBNXM_OK;
 }
 
-int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int* chosenMaxPairs, int* chosenWavesPerBlock)
+int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int computeEnergy, int* chosenMaxPairs, int* chosenWavesPerBlock)
 {
     if (!e || numBuckets < 1 || (numBuckets > 1 && !bucketEdges))
     {
@@ -870,7 +870,7 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
     const int blocks = 3;
     const double margin = 0.998;
     double bestMs = 0.0;
-    rc = timeBatched(e, reps, &bestMs, blocks);
+    rc = timeBatched(e, reps, &bestMs, blocks, computeEnergy != 0);
     if (rc != NBNXM_OK)
     {
         return rc;
@@ -893,7 +893,7 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
                     return rc;
                 }
                 double ms = 0.0;
-                rc = timeBatched(e, reps, &ms, blocks);
+                rc = timeBatched(e, reps, &ms, blocks, computeEnergy != 0);
                 if (rc != NBNXM_OK)
                 {
                     return rc;
@@ -920,7 +920,7 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
         }
         e->launchConfig.wavesPerBlock = w;
         double ms = 0.0;
-        rc = timeBatched(e, reps, &ms, blocks);
+        rc = timeBatched(e, reps, &ms, blocks, computeEnergy != 0);
         if (rc != NBNXM_OK)
         {
             return rc;
@@ -937,7 +937,7 @@ int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* buck
     e->launchConfig.iClusterInLds = 1 - autoLds;
     {
         double ms = 0.0;
-        rc = timeBatched(e, reps, &ms, blocks);
+        rc = timeBatched(e, reps, &ms, blocks, computeEnergy != 0);
         if (rc != NBNXM_OK)
         {
             return rc;

@@ -192,7 +192,7 @@ SIGNATURES = [
     ("nbnxm_gpu_free", None, [_vp]),
     ("nbnxm_ensemble_create", _c_int, [_c_int, _vp, _vp, _PTR(_vp)]),
     ("nbnxm_ensemble_set_split", _c_int, [_vp, _c_int, _vp, _vp, _c_int]),
-    ("nbnxm_ensemble_autotune", _c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    ("nbnxm_ensemble_autotune", _c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
     ("nbnxm_ensemble_set_i_cluster_in_lds", _c_int, [_vp, _c_int]),
     ("nbnxm_ensemble_get_launch_config", _c_int, [_vp, _vp, _vp]),
     ("nbnxm_ensemble_run", _c_int, [_vp, _c_int, _c_int, ctypes.c_float, ctypes.c_uint64, _c_int, _c_int, _c_int]),

@@ -78,11 +78,14 @@ class Ensemble:
         mp = np.ascontiguousarray(max_pairs, dtype=np.int32)
         check(self._lib.nbnxm_ensemble_set_split(self._h, len(mp), ptr(edges) if edges.size else None, ptr(mp), int(waves_per_block)), "ensemble_set_split")
 
-    def autotune(self, bucket_edges):
+    def autotune(self, bucket_edges, compute_energy=False):
+        """Picks the split per size bucket and the waves per block by timing
+        the batched launch (F+E with compute_energy, else F)."""
         edges = np.ascontiguousarray(bucket_edges, dtype=np.int32)
         chosen = np.zeros(len(edges) + 1, dtype=np.int32)
         w = ctypes.c_int(0)
-        check(self._lib.nbnxm_ensemble_autotune(self._h, len(chosen), ptr(edges) if edges.size else None, ptr(chosen), ctypes.byref(w)), "ensemble_autotune")
+        check(self._lib.nbnxm_ensemble_autotune(self._h, len(chosen), ptr(edges) if edges.size else None,
+                                                int(compute_energy), ptr(chosen), ctypes.byref(w)), "ensemble_autotune")
         return chosen.tolist(), w.value
 
     def set_i_cluster_in_lds(self, mode):

@@ -523,9 +523,10 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
 /* Re-splits the entries of the systems in each size bucket: bucketEdges
  * (numBuckets-1 ascending atom counts) and maxPairs (numBuckets, 0 = no split). */
 int nbnxm_ensemble_set_split(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, const int* maxPairs, int wavesPerBlock);
-/* Times candidate splits per size bucket and keeps the fastest; writes the
+/* Times candidate splits per size bucket and keeps the fastest, timing the
+ * batched launch the caller will run (computeEnergy: F+E, else F); writes the
  * chosen maxPairs per bucket and waves per block when the pointers are set. */
-int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int* chosenMaxPairs, int* chosenWavesPerBlock);
+int nbnxm_ensemble_autotune(nbnxm_ensemble_t* e, int numBuckets, const int* bucketEdges, int computeEnergy, int* chosenMaxPairs, int* chosenWavesPerBlock);
 /* The i-cluster placement of the batched launch (as nbnxm_gpu_set_i_cluster_in_lds;
  * the autotuner sets it).  Per-system launches always choose by entry count. */
 int nbnxm_ensemble_set_i_cluster_in_lds(nbnxm_ensemble_t* e, int mode);

@@ -141,7 +141,7 @@ def test_split_and_autotune_keep_forces(systems, ens):
     announce("autotune keeps forces")
     ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
     ref = [ens.forces(k) for k in range(len(systems))]
-    chosen, wpb = ens.autotune([1000, 3000])
+    chosen, wpb = ens.autotune([1000, 3000], compute_energy=True)
     assert len(chosen) == 3 and 1 <= wpb <= 4
     ens.set_split([1000, 3000], [0, 32, 16], 2)
     ens.run(1, nb.MODE_BATCHED, 0.0, 1, True)
