@@ -26,11 +26,12 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed FP32)
 # The same FP32 vector pipe without packed FP32 instructions, which this build
 # does not use: one wave64 FMA per SIMD every 4 cycles.
 FP32_NONPACKED_TFLOPS = 78.6
-# HBM bytes per batched F+E launch of this workload and schedule: FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy) from two rocprofv3 --pmc
-# runs, profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
+# HBM bytes per batched F+E launch of this workload (no split, 2 waves per
+# block): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device
+# copy) from two rocprofv3 --pmc runs of the current kernel,
+# profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
 # inside the timed run, so the measured figure is carried here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.906e9
+TRAFFIC_BYTES_PER_LAUNCH = 1.907e9
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_VF = 107       # Ewald + LJ force+energy (src/gromacs/gmxlib/nrnb.cpp:90-103)
 RC, RLIST_OUTER, RLIST_INNER, EWALD_RTOL, SIGMA = 1.0, 1.1, 1.04, 1e-5, 0.002
@@ -108,6 +109,35 @@ def water96k_setup(nb, vdw):
     s = nb.systems.config_water_96k()
     search, plist, atoms = nb.build(s, RLIST_OUTER, vdw)
     return s, search, plist, atoms
+
+
+def fresh_list_table(launches):
+    """One JSON line per (F / F+E, path): the first launch on a fresh list of
+    the 96,000-atom water box (list 1.1 nm, fixed point), either the combined
+    prune and interaction kernel (no dynamic pruning: the list is pruned to
+    rlistOuter inside the interaction kernel, then the entries are sorted) or
+    the prune kernel followed by the interaction kernel on the pruned, sorted
+    list (dynamic pruning with rlistInner = rlistOuter).  Wall clock per
+    fresh launch (upload of the list excluded) over `launches` uploads."""
+    import nbnxm_amd as nb
+
+    s, search, plist, atoms = water96k_setup(nb, nb.VDW_CUT_COMB_GEOM)
+    ic = nb.interaction_const(RC, EWALD_RTOL)
+    for energy in (False, True):
+        for combined in (True, False):
+            task = nb.GpuTask(ic, atoms, RLIST_OUTER, None, nb.ACCUM_FIXED64, dynamic_pruning=not combined)
+            times = []
+            for k in range(launches + 2):
+                task.init_pairlist(plist)
+                t0 = time.perf_counter()
+                task.launch(energy, False)
+                task.fetch(want_double=False)
+                if k >= 2:
+                    times.append(time.perf_counter() - t0)
+            print(json.dumps({"fresh_list": "combined prune kernel" if combined else "prune kernel + interaction kernel",
+                              "energies": energy, "atoms": s.n, "ms_per_fresh_launch": 1e3 * float(np.median(times)),
+                              "launches": launches}), flush=True)
+            del task
 
 
 def placement_table(launches):
@@ -235,12 +265,17 @@ def main():
                     help="only print the i-cluster placement table (registers vs LDS) on 96k-3M-atom water boxes and exit")
     ap.add_argument("--md-step", type=int, default=0, metavar="STEPS",
                     help="only time the nonbonded part of an MD step (96k-atom water) and exit")
+    ap.add_argument("--fresh-list", type=int, default=0, metavar="LAUNCHES",
+                    help="only time fresh-list launches, combined prune kernel vs prune + interaction (96k-atom water), and exit")
     args = ap.parse_args()
     if args.flavours:
         flavour_table(args.flavours)
         return
     if args.md_step:
         md_step_table(args.md_step)
+        return
+    if args.fresh_list:
+        fresh_list_table(args.fresh_list)
         return
     if args.placement:
         placement_table(args.placement)
