@@ -412,11 +412,12 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     // Energies are summed in FP32 over the group and added in FP64 per group;
     // the LJ energy is kept as 12*E without its constant shift, plus the
     // sums of C6 and C12 that carry the shift.
-    auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl, auto withExcl) {
+    auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl,
+                     const unsigned int groupExclBits, auto withExcl) {
         constexpr bool hasExcl = decltype(withExcl)::value;
         // Bit b set: some lane of cluster pair b has an excluded (or Newton
-        // masked) pair.
-        const unsigned int exclBits = hasExcl ? waveOr(~wexcl) : 0u;
+        // masked) pair (waveOr(~wexcl) of the group).
+        const unsigned int exclBits = hasExcl ? groupExclBits : 0u;
         // Bit b set: cluster pair b lies on the diagonal of the central shift
         // (i-cluster firstCluster + c is j-cluster cj), at most one per j-cluster.
         unsigned int diagBits = 0u;
@@ -680,13 +681,6 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         pendMask = 0u;
     };
 
-    auto nextNonEmpty = [&](int g) {
-        while (g < s.cjPackedEnd && cjList[g].imei[0].imask == 0u)
-        {
-            g++;
-        }
-        return g;
-    };
     // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
     float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool stageK1 = (stageAtom >> 3) == 1;
@@ -734,58 +728,122 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         __builtin_amdgcn_wave_barrier();
     };
 
-    int jp = nextNonEmpty(s.cjPackedBegin);
-    if (jp < s.cjPackedEnd)
     {
-        prefetch(jp);
-        stage();
-    }
-    while (jp < s.cjPackedEnd)
-    {
-        const unsigned int imask = cjList[jp].imei[0].imask;
-        const int exclInd = cjList[jp].imei[0].excl_ind;
-        const int cj0 = cjList[jp].cj[0];
-        const int cj1 = cjList[jp].cj[1];
-        const int cj2 = cjList[jp].cj[2];
-        const int cj3 = cjList[jp].cj[3];
-        const unsigned int wexcl = stagedExcl;
-        // Atomics of the previous group, then the next group's loads.
-        flushPending();
-        const int jn = nextNonEmpty(jp + 1);
-        if (jn < s.cjPackedEnd)
-        {
-            prefetch(jn);
-        }
-        if constexpr (kPrune)
-        {
-            prunedMask = imask;
-        }
-        if (exclInd != 0)
-        {
-            group(cj0, cj1, cj2, cj3, imask, wexcl, std::true_type{});
-        }
-        else
-        {
-            group(cj0, cj1, cj2, cj3, imask, 0xFFFFFFFFu, std::false_type{});
-        }
-        if constexpr (kPrune)
-        {
-            // The group's list words were read (scalar) before this store, and
-            // no other wave reads them.
-            if (lane == 0)
+        // Two passes over the entry's groups, each a loop with one path, so
+        // the accumulators stay in their registers through it (a loop that
+        // chose between the two group bodies per group kept the i-force
+        // accumulators in two register sets and copied them at every
+        // iteration): pass 0 takes the groups without an exclusion mask,
+        // pass 1 the groups with one.  Pass 0 records where the latter are
+        // (a scalar bit mask of the first 64 groups, a scan beyond), so
+        // pass 1 finds them without reading the list again.  Every group is
+        // evaluated once, in the same order in both placements of the
+        // i-cluster data and in the combined prune kernel.
+        const int begin = s.cjPackedBegin;
+        unsigned long long exclGroups = 0ull; // bit g - begin: group g has an exclusion mask
+        bool exclBeyond = false;              // such a group at begin + 64 or later
+        auto nextPlain = [&](int g) {
+            while (g < s.cjPackedEnd)
             {
-                a.pruneCjPacked[jp].imei[0].imask = prunedMask;
-                a.pruneImaskOuter[jp] = prunedMask;
+                const unsigned int m = cjList[g].imei[0].imask;
+                const int ex = cjList[g].imei[0].excl_ind;
+                if (m != 0u && ex == 0)
+                {
+                    break;
+                }
+                if (m != 0u)
+                {
+                    if (g - begin < 64)
+                    {
+                        exclGroups |= 1ull << (g - begin);
+                    }
+                    else
+                    {
+                        exclBeyond = true;
+                    }
+                }
+                g++;
             }
-            prunedPairCount += __popc(prunedMask);
-        }
-        if (jn < s.cjPackedEnd)
-        {
-            stage();
-        }
-        jp = jn;
+            return g;
+        };
+        auto nextExcl = [&](int g) {
+            if (g - begin < 64)
+            {
+                const unsigned long long m = exclGroups & (~0ull << (g - begin));
+                if (m != 0ull)
+                {
+                    return begin + static_cast<int>(__builtin_ctzll(m));
+                }
+                g = begin + 64;
+            }
+            if (!exclBeyond)
+            {
+                return s.cjPackedEnd;
+            }
+            while (g < s.cjPackedEnd && (cjList[g].imei[0].imask == 0u || cjList[g].imei[0].excl_ind == 0))
+            {
+                g++;
+            }
+            return g;
+        };
+        auto walk = [&](auto passTag) {
+            constexpr bool exclPass = decltype(passTag)::value;
+            auto nextInPass = [&](int g) { return exclPass ? nextExcl(g) : nextPlain(g); };
+            int jp = nextInPass(begin);
+            if (jp < s.cjPackedEnd)
+            {
+                prefetch(jp);
+                stage();
+            }
+            while (jp < s.cjPackedEnd)
+            {
+                const unsigned int imask = cjList[jp].imei[0].imask;
+                const int cj0 = cjList[jp].cj[0];
+                const int cj1 = cjList[jp].cj[1];
+                const int cj2 = cjList[jp].cj[2];
+                const int cj3 = cjList[jp].cj[3];
+                const unsigned int wexcl = stagedExcl;
+                flushPending();
+                const int jn = nextInPass(jp + 1);
+                if (jn < s.cjPackedEnd)
+                {
+                    prefetch(jn);
+                }
+                if constexpr (kPrune)
+                {
+                    prunedMask = imask;
+                }
+                if constexpr (exclPass)
+                {
+                    group(cj0, cj1, cj2, cj3, imask, wexcl, waveOr(~wexcl), std::true_type{});
+                }
+                else
+                {
+                    group(cj0, cj1, cj2, cj3, imask, 0xFFFFFFFFu, 0u, std::false_type{});
+                }
+                if constexpr (kPrune)
+                {
+                    // The group's list words were read (scalar) before this
+                    // store (its mask by the pass scan), and no other wave
+                    // reads them.
+                    if (lane == 0)
+                    {
+                        a.pruneCjPacked[jp].imei[0].imask = prunedMask;
+                        a.pruneImaskOuter[jp] = prunedMask;
+                    }
+                    prunedPairCount += __popc(prunedMask);
+                }
+                if (jn < s.cjPackedEnd)
+                {
+                    stage();
+                }
+                jp = jn;
+            }
+            flushPending();
+        };
+        walk(std::false_type{});
+        walk(std::true_type{});
     }
-    flushPending();
     if constexpr (kPrune)
     {
         // The entry's histogram bin for the entry sort (:722-741); one wave per entry.
