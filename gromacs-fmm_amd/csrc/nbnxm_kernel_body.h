@@ -137,14 +137,16 @@ __host__ __device__ constexpr size_t typeTableLdsBytes(int numTypes)
     return (sizeof(float2) * numTypes * numTypes + 15) / 16 * 16;
 }
 
-// Waves per SIMD an instantiation reaches without spilling once the
-// i-cluster data is in LDS, where that is more than with the data in
-// registers (0: no LDS variant).  4 (128 VGPRs): every F kernel of the
-// compile-time classes, and F+E with a combination rule for the tuned,
-// reaction-field and tabulated-Ewald classes.  3 (168 VGPRs): F+E of the
-// LJ force-switch and LJ-PME classes, which take 172-184 VGPRs (2 waves)
-// with the data in registers.  The other F+E kernels spill 20-92 B per lane
-// at 128 VGPRs and already run 3 waves with the data in registers.
+// Occupancy floor (waves per SIMD) of the instantiations that have a variant
+// with the i-cluster data in LDS (0: none).  With the two-pass list walk the
+// LDS variants allocate 72-101 VGPRs (F) and 85-122 (F+E), 4-7 waves per
+// SIMD, which the floors below do not constrain, except the tuned F+E kernel
+// with a combination rule: 104 VGPRs, held to 96 for a fifth wave; its 8
+// spilled registers are reloaded outside the exclusion-free pass, and the
+// batched ensemble launch measured 1.352 -> 1.236 ms (6 waves: 23 spilled
+// registers, 1.699 ms).  The register variants need no floor (the tuned F+E
+// kernel held to 4 waves, 128 VGPRs, spills 8 registers and measured 2 %
+// slower on a 96,000-atom box).
 template<bool computeEnergy, int vdwType, int kFlav>
 constexpr int iClusterLdsWaves()
 {
@@ -155,6 +157,10 @@ constexpr int iClusterLdsWaves()
     if (!computeEnergy)
     {
         return 4;
+    }
+    if (kFlav == c_flavTuned && vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
+    {
+        return 5;
     }
     if (kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldLjPme)
     {
