@@ -166,8 +166,11 @@ constexpr int iClusterLdsWaves()
     {
         return 3;
     }
-    const bool fits4 = vdwType != NBNXM_VDW_CUT_TYPE_TABLE
-                       && (kFlav == c_flavTuned || kFlav == c_flavRfPotShift || kFlav == c_flavEwaldTabPotShift);
+    // The tabulated-Ewald F+E kernel runs 4 waves per SIMD either way (128 and
+    // 98 VGPRs), so the extra LDS read per cluster pair is all the LDS
+    // variant adds: it measured 6-9 % slower on 384,000- and 3 M-atom boxes,
+    // and the kernel has none.
+    const bool fits4 = vdwType != NBNXM_VDW_CUT_TYPE_TABLE && (kFlav == c_flavTuned || kFlav == c_flavRfPotShift);
     return fits4 ? 4 : 0;
 }
 
