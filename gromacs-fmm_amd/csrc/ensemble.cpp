@@ -301,7 +301,8 @@ NbKernelArgs kernelArgs(nbnxm_ensemble* e)
     return a;
 }
 
-// One evaluation: perturb -> nonbonded -> convert/clear forces -> FP64 reductions -> step++.
+// One evaluation: perturb -> nonbonded -> convert/clear forces, FP64 reductions
+// and step++ (one launch in the batched mode).
 int enqueueStep(nbnxm_ensemble* e, int mode, float sigma, uint64_t seed, bool computeEnergy, hipEvent_t nbStart = nullptr, hipEvent_t nbStop = nullptr)
 {
     const int nsys = static_cast<int>(e->sys.size());
@@ -323,11 +324,11 @@ int enqueueStep(nbnxm_ensemble* e, int mode, float sigma, uint64_t seed, bool co
         {
             NBNXM_HIP_CHECK(hipEventRecord(nbStop, e->stream));
         }
-        if (fixed)
-        {
-            NBNXM_HIP_CHECK(launchConvertForces(e->fFixed.ptr, e->totalSlots, e->fOut.ptr, e->fOutDouble.ptr, e->stream));
-        }
-        NBNXM_HIP_CHECK(launchReduceEnergies(e->partials.ptr, e->entryRanges.ptr, e->entryList.ptr, e->shiftRanges.ptr, e->shiftEntryList.ptr, nsys, e->energies.ptr, e->fshift.ptr, 1, e->stream));
+        // Force conversion, energy reduction and the step counter in one launch.
+        NBNXM_HIP_CHECK(launchFinishStep(fixed ? e->fFixed.ptr : nullptr, e->totalSlots, e->fOut.ptr, e->fOutDouble.ptr, e->partials.ptr, e->entryRanges.ptr,
+                                         e->entryList.ptr, e->shiftRanges.ptr, e->shiftEntryList.ptr, nsys, e->energies.ptr, e->fshift.ptr, 1,
+                                         e->step.ptr, e->stream));
+        return NBNXM_OK;
     }
     else
     {

@@ -278,5 +278,23 @@ hipError_t launchAdvanceStep(unsigned long long* step, hipStream_t stream);
 // Fill kernels (graph-safe clears).
 hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream);
 hipError_t launchFillI32(int* p, size_t n, int value, hipStream_t stream);
+// Convert and clear the fixed-point forces (fFixed may be null: no forces to
+// convert), reduce energies and shift forces per system and advance the step
+// counter (step may be null), in one launch.
+hipError_t launchFinishStep(unsigned long long* fFixed,
+                            int numAtoms,
+                            float* fOut,
+                            double* fOutDouble,
+                            const double* entryPartials,
+                            const int* entryRanges,
+                            const int* entryList,
+                            const int* shiftRanges,
+                            const int* shiftEntryList,
+                            int numSystems,
+                            double* energiesOut,
+                            double* fshiftOut,
+                            int computeFshift,
+                            unsigned long long* step,
+                            hipStream_t stream);
 
 } // namespace nbnxm_amd

@@ -396,13 +396,8 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
     }
 }
 
-__global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* fOut, double* fOutDouble, int clear)
+__device__ __forceinline__ void convertForce(int i, unsigned long long* fFixed, float* fOut, double* fOutDouble, int clear)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3)
-    {
-        return;
-    }
     const long long v = static_cast<long long>(fFixed[i]);
     const double d = static_cast<double>(v) * (1.0 / c_fixedScaleD);
     if (fOut)
@@ -419,19 +414,29 @@ __global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* f
     }
 }
 
-// One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
-__global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partials,
-                                                           const int* entryRanges,
-                                                           const int* entryList,
-                                                           const int* shiftRanges,
-                                                           const int* shiftEntryList,
-                                                           int computeFshift,
-                                                           double* energiesOut,
-                                                           double* fshiftOut)
+__global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* fOut, double* fOutDouble, int clear)
 {
-    const int sys = blockIdx.x / (1 + NBNXM_NUM_SHIFTS);
-    const int slot = blockIdx.x % (1 + NBNXM_NUM_SHIFTS);
-    const int lane = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3)
+    {
+        return;
+    }
+    convertForce(i, fFixed, fOut, fOutDouble, clear);
+}
+
+// One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
+__device__ __forceinline__ void reduceEnergySlot(int sys,
+                                                 int slot,
+                                                 int lane,
+                                                 const double* partials,
+                                                 const int* entryRanges,
+                                                 const int* entryList,
+                                                 const int* shiftRanges,
+                                                 const int* shiftEntryList,
+                                                 int computeFshift,
+                                                 double* energiesOut,
+                                                 double* fshiftOut)
+{
     if (slot == 0)
     {
         double e0 = 0.0, e1 = 0.0;
@@ -472,6 +477,63 @@ __global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partial
         fshiftOut[3 * sidx + 1] = f1;
         fshiftOut[3 * sidx + 2] = f2;
     }
+}
+
+__global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partials,
+                                                           const int* entryRanges,
+                                                           const int* entryList,
+                                                           const int* shiftRanges,
+                                                           const int* shiftEntryList,
+                                                           int computeFshift,
+                                                           double* energiesOut,
+                                                           double* fshiftOut)
+{
+    reduceEnergySlot(blockIdx.x / (1 + NBNXM_NUM_SHIFTS), blockIdx.x % (1 + NBNXM_NUM_SHIFTS), threadIdx.x, partials, entryRanges,
+                     entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+}
+
+// The end of a batched step in one launch: blocks [0, convertBlocks) convert
+// and clear the fixed-point forces (as convertForcesKernel), every wave of the
+// blocks after them reduces one (system, slot) of the energies and shift
+// forces (as reduceEnergiesKernel), and thread 0 of block 0 advances the step
+// counter (as advanceStepKernel; the step's perturbation read it before the
+// nonbonded launch).  One launch instead of three dependent ones.
+__global__ __launch_bounds__(256) void finishStepKernel(unsigned long long* fFixed,
+                                                        int n3,
+                                                        float* fOut,
+                                                        double* fOutDouble,
+                                                        int convertBlocks,
+                                                        const double* partials,
+                                                        const int* entryRanges,
+                                                        const int* entryList,
+                                                        const int* shiftRanges,
+                                                        const int* shiftEntryList,
+                                                        int numSystems,
+                                                        int computeFshift,
+                                                        double* energiesOut,
+                                                        double* fshiftOut,
+                                                        unsigned long long* step)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && step)
+    {
+        *step += 1ull;
+    }
+    if (static_cast<int>(blockIdx.x) < convertBlocks)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n3)
+        {
+            convertForce(i, fFixed, fOut, fOutDouble, 1);
+        }
+        return;
+    }
+    const int pair = (static_cast<int>(blockIdx.x) - convertBlocks) * (blockDim.x / 64) + static_cast<int>(threadIdx.x / 64);
+    if (pair >= numSystems * (1 + NBNXM_NUM_SHIFTS))
+    {
+        return;
+    }
+    reduceEnergySlot(pair / (1 + NBNXM_NUM_SHIFTS), pair % (1 + NBNXM_NUM_SHIFTS), threadIdx.x & 63, partials, entryRanges, entryList,
+                     shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
 }
 
 // Single-sum virial, atom part: per block FP64 partial sums of x (x) f over the
@@ -883,6 +945,35 @@ hipError_t launchReduceEnergies(const double* entryPartials,
         return hipSuccess;
     }
     hipLaunchKernelGGL(reduceEnergiesKernel, dim3(numSystems * (1 + NBNXM_NUM_SHIFTS)), dim3(64), 0, stream, entryPartials, entryRanges, entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    return hipGetLastError();
+}
+
+hipError_t launchFinishStep(unsigned long long* fFixed,
+                            int numAtoms,
+                            float* fOut,
+                            double* fOutDouble,
+                            const double* entryPartials,
+                            const int* entryRanges,
+                            const int* entryList,
+                            const int* shiftRanges,
+                            const int* shiftEntryList,
+                            int numSystems,
+                            double* energiesOut,
+                            double* fshiftOut,
+                            int computeFshift,
+                            unsigned long long* step,
+                            hipStream_t stream)
+{
+    const int n3 = fFixed ? 3 * numAtoms : 0;
+    const int convertBlocks = (n3 + 255) / 256;
+    const int reduceBlocks = (numSystems * (1 + NBNXM_NUM_SHIFTS) + 3) / 4;
+    const int blocks = convertBlocks + reduceBlocks;
+    if (blocks == 0)
+    {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(finishStepKernel, dim3(blocks), dim3(256), 0, stream, fFixed, n3, fOut, fOutDouble, convertBlocks, entryPartials,
+                       entryRanges, entryList, shiftRanges, shiftEntryList, numSystems, computeFshift, energiesOut, fshiftOut, step);
     return hipGetLastError();
 }
 
