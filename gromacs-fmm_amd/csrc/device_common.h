@@ -71,6 +71,19 @@ __device__ __forceinline__ double reduceWaveD(double v)
     return addXorRowsD<true>(v);
 }
 
+// A wave-uniform float computed in the vector unit, moved to a scalar register
+// (a VOP2 or VOP3 instruction takes one scalar operand), so that it holds no
+// vector register through the list walk.
+__device__ __forceinline__ float uniformF(float v)
+{
+    // readfirstlane folds away on a value the compiler knows to be uniform,
+    // which would leave it in a vector register; the empty asm hides that.
+    // The move itself stays the builtin, so the compiler's hazard checks
+    // (wait states after a VALU write of a scalar register) apply to it.
+    asm("" : "+v"(v));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // FP32 force -> int64 fixed point (2^-32 units) in two exact 32-bit halves:
 // hi = floor(v), lo = (v - floor(v)) * 2^32 in [0, 2^32), truncated.  Six
 // VALU operations instead of a generic float-to-int64 conversion; a
@@ -78,7 +91,10 @@ __device__ __forceinline__ double reduceWaveD(double v)
 // accumulation needs.  |v| is clamped to 2^30 kJ/mol/nm.
 __device__ __forceinline__ unsigned long long toFixed(float v)
 {
-    v = __builtin_amdgcn_fmed3f(v, -c_fixedMaxF, c_fixedMaxF);
+    // One med3 whose two bounds are the same scalar register, once negated
+    // (a source modifier): no vector register holds a constant.
+    const float bound = uniformF(c_fixedMaxF);
+    v = __builtin_amdgcn_fmed3f(v, -bound, bound);
     const float hi = __builtin_floorf(v);
     const float lo = (v - hi) * c_fixedScale;
     const unsigned int loBits = static_cast<unsigned int>(lo);

@@ -75,6 +75,37 @@ __device__ __forceinline__ unsigned int waveOr(unsigned int v)
                                      | __builtin_amdgcn_readlane(static_cast<int>(v), 32) | __builtin_amdgcn_readlane(static_cast<int>(v), 48));
 }
 
+// Atomic add to element `index` (a 32-bit lane offset) of a global array at a
+// wave-uniform base, issued with the scalar-base address form (base in scalar
+// registers plus a 32-bit vector offset).  The base passes through an empty
+// asm that holds it in scalar registers; otherwise the compiler folds the lane
+// offset into a 64-bit vector address that stays live (two registers) through
+// the list walk.
+template<typename T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
+template<typename T>
+__device__ __forceinline__ void atomicAddAtLane(T* uniformBase, unsigned int index, T value)
+{
+    GlobalPtr<char> base = (GlobalPtr<char>)(uniformBase);
+    unsigned int byteOffset = index * static_cast<unsigned int>(sizeof(T));
+    asm("" : "+s"(base), "+v"(byteOffset));
+    GlobalPtr<T> p = (GlobalPtr<T>)(base + static_cast<size_t>(byteOffset));
+    __hip_atomic_fetch_add(p, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The lane index (0-63) from the lane counter, opaque to the compiler: a pass
+// of the list walk or the final reduction that takes its lane-dependent
+// indices and LDS addresses from its own read keeps none of them live in a
+// register through the other passes (the compiler would keep one shared copy
+// of every such value alive across the whole kernel).
+__device__ __forceinline__ int laneIdOpaque()
+{
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 // Sum over lanes with the same (lane & 7): xor 8 (row_ror:8), then rows
 // xor 16 and xor 32 with the VALU lane swaps.
 __device__ __forceinline__ float reduceOverStride8(float v)
@@ -137,16 +168,34 @@ __host__ __device__ constexpr size_t typeTableLdsBytes(int numTypes)
     return (sizeof(float2) * numTypes * numTypes + 15) / 16 * 16;
 }
 
+// Per-wave LDS record of 16-byte slots, after the tables in dynamic LDS: the
+// staged group's 32 j-atoms (xq, then the LJ data in .x/.y), the 64 i-atoms'
+// LJ combination parameters (.x/.y) and, with the i-cluster data in LDS, their
+// shifted coordinates and charges times epsfac.  One slot size for all four
+// arrays lets a pass address each side with one lane register (the slot of
+// j-atom tj, of i-atom ti, or of the lane) plus constant offsets.
+constexpr int c_slotJXq = 0;
+constexpr int c_slotJLj = 32;
+constexpr int c_slotILj = 64;
+constexpr int c_slotIXq = 128;
+__host__ __device__ constexpr int waveStageSlots(bool iClusterInLds)
+{
+    return iClusterInLds ? 192 : 128;
+}
+
 // Occupancy floor (waves per SIMD) of the instantiations that have a variant
-// with the i-cluster data in LDS (0: none).  With the two-pass list walk the
-// LDS variants allocate 72-101 VGPRs (F) and 85-122 (F+E), 4-7 waves per
-// SIMD, which the floors below do not constrain, except the tuned F+E kernel
-// with a combination rule: 104 VGPRs, held to 96 for a fifth wave; its 8
-// spilled registers are reloaded outside the exclusion-free pass, and the
-// batched ensemble launch measured 1.352 -> 1.236 ms (6 waves: 23 spilled
-// registers, 1.699 ms).  The register variants need no floor (the tuned F+E
-// kernel held to 4 waves, 128 VGPRs, spills 8 registers and measured 2 %
-// slower on a 96,000-atom box).
+// with the i-cluster data in LDS (0: none).  Each pass of the list walk takes
+// its lane indices from its own lane-counter read, uniform constants sit in
+// scalar registers and the j-force atomics use the scalar-base address form,
+// so the exclusion-free pass of the LDS variants needs 61-75 VGPRs (F) and
+// 70-90 (F+E); the exclusion pass and the final reductions need more.  The
+// tuned F+E kernel with a combination rule allocates 87 VGPRs (5 waves) and
+// is held to 72 for a seventh wave: its spills are stored and reloaded in the
+// exclusion pass only, and the batched ensemble launch measured 1.218 ms
+// (5 waves) -> 1.155 (6) -> 1.139 ms (7).  The other floors do not constrain
+// the allocation.  The register variants need no floor (the tuned F+E kernel
+// held to 4 waves, 128 VGPRs, spilled 8 registers and measured 2 % slower on a
+// 96,000-atom box).
 template<bool computeEnergy, int vdwType, int kFlav>
 constexpr int iClusterLdsWaves()
 {
@@ -160,7 +209,7 @@ constexpr int iClusterLdsWaves()
     }
     if (kFlav == c_flavTuned && vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
     {
-        return 5;
+        return 7;
     }
     if (kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldLjPme)
     {
@@ -194,7 +243,7 @@ constexpr int minWavesPerSimd()
 // and the entry's pruned cluster-pair count goes to the histogram bins of the
 // entry sort.
 template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false, bool kPrune = false>
-__global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock)
+__global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock, int stageOffset)
 {
     static_assert(!(kPrune && kILds), "the combined prune kernel keeps the i-clusters in registers");
     constexpr bool generic = kFlav != c_flavTuned;
@@ -262,8 +311,8 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     const float rcoulomb_sq = pA.x;
     const float epsfac = pA.y;
     const float beta = pA.z;
-    const float beta2 = beta * beta;
-    const float beta3 = beta * beta * beta;
+    const float beta2 = uniformF(beta * beta);
+    const float beta3 = uniformF(beta * beta * beta);
     const float ewaldShift = pA.w;
     const float cpotRep = pB.x;
     const float cpotDisp = pB.y;
@@ -273,12 +322,11 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     // i-atom coordinates (shifted) and charges (times epsfac) of all 8
     // clusters in registers, or with kILds in per-wave LDS; the type-table row
     // offsets in registers; combination parameters in per-wave LDS.
-    __shared__ float2 sLi[4][c_ncps * c_cl];
-    float2* stageLi = sLi[threadIdx.x >> 6];
     static_assert(!kILds || iClusterLdsWaves<computeEnergy, vdwType, kFlav>() > 0, "no LDS variant for this instantiation");
     constexpr bool iInLds = kILds;
-    __shared__ float4 sI[iInLds ? 4 : 1][iInLds ? c_ncps * c_cl : 1];
-    float4* stageI = sI[iInLds ? (threadIdx.x >> 6) : 0];
+    // This wave's LDS record (stageOffset: bytes of the tables before the records).
+    float4* const slots = reinterpret_cast<float4*>(reinterpret_cast<char*>(sDyn) + stageOffset)
+                          + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * waveStageSlots(iInLds);
     float xi[iInLds ? 1 : c_ncps], yi[iInLds ? 1 : c_ncps], zi[iInLds ? 1 : c_ncps], qi[iInLds ? 1 : c_ncps];
     [[maybe_unused]] int tOffI[c_ncps]; // type table: row offset numTypes * type_i, in registers
 #pragma unroll
@@ -288,7 +336,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         const float4 v = loadF4(xqc, ai);
         if constexpr (iInLds)
         {
-            stageI[c * c_cl + ti] = make_float4(v.x + sh.x, v.y + sh.y, v.z + sh.z, v.w * epsfac);
+            slots[c_slotIXq + c * c_cl + ti] = make_float4(v.x + sh.x, v.y + sh.y, v.z + sh.z, v.w * epsfac);
         }
         else
         {
@@ -305,7 +353,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     if constexpr (vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
     {
         const int ai = s.sci * c_ncps * c_cl + lane; // lane = c*8 + ti
-        stageLi[lane] = loadF2(asConst(a.ljComb), ai);
+        *reinterpret_cast<float2*>(slots + c_slotILj + lane) = loadF2(asConst(a.ljComb), ai);
     }
 
     float fciX[c_ncps], fciY[c_ncps], fciZ[c_ncps];
@@ -356,7 +404,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
 #pragma unroll
                 for (int c = 0; c < c_ncps; c++)
                 {
-                    const float qc = iInLds ? stageI[c * c_cl + ti].w : qi[iInLds ? 0 : c];
+                    const float qc = iInLds ? slots[c_slotIXq + c * c_cl + ti].w : qi[iInLds ? 0 : c];
                     q2 += qc * qc;
                 }
                 eElAcc -= q2 / epsfac * beta * static_cast<float>(c_oneSqrtPi);
@@ -378,364 +426,23 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         }
     }
 
-    // Newton / self masking on the diagonal of the central shift (:423): on a
-    // cluster pair with ci == cj only lanes with tj > ti interact; their lane
-    // mask (a constant of the layout) as a wave-uniform value.
-    const unsigned long long upperLanes = __builtin_amdgcn_ballot_w64(tj > ti);
-    const bool lowerLane = ti < 4;
-    const bool xLane = ti < 2;
-    const bool yLane = (ti >> 1) == 1;
     const int firstCluster = __builtin_amdgcn_readfirstlane(s.sci * c_ncps);
     // Constants folded for the exclusion-free path.
-    const float negBeta2Log2e = -beta2 * 1.44269504088896340736f; // exp(-b^2 r^2) = exp2(r^2 * this)
-    const float betaT = c_erfcxPolyScale * beta;                   // t = 1/(1 + 0.4 beta r)
-    const float twoBetaSqrtPi = beta * static_cast<float>(2.0 * c_oneSqrtPi);
+    const float negBeta2Log2e = uniformF(-beta2 * 1.44269504088896340736f); // exp(-b^2 r^2) = exp2(r^2 * this)
+    const float betaT = uniformF(c_erfcxPolyScale * beta);                   // t = 1/(1 + 0.4 beta r)
+    const float twoBetaSqrtPi = uniformF(beta * static_cast<float>(2.0 * c_oneSqrtPi));
+    const float twoCpotDisp = uniformF(2.0f * cpotDisp);
 
-    // Per-wave LDS staging of one packed group: 32 j-atoms' xq and LJ data.
-    // The next group is fetched into registers (one coalesced load per lane)
+    // The packed group's 32 j-atoms are staged in the wave's LDS record.  The
+    // next group is fetched into registers (one coalesced load per lane)
     // before the current group's force atomics are issued, so waiting for
     // j-data never waits for those atomics (vmcnt counts both, in order).
-    __shared__ float4 sXq[4][c_jg * c_cl];
-    __shared__ float2 sLj[4][c_jg * c_cl];
-    float4* stageXq = sXq[threadIdx.x >> 6];
-    float2* stageLj = sLj[threadIdx.x >> 6];
-    const int stageAtom = lane & 31; // j-cluster stageAtom/8, atom stageAtom%8
-
-    // Deferred j-force atomics: the reduced force component of lane ti < 3
-    // for each j-cluster of the group, issued at the start of the next group.
-    float pendV[c_jg];
-    int pendCj[c_jg];
-    unsigned int pendMask = 0u;
 
     // Combined prune: the current group's mask less its cluster pairs with no
     // pair inside rlist, and the entry's count of kept cluster pairs.
     [[maybe_unused]] const float pruneRlist_sq = a.pruneRlist_sq;
     [[maybe_unused]] unsigned int prunedMask = 0u;
     [[maybe_unused]] int prunedPairCount = 0;
-
-    // One packed group of 4 j-clusters against the 8 i-clusters, instantiated
-    // with and without exclusion handling.  In a group with exclusions, only
-    // cluster pairs that hold an excluded lane take the interaction-bit path;
-    // the others, like every pair of an exclusion-free group, take the erfc
-    // path, where one exp(-z^2) serves both Coulomb force and energy.
-    // Energies are summed in FP32 over the group and added in FP64 per group;
-    // the LJ energy is kept as 12*E without its constant shift, plus the
-    // sums of C6 and C12 that carry the shift.
-    auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl,
-                     const unsigned int groupExclBits, auto withExcl) {
-        constexpr bool hasExcl = decltype(withExcl)::value;
-        // Bit b set: some lane of cluster pair b has an excluded (or Newton
-        // masked) pair (waveOr(~wexcl) of the group).
-        const unsigned int exclBits = hasExcl ? groupExclBits : 0u;
-        // Bit b set: cluster pair b lies on the diagonal of the central shift
-        // (i-cluster firstCluster + c is j-cluster cj), at most one per j-cluster.
-        unsigned int diagBits = 0u;
-        if (centralShift)
-        {
-            const int cjs[c_jg] = { cj0, cj1, cj2, cj3 };
-#pragma unroll
-            for (int jm = 0; jm < c_jg; jm++)
-            {
-                const unsigned int d = static_cast<unsigned int>(cjs[jm] - firstCluster);
-                diagBits |= (d < static_cast<unsigned int>(c_ncps)) ? (1u << (jm * c_ncps + d)) : 0u;
-            }
-        }
-        float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
-        [[maybe_unused]] float eLJg = 0.0f;
-#pragma unroll
-        for (int jm = 0; jm < c_jg; jm++)
-        {
-            if (!(imask & (0xFFu << (jm * c_ncps))))
-            {
-                continue;
-            }
-            const int cj = (jm == 0) ? cj0 : (jm == 1) ? cj1 : (jm == 2) ? cj2 : cj3;
-            const float4 xqj = stageXq[jm * c_cl + tj];
-            const float2 ljStaged = stageLj[jm * c_cl + tj];
-            const int typej = __float_as_int(ljStaged.x);
-            const float2 ljcj = ljStaged;
-            float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
-            unsigned long long anyInRange = 0ull;
-#pragma unroll
-            for (int c = 0; c < c_ncps; c++)
-            {
-                const int bit = jm * c_ncps + c;
-                if (!(imask & (1u << bit)))
-                {
-                    continue;
-                }
-                float4 iv;
-                if constexpr (iInLds)
-                {
-                    iv = stageI[c * c_cl + ti];
-                }
-                else
-                {
-                    iv = make_float4(xi[c], yi[c], zi[c], qi[c]);
-                }
-                const float dx = iv.x - xqj.x;
-                const float dy = iv.y - xqj.y;
-                const float dz = iv.z - xqj.z;
-                float r2 = norm2Fixed(dx, dy, dz);
-                if constexpr (kPrune)
-                {
-                    // No pair of the 64 inside rlist: the cluster pair leaves the list (:494-501).
-                    if (__builtin_amdgcn_ballot_w64(r2 < pruneRlist_sq) == 0ull)
-                    {
-                        prunedMask &= ~(1u << bit);
-                    }
-                }
-                // On the diagonal of the central shift only pairs with j > i
-                // interact (nonSelfInteraction, :423,508), whatever the list's
-                // exclusion words say: the lanes with tj <= ti leave the
-                // in-range mask.  Wave-uniform: exclHere from the group's OR of
-                // its exclusion words, the lane mask from list indices, so the
-                // rule costs a scalar AND of masks rather than a select per lane.
-                const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
-                const unsigned long long keepLanes = ((diagBits >> bit) & 1u) ? upperLanes : ~0ull;
-                const unsigned long long inRangeLanes = __builtin_amdgcn_ballot_w64(r2 < rcoulomb_sq) & keepLanes;
-                anyInRange |= inRangeLanes;
-                // With the i-cluster data in LDS: a wave-uniform test skips
-                // the pair work when no lane is in range; inside it the lanes
-                // out of range keep F_invr = 0, so the force updates after the
-                // pair math run on the whole wave and accumulate in place (no
-                // copies of the accumulators for the lanes a divergent region
-                // would leave untouched).  With the data in registers the
-                // updates stay in the divergent region, which measured faster
-                // there (3-4 % per launch on a 96,000-atom box), while the
-                // form above took 4.5 % off the LDS variant.
-                constexpr bool forcesAfterPairMath = iInLds;
-                if (forcesAfterPairMath && inRangeLanes == 0ull)
-                {
-                    continue;
-                }
-                float F_invr = 0.0f;
-                auto accumulateForces = [&]() {
-                    fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
-                    fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
-                    fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
-                    fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
-                    fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
-                    fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
-                };
-                if (__builtin_amdgcn_inverse_ballot_w64(inRangeLanes))
-                {
-                    float c6, c12;
-                    float sigma = 0.0f, epsilon = 0.0f;
-                    if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
-                    {
-                        // One LDS read: the i row offset is a register.
-                        const float2 p = sNbfp[tOffI[c] + typej];
-                        c6 = p.x;
-                        c12 = p.y;
-                    }
-                    else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
-                    {
-                        const float2 ljci = stageLi[c * c_cl + ti];
-                        c6 = ljci.x * ljcj.x;
-                        c12 = ljci.y * ljcj.y;
-                    }
-                    else
-                    {
-                        const float2 ljci = stageLi[c * c_cl + ti];
-                        sigma = ljci.x + ljcj.x;
-                        epsilon = ljci.y * ljcj.y;
-                        const float sigma2 = sigma * sigma;
-                        const float sigma6 = sigma2 * sigma2 * sigma2;
-                        c6 = epsilon * sigma6;
-                        c12 = c6 * sigma6;
-                    }
-                    r2 = __builtin_amdgcn_fmed3f(r2, NBNXM_MIN_DISTANCE_SQUARED, rcoulomb_sq);
-                    const float inv_r = __builtin_amdgcn_rsqf(r2);
-                    const float inv_r2 = inv_r * inv_r;
-                    const float qq = iv.w * xqj.w;
-                    // Energy terms of the tuned paths, accumulated after them.
-                    [[maybe_unused]] float eLj6 = 0.0f, eLjArg = 0.0f, eElArg = 0.0f;
-                    if constexpr (generic)
-                    {
-                        const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
-                        float c6grid = 0.0f;
-                        if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE && !ldsEwaldTable)
-                        {
-                            c6grid = flavLjPme<kFlav>(flav) ? sGrid[tOffI[c] + typej] : 0.0f;
-                        }
-                        F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, c6grid, eLJg, eElG,
-                                                                        ldsEwaldTable ? sEwald : nullptr, computeEnergy && !(hasExcl && exclHere));
-                    }
-                    else if (hasExcl && exclHere)
-                    {
-                        // The reference form with the interaction bit (:548-659).
-                        const float int_bit = static_cast<float>((wexcl >> bit) & 1u);
-                        const float inv_r6 = int_bit * (inv_r2 * inv_r2 * inv_r2);
-                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
-                        {
-                            const float sig_r = sigma * inv_r;
-                            const float sig_r2 = sig_r * sig_r;
-                            const float sig_r6 = int_bit * (sig_r2 * sig_r2 * sig_r2);
-                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
-                        }
-                        else
-                        {
-                            F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
-                        }
-                        const float z2 = beta2 * r2;
-                        F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(z2) * beta3);
-                        if constexpr (computeEnergy)
-                        {
-                            eLj6 = inv_r6;
-                            eLjArg = __builtin_fmaf(c12, inv_r6, -2.0f * c6);
-                            eElArg = inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift;
-                            eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
-                            eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
-                        }
-                    }
-                    else
-                    {
-                        const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                        if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
-                        {
-                            const float sig_r = sigma * inv_r;
-                            const float sig_r2 = sig_r * sig_r;
-                            const float sig_r6 = sig_r2 * sig_r2 * sig_r2;
-                            F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2 + qq * (inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
-                        }
-                        else if constexpr (computeEnergy)
-                        {
-                            // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2, V/qq = e R / r - sh_ewald,
-                            // with e = exp(-beta^2 r^2) and erfc(beta r) = e R.
-                            const float r = r2 * inv_r;
-                            const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(betaT, r, 1.0f));
-                            const float R = erfcxPolyT(t);
-                            const float e = __builtin_amdgcn_exp2f(r2 * negBeta2Log2e);
-                            const float w = qq * e;
-                            const float u = __builtin_fmaf(R, inv_r, twoBetaSqrtPi);
-                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
-                            F_invr = __builtin_fmaf(inv_r6, tl, w * u) * inv_r2;
-                            eLj6 = inv_r6;
-                            eLjArg = tl - c6;
-                            eElArg = __builtin_fmaf(e * R, inv_r, -ewaldShift);
-                            eSumC6 += c6;
-                            eSumC12 += c12;
-                        }
-                        else
-                        {
-                            const float tl = __builtin_fmaf(c12, inv_r6, -c6);
-                            const float coul = qq * __builtin_fmaf(pmecorrF(beta2 * r2), beta3, inv_r2 * inv_r);
-                            F_invr = __builtin_fmaf(inv_r6 * tl, inv_r2, coul);
-                        }
-                    }
-                    if constexpr (computeEnergy && !generic)
-                    {
-                        eLJ12 = __builtin_fmaf(eLj6, eLjArg, eLJ12);
-                        eElG = __builtin_fmaf(qq, eElArg, eElG);
-                    }
-                    if constexpr (!forcesAfterPairMath)
-                    {
-                        accumulateForces();
-                    }
-                }
-                if constexpr (forcesAfterPairMath)
-                {
-                    accumulateForces();
-                }
-            }
-            // j-force: sum over the 8 i-lanes, skipped when no pair was inside
-            // the cut-off; lanes ti = 0, 1, 2 keep x, y, z for the deferred
-            // atomic (consecutive lanes on consecutive addresses coalesce).
-            if (anyInRange != 0ull)
-            {
-                pendV[jm] = reduceScatterOver8Lanes(fcjX, fcjY, fcjZ, lowerLane, xLane, yLane);
-                pendCj[jm] = cj;
-                pendMask |= 1u << jm;
-            }
-        }
-        if constexpr (computeEnergy && generic)
-        {
-            eLJAcc += static_cast<EAcc>(eLJg);
-            eElAcc += static_cast<EAcc>(eElG);
-        }
-        else if constexpr (computeEnergy)
-        {
-            const float eLJ = (eLJ12 + cpotRep * eSumC12 - 2.0f * cpotDisp * eSumC6) * (1.0f / 12.0f);
-            eLJAcc += static_cast<EAcc>(eLJ);
-            eElAcc += static_cast<EAcc>(eElG);
-        }
-    };
-
-    // Lanes that issue the j-force atomics (ti = 0, 1, 2 for x, y, z) and the
-    // lane part of the address: component ti of j-atom tj of a cluster.
-    const bool jForceLane = (ti & 1) == 0 && ti < 6;
-    const int jForceLaneOffset = 3 * tj + (ti >> 1);
-    auto flushPending = [&]() {
-#pragma unroll
-        for (int jm = 0; jm < c_jg; jm++)
-        {
-            if ((pendMask >> jm) & 1u)
-            {
-                if (jForceLane)
-                {
-                    // Uniform cluster base (scalar) plus a 32-bit lane offset.
-                    const size_t base = static_cast<size_t>(pendCj[jm]) * (3 * c_cl);
-                    if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
-                    {
-                        atomicAdd(a.fFloat + base + jForceLaneOffset, pendV[jm]);
-                    }
-                    else
-                    {
-                        atomicAdd(a.fFixed + base + jForceLaneOffset, toFixed(pendV[jm]));
-                    }
-                }
-            }
-        }
-        pendMask = 0u;
-    };
-
-    // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
-    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const bool stageK1 = (stageAtom >> 3) == 1;
-    const bool stageK2 = (stageAtom >> 3) == 2;
-    const bool stageK3 = (stageAtom >> 3) == 3;
-    unsigned int pfExcl = 0xFFFFFFFFu;
-    auto prefetch = [&](int g) {
-        // Scalar loads first, then a per-lane select (a select of the
-        // addresses would become a vector load whose wait also waits for the
-        // in-flight force atomics).
-        const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
-        const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
-        const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
-        const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
-        const int cjl = stageK1 ? g1 : (stageK2 ? g2 : (stageK3 ? g3 : g0));
-        const int atom = cjl * c_cl + (stageAtom & 7);
-        if (lane < 32)
-        {
-            pf = loadF4(xqc, atom);
-        }
-        else if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
-        {
-            pf = make_float4(__int_as_float(asConst(a.atomTypes)[atom]), 0.0f, 0.0f, 0.0f);
-        }
-        else
-        {
-            const float2 lj = loadF2(asConst(a.ljComb), atom);
-            pf = make_float4(lj.x, lj.y, 0.0f, 0.0f);
-        }
-        const int ex = cjList[g].imei[0].excl_ind;
-        pfExcl = (ex != 0) ? asConst(a.excl)[ex].pair[lane] : 0xFFFFFFFFu;
-    };
-    unsigned int stagedExcl = 0xFFFFFFFFu;
-    auto stage = [&]() {
-        __builtin_amdgcn_wave_barrier();
-        if (lane < 32)
-        {
-            stageXq[stageAtom] = pf;
-        }
-        else
-        {
-            stageLj[stageAtom] = make_float2(pf.x, pf.y);
-        }
-        stagedExcl = pfExcl;
-        __builtin_amdgcn_wave_barrier();
-    };
 
     {
         // Two passes over the entry's groups, each a loop with one path, so
@@ -797,6 +504,353 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         };
         auto walk = [&](auto passTag) {
             constexpr bool exclPass = decltype(passTag)::value;
+            // Lane indices of this pass from its own read of the lane counter:
+            // no lane-dependent index or LDS address stays live in a register
+            // through the other pass and the final reductions.
+            const int lane = laneIdOpaque();
+            const int ti = lane & 7;
+            const int tj = lane >> 3;
+            // Newton / self masking on the diagonal of the central shift (:423): on a
+            // cluster pair with ci == cj only lanes with tj > ti interact; their lane
+            // mask (a constant of the layout) as a wave-uniform value.
+            const unsigned long long upperLanes = __builtin_amdgcn_ballot_w64(tj > ti);
+            const bool lowerLane = ti < 4;
+            const bool xLane = ti < 2;
+            const bool yLane = (ti >> 1) == 1;
+            const int stageAtom = lane & 31; // j-cluster stageAtom/8, atom stageAtom%8
+            const float4* const jSlot = slots + tj; // j-atom tj's slots
+            const float4* const iSlot = slots + ti; // i-atom ti's slots
+
+            // Deferred j-force atomics: the reduced force component of lane ti < 3
+            // for each j-cluster of the group, issued at the start of the next group.
+            float pendV[c_jg];
+            int pendCj[c_jg];
+            unsigned int pendMask = 0u;
+
+            // One packed group of 4 j-clusters against the 8 i-clusters, instantiated
+            // with and without exclusion handling.  In a group with exclusions, only
+            // cluster pairs that hold an excluded lane take the interaction-bit path;
+            // the others, like every pair of an exclusion-free group, take the erfc
+            // path, where one exp(-z^2) serves both Coulomb force and energy.
+            // Energies are summed in FP32 over the group and added in FP64 per group;
+            // the LJ energy is kept as 12*E without its constant shift, plus the
+            // sums of C6 and C12 that carry the shift.
+            auto group = [&](const int cj0, const int cj1, const int cj2, const int cj3, const unsigned int imask, const unsigned int wexcl,
+                             const unsigned int groupExclBits, auto withExcl) {
+                constexpr bool hasExcl = decltype(withExcl)::value;
+                // Bit b set: some lane of cluster pair b has an excluded (or Newton
+                // masked) pair (waveOr(~wexcl) of the group).
+                const unsigned int exclBits = hasExcl ? groupExclBits : 0u;
+                // Bit b set: cluster pair b lies on the diagonal of the central shift
+                // (i-cluster firstCluster + c is j-cluster cj), at most one per j-cluster.
+                unsigned int diagBits = 0u;
+                if (centralShift)
+                {
+                    const int cjs[c_jg] = { cj0, cj1, cj2, cj3 };
+#pragma unroll
+                    for (int jm = 0; jm < c_jg; jm++)
+                    {
+                        const unsigned int d = static_cast<unsigned int>(cjs[jm] - firstCluster);
+                        diagBits |= (d < static_cast<unsigned int>(c_ncps)) ? (1u << (jm * c_ncps + d)) : 0u;
+                    }
+                }
+                float eLJ12 = 0.0f, eSumC6 = 0.0f, eSumC12 = 0.0f, eElG = 0.0f;
+                [[maybe_unused]] float eLJg = 0.0f;
+#pragma unroll
+                for (int jm = 0; jm < c_jg; jm++)
+                {
+                    if (!(imask & (0xFFu << (jm * c_ncps))))
+                    {
+                        continue;
+                    }
+                    const int cj = (jm == 0) ? cj0 : (jm == 1) ? cj1 : (jm == 2) ? cj2 : cj3;
+                    const float4 xqj = jSlot[c_slotJXq + jm * c_cl];
+                    const float2 ljStaged = *reinterpret_cast<const float2*>(jSlot + c_slotJLj + jm * c_cl);
+                    const int typej = __float_as_int(ljStaged.x);
+                    const float2 ljcj = ljStaged;
+                    float fcjX = 0.0f, fcjY = 0.0f, fcjZ = 0.0f;
+                    unsigned long long anyInRange = 0ull;
+#pragma unroll
+                    for (int c = 0; c < c_ncps; c++)
+                    {
+                        const int bit = jm * c_ncps + c;
+                        if (!(imask & (1u << bit)))
+                        {
+                            continue;
+                        }
+                        float4 iv;
+                        if constexpr (iInLds)
+                        {
+                            iv = iSlot[c_slotIXq + c * c_cl];
+                        }
+                        else
+                        {
+                            iv = make_float4(xi[c], yi[c], zi[c], qi[c]);
+                        }
+                        const float dx = iv.x - xqj.x;
+                        const float dy = iv.y - xqj.y;
+                        const float dz = iv.z - xqj.z;
+                        float r2 = norm2Fixed(dx, dy, dz);
+                        if constexpr (kPrune)
+                        {
+                            // No pair of the 64 inside rlist: the cluster pair leaves the list (:494-501).
+                            if (__builtin_amdgcn_ballot_w64(r2 < pruneRlist_sq) == 0ull)
+                            {
+                                prunedMask &= ~(1u << bit);
+                            }
+                        }
+                        // On the diagonal of the central shift only pairs with j > i
+                        // interact (nonSelfInteraction, :423,508), whatever the list's
+                        // exclusion words say: the lanes with tj <= ti leave the
+                        // in-range mask.  Wave-uniform: exclHere from the group's OR of
+                        // its exclusion words, the lane mask from list indices, so the
+                        // rule costs a scalar AND of masks rather than a select per lane.
+                        const bool exclHere = hasExcl && ((exclBits >> bit) & 1u);
+                        const unsigned long long keepLanes = ((diagBits >> bit) & 1u) ? upperLanes : ~0ull;
+                        const unsigned long long inRangeLanes = __builtin_amdgcn_ballot_w64(r2 < rcoulomb_sq) & keepLanes;
+                        anyInRange |= inRangeLanes;
+                        // With the i-cluster data in LDS: a wave-uniform test skips
+                        // the pair work when no lane is in range; inside it the lanes
+                        // out of range keep F_invr = 0, so the force updates after the
+                        // pair math run on the whole wave and accumulate in place (no
+                        // copies of the accumulators for the lanes a divergent region
+                        // would leave untouched).  With the data in registers the
+                        // updates stay in the divergent region, which measured faster
+                        // there (3-4 % per launch on a 96,000-atom box), while the
+                        // form above took 4.5 % off the LDS variant.
+                        constexpr bool forcesAfterPairMath = iInLds;
+                        if (forcesAfterPairMath && inRangeLanes == 0ull)
+                        {
+                            continue;
+                        }
+                        float F_invr = 0.0f;
+                        auto accumulateForces = [&]() {
+                            fcjX = __builtin_fmaf(-dx, F_invr, fcjX);
+                            fcjY = __builtin_fmaf(-dy, F_invr, fcjY);
+                            fcjZ = __builtin_fmaf(-dz, F_invr, fcjZ);
+                            fciX[c] = __builtin_fmaf(dx, F_invr, fciX[c]);
+                            fciY[c] = __builtin_fmaf(dy, F_invr, fciY[c]);
+                            fciZ[c] = __builtin_fmaf(dz, F_invr, fciZ[c]);
+                        };
+                        if (__builtin_amdgcn_inverse_ballot_w64(inRangeLanes))
+                        {
+                            float c6, c12;
+                            float sigma = 0.0f, epsilon = 0.0f;
+                            if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                            {
+                                // One LDS read: the i row offset is a register.
+                                const float2 p = sNbfp[tOffI[c] + typej];
+                                c6 = p.x;
+                                c12 = p.y;
+                            }
+                            else if constexpr (vdwType == NBNXM_VDW_CUT_COMB_GEOM)
+                            {
+                                const float2 ljci = *reinterpret_cast<const float2*>(iSlot + c_slotILj + c * c_cl);
+                                c6 = ljci.x * ljcj.x;
+                                c12 = ljci.y * ljcj.y;
+                            }
+                            else
+                            {
+                                const float2 ljci = *reinterpret_cast<const float2*>(iSlot + c_slotILj + c * c_cl);
+                                sigma = ljci.x + ljcj.x;
+                                epsilon = ljci.y * ljcj.y;
+                                const float sigma2 = sigma * sigma;
+                                const float sigma6 = sigma2 * sigma2 * sigma2;
+                                c6 = epsilon * sigma6;
+                                c12 = c6 * sigma6;
+                            }
+                            // Only lanes inside the cut-off get here: the lower clamp
+                            // (:536) is all that applies.  r2 >= 0, so the float order
+                            // is the order of the bit patterns: one integer max with a
+                            // literal operand (a float max would first quiet NaNs).
+                            r2 = __int_as_float(max(__float_as_int(r2), __float_as_int(NBNXM_MIN_DISTANCE_SQUARED)));
+                            const float inv_r = __builtin_amdgcn_rsqf(r2);
+                            const float inv_r2 = inv_r * inv_r;
+                            const float qq = iv.w * xqj.w;
+                            // Energy terms of the tuned paths, accumulated after them.
+                            [[maybe_unused]] float eLj6 = 0.0f, eLjArg = 0.0f, eElArg = 0.0f;
+                            if constexpr (generic)
+                            {
+                                const float int_bit = (hasExcl && exclHere) ? static_cast<float>((wexcl >> bit) & 1u) : 1.0f;
+                                float c6grid = 0.0f;
+                                if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE && !ldsEwaldTable)
+                                {
+                                    c6grid = flavLjPme<kFlav>(flav) ? sGrid[tOffI[c] + typej] : 0.0f;
+                                }
+                                F_invr = pairForce<float, computeEnergy, kFlav>(flav, fk, r2, int_bit, qq, c6, c12, c6grid, eLJg, eElG,
+                                                                                ldsEwaldTable ? sEwald : nullptr, computeEnergy && !(hasExcl && exclHere));
+                            }
+                            else if (hasExcl && exclHere)
+                            {
+                                // The reference form with the interaction bit (:548-659).
+                                const float int_bit = static_cast<float>((wexcl >> bit) & 1u);
+                                const float inv_r6 = int_bit * (inv_r2 * inv_r2 * inv_r2);
+                                if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                                {
+                                    const float sig_r = sigma * inv_r;
+                                    const float sig_r2 = sig_r * sig_r;
+                                    const float sig_r6 = int_bit * (sig_r2 * sig_r2 * sig_r2);
+                                    F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2;
+                                }
+                                else
+                                {
+                                    F_invr = inv_r6 * (c12 * inv_r6 - c6) * inv_r2;
+                                }
+                                const float z2 = beta2 * r2;
+                                F_invr += qq * (int_bit * inv_r2 * inv_r + pmecorrF(z2) * beta3);
+                                if constexpr (computeEnergy)
+                                {
+                                    eLj6 = inv_r6;
+                                    eLjArg = __builtin_fmaf(c12, inv_r6, -2.0f * c6);
+                                    eElArg = inv_r * (int_bit - erff(r2 * inv_r * beta)) - int_bit * ewaldShift;
+                                    eSumC6 = __builtin_fmaf(int_bit, c6, eSumC6);
+                                    eSumC12 = __builtin_fmaf(int_bit, c12, eSumC12);
+                                }
+                            }
+                            else
+                            {
+                                const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                                if constexpr (vdwType == NBNXM_VDW_CUT_COMB_LB && !computeEnergy)
+                                {
+                                    const float sig_r = sigma * inv_r;
+                                    const float sig_r2 = sig_r * sig_r;
+                                    const float sig_r6 = sig_r2 * sig_r2 * sig_r2;
+                                    F_invr = epsilon * sig_r6 * (sig_r6 - 1.0f) * inv_r2 + qq * (inv_r2 * inv_r + pmecorrF(beta2 * r2) * beta3);
+                                }
+                                else if constexpr (computeEnergy)
+                                {
+                                    // F/(qq r) = e (R/r + 2 beta/sqrt(pi)) / r^2, V/qq = e R / r - sh_ewald,
+                                    // with e = exp(-beta^2 r^2) and erfc(beta r) = e R.
+                                    const float r = r2 * inv_r;
+                                    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(betaT, r, 1.0f));
+                                    const float R = erfcxPolyT(t);
+                                    const float e = __builtin_amdgcn_exp2f(r2 * negBeta2Log2e);
+                                    const float w = qq * e;
+                                    const float u = __builtin_fmaf(R, inv_r, twoBetaSqrtPi);
+                                    const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                                    F_invr = __builtin_fmaf(inv_r6, tl, w * u) * inv_r2;
+                                    eLj6 = inv_r6;
+                                    eLjArg = tl - c6;
+                                    eElArg = __builtin_fmaf(e * R, inv_r, -ewaldShift);
+                                    eSumC6 += c6;
+                                    eSumC12 += c12;
+                                }
+                                else
+                                {
+                                    const float tl = __builtin_fmaf(c12, inv_r6, -c6);
+                                    const float coul = qq * __builtin_fmaf(pmecorrF(beta2 * r2), beta3, inv_r2 * inv_r);
+                                    F_invr = __builtin_fmaf(inv_r6 * tl, inv_r2, coul);
+                                }
+                            }
+                            if constexpr (computeEnergy && !generic)
+                            {
+                                eLJ12 = __builtin_fmaf(eLj6, eLjArg, eLJ12);
+                                eElG = __builtin_fmaf(qq, eElArg, eElG);
+                            }
+                            if constexpr (!forcesAfterPairMath)
+                            {
+                                accumulateForces();
+                            }
+                        }
+                        if constexpr (forcesAfterPairMath)
+                        {
+                            accumulateForces();
+                        }
+                    }
+                    // j-force: sum over the 8 i-lanes, skipped when no pair was inside
+                    // the cut-off; lanes ti = 0, 1, 2 keep x, y, z for the deferred
+                    // atomic (consecutive lanes on consecutive addresses coalesce).
+                    if (anyInRange != 0ull)
+                    {
+                        pendV[jm] = reduceScatterOver8Lanes(fcjX, fcjY, fcjZ, lowerLane, xLane, yLane);
+                        pendCj[jm] = cj;
+                        pendMask |= 1u << jm;
+                    }
+                }
+                if constexpr (computeEnergy && generic)
+                {
+                    eLJAcc += static_cast<EAcc>(eLJg);
+                    eElAcc += static_cast<EAcc>(eElG);
+                }
+                else if constexpr (computeEnergy)
+                {
+                    const float eLJ = (eLJ12 + cpotRep * eSumC12 - twoCpotDisp * eSumC6) * (1.0f / 12.0f);
+                    eLJAcc += static_cast<EAcc>(eLJ);
+                    eElAcc += static_cast<EAcc>(eElG);
+                }
+            };
+
+            // Lanes that issue the j-force atomics (ti = 0, 1, 2 for x, y, z) and the
+            // lane part of the address: component ti of j-atom tj of a cluster.
+            const bool jForceLane = (ti & 1) == 0 && ti < 6;
+            const unsigned int jForceLaneOffset = 3 * tj + (ti >> 1);
+            auto flushPending = [&]() {
+#pragma unroll
+                for (int jm = 0; jm < c_jg; jm++)
+                {
+                    if ((pendMask >> jm) & 1u)
+                    {
+                        if (jForceLane)
+                        {
+                            // Uniform cluster base (scalar registers) plus a 32-bit
+                            // lane offset: the atomic's scalar-base address form.
+                            const size_t base = static_cast<size_t>(pendCj[jm]) * (3 * c_cl);
+                            if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                            {
+                                atomicAddAtLane(a.fFloat + base, jForceLaneOffset, pendV[jm]);
+                            }
+                            else
+                            {
+                                atomicAddAtLane(a.fFixed + base, jForceLaneOffset, toFixed(pendV[jm]));
+                            }
+                        }
+                    }
+                }
+                pendMask = 0u;
+            };
+
+            // Lanes 0-31 fetch xq, lanes 32-63 the LJ data of j-atom stageAtom of group g.
+            float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const bool stageK1 = (stageAtom >> 3) == 1;
+            const bool stageK2 = (stageAtom >> 3) == 2;
+            const bool stageK3 = (stageAtom >> 3) == 3;
+            unsigned int pfExcl = 0xFFFFFFFFu;
+            auto prefetch = [&](int g) {
+                // Scalar loads first, then a per-lane select (a select of the
+                // addresses would become a vector load whose wait also waits for the
+                // in-flight force atomics).
+                const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
+                const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
+                const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
+                const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
+                const int cjl = stageK1 ? g1 : (stageK2 ? g2 : (stageK3 ? g3 : g0));
+                const int atom = cjl * c_cl + (stageAtom & 7);
+                if (lane < 32)
+                {
+                    pf = loadF4(xqc, atom);
+                }
+                else if constexpr (vdwType == NBNXM_VDW_CUT_TYPE_TABLE)
+                {
+                    pf = make_float4(__int_as_float(asConst(a.atomTypes)[atom]), 0.0f, 0.0f, 0.0f);
+                }
+                else
+                {
+                    const float2 lj = loadF2(asConst(a.ljComb), atom);
+                    pf = make_float4(lj.x, lj.y, 0.0f, 0.0f);
+                }
+                const int ex = cjList[g].imei[0].excl_ind;
+                pfExcl = (ex != 0) ? asConst(a.excl)[ex].pair[lane] : 0xFFFFFFFFu;
+            };
+            unsigned int stagedExcl = 0xFFFFFFFFu;
+            auto stage = [&]() {
+                __builtin_amdgcn_wave_barrier();
+                // Lanes 0-31 fill the xq slots, lanes 32-63 the LJ slots.
+                slots[c_slotJXq + lane] = pf;
+                stagedExcl = pfExcl;
+                __builtin_amdgcn_wave_barrier();
+            };
+
+
             auto nextInPass = [&](int g) { return exclPass ? nextExcl(g) : nextPlain(g); };
             int jp = nextInPass(begin);
             if (jp < s.cjPackedEnd)
@@ -853,64 +907,72 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         walk(std::false_type{});
         walk(std::true_type{});
     }
-    if constexpr (kPrune)
-    {
-        // The entry's histogram bin for the entry sort (:722-741); one wave per entry.
-        if (lane == 0)
-        {
-            const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
-            atomicAdd(a.pruneSciHistogram + index, 1);
-            a.pruneSciCount[work] = index;
-        }
-    }
 
-    // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
-    double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
-#pragma unroll
-    for (int c = 0; c < c_ncps; c++)
     {
-        const float fx = reduceOverStride8(fciX[c]);
-        const float fy = reduceOverStride8(fciY[c]);
-        const float fz = reduceOverStride8(fciZ[c]);
-        const int ai = (s.sci * c_ncps + c) * c_cl + ti;
-        if (tj < 3)
+        // Lane indices of the final reductions from their own read of the lane
+        // counter (see laneIdOpaque).
+        const int lane = laneIdOpaque();
+        const int ti = lane & 7;
+        const int tj = lane >> 3;
+        if constexpr (kPrune)
         {
-            const float v = (tj == 0) ? fx : ((tj == 1) ? fy : fz);
-            if (v != 0.0f)
+            // The entry's histogram bin for the entry sort (:722-741); one wave per entry.
+            if (lane == 0)
             {
-                if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
-                {
-                    atomicAdd(a.fFloat + 3 * ai + tj, v);
-                }
-                else
-                {
-                    atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v));
-                }
+                const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
+                atomicAdd(a.pruneSciHistogram + index, 1);
+                a.pruneSciCount[work] = index;
             }
         }
-        fsX += static_cast<double>(fx);
-        fsY += static_cast<double>(fy);
-        fsZ += static_cast<double>(fz);
-    }
 
-    if (computeEnergy || a.computeFshift)
-    {
-        // Every lane of an i-column holds the column's sums: take tj == 0 lanes.
-        const double keep = (tj == 0) ? 1.0 : 0.0;
-        double v0 = 0.0, v1 = 0.0;
-        if constexpr (computeEnergy)
+        // i-forces: sum over the 8 j-lanes, lanes tj = 0,1,2 store x,y,z.
+        double fsX = 0.0, fsY = 0.0, fsZ = 0.0;
+#pragma unroll
+        for (int c = 0; c < c_ncps; c++)
         {
-            v0 = reduceWaveD(static_cast<double>(eLJAcc));
-            v1 = reduceWaveD(static_cast<double>(eElAcc));
+            const float fx = reduceOverStride8(fciX[c]);
+            const float fy = reduceOverStride8(fciY[c]);
+            const float fz = reduceOverStride8(fciZ[c]);
+            const int ai = (s.sci * c_ncps + c) * c_cl + ti;
+            if (tj < 3)
+            {
+                const float v = (tj == 0) ? fx : ((tj == 1) ? fy : fz);
+                if (v != 0.0f)
+                {
+                    if constexpr (accumulation == NBNXM_ACCUM_FLOAT)
+                    {
+                        atomicAdd(a.fFloat + 3 * ai + tj, v);
+                    }
+                    else
+                    {
+                        atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v));
+                    }
+                }
+            }
+            fsX += static_cast<double>(fx);
+            fsY += static_cast<double>(fy);
+            fsZ += static_cast<double>(fz);
         }
-        const bool doShift = a.computeFshift && !centralShift;
-        const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
-        const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
-        const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
-        if (lane < 8)
+
+        if (computeEnergy || a.computeFshift)
         {
-            const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
-            a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+            // Every lane of an i-column holds the column's sums: take tj == 0 lanes.
+            const double keep = (tj == 0) ? 1.0 : 0.0;
+            double v0 = 0.0, v1 = 0.0;
+            if constexpr (computeEnergy)
+            {
+                v0 = reduceWaveD(static_cast<double>(eLJAcc));
+                v1 = reduceWaveD(static_cast<double>(eElAcc));
+            }
+            const bool doShift = a.computeFshift && !centralShift;
+            const double s0 = doShift ? reduceWaveD(fsX * keep) : 0.0;
+            const double s1 = doShift ? reduceWaveD(fsY * keep) : 0.0;
+            const double s2 = doShift ? reduceWaveD(fsZ * keep) : 0.0;
+            if (lane < 8)
+            {
+                const double out = (lane == 0) ? v0 : (lane == 1) ? v1 : (lane == 2) ? s0 : (lane == 3) ? s1 : (lane == 4) ? s2 : 0.0;
+                a.entryPartials[8 * static_cast<size_t>(entry) + lane] = out;
+            }
         }
     }
 }
@@ -940,8 +1002,11 @@ hipError_t launchTunedKernel(const NbKernelArgs& a, const NbFlavourParams& flav,
     {
         return hipErrorInvalidValue;
     }
-    const size_t ldsBytes = ((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes + gridBytes;
-    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds, kPrune>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb);
+    // Tables first (16-byte aligned end), then one record per wave.
+    const size_t tableBytes = (((V == NBNXM_VDW_CUT_TYPE_TABLE) ? typeTableLdsBytes(a.numTypes) : 0) + ewaldBytes + gridBytes + 15) / 16 * 16;
+    const size_t ldsBytes = tableBytes + static_cast<size_t>(wpb) * waveStageSlots(kILds) * sizeof(float4);
+    hipLaunchKernelGGL((nbnxmNonbondedKernel<E, V, A, kFlav, kILds, kPrune>), dim3(blocks), dim3(64 * wpb), ldsBytes, stream, a, flav, wpb,
+                       static_cast<int>(tableBytes));
     return hipGetLastError();
 }
 
