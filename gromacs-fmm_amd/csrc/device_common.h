@@ -88,12 +88,13 @@ __device__ __forceinline__ float uniformF(float v)
 // hi = floor(v), lo = (v - floor(v)) * 2^32 in [0, 2^32), truncated.  Six
 // VALU operations instead of a generic float-to-int64 conversion; a
 // deterministic function of v, which is all that order-independent
-// accumulation needs.  |v| is clamped to 2^30 kJ/mol/nm.
-__device__ __forceinline__ unsigned long long toFixed(float v)
+// accumulation needs.  |v| is clamped to 2^30 kJ/mol/nm: bound is c_fixedMaxF,
+// which a caller in a loop passes from a scalar register (uniformF, read once
+// before the loop), so that the clamp is one med3 whose two bounds are that
+// register, once negated (a source modifier), and no vector register holds a
+// constant.
+__device__ __forceinline__ unsigned long long toFixed(float v, float bound = c_fixedMaxF)
 {
-    // One med3 whose two bounds are the same scalar register, once negated
-    // (a source modifier): no vector register holds a constant.
-    const float bound = uniformF(c_fixedMaxF);
     v = __builtin_amdgcn_fmed3f(v, -bound, bound);
     const float hi = __builtin_floorf(v);
     const float lo = (v - hi) * c_fixedScale;

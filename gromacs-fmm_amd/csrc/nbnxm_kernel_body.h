@@ -77,10 +77,11 @@ __device__ __forceinline__ unsigned int waveOr(unsigned int v)
 
 // Atomic add to element `index` (a 32-bit lane offset) of a global array at a
 // wave-uniform base, issued with the scalar-base address form (base in scalar
-// registers plus a 32-bit vector offset).  The base passes through an empty
-// asm that holds it in scalar registers; otherwise the compiler folds the lane
-// offset into a 64-bit vector address that stays live (two registers) through
-// the list walk.
+// registers plus a 32-bit vector offset).  Base and offset pass through an
+// empty asm: otherwise the compiler hoists the widened lane offset out of the
+// caller's loop as a 64-bit vector address (two registers live through the
+// list walk) and adds the base in the vector unit.  The asm costs one vector
+// move (a copy of the offset) per call.
 template<typename T>
 using GlobalPtr = __attribute__((address_space(1))) T*;
 
@@ -432,6 +433,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
     const float betaT = uniformF(c_erfcxPolyScale * beta);                   // t = 1/(1 + 0.4 beta r)
     const float twoBetaSqrtPi = uniformF(beta * static_cast<float>(2.0 * c_oneSqrtPi));
     const float twoCpotDisp = uniformF(2.0f * cpotDisp);
+    const float fixedBound = uniformF(c_fixedMaxF);
 
     // The packed group's 32 j-atoms are staged in the wave's LDS record.  The
     // next group is fetched into registers (one coalesced load per lane)
@@ -801,7 +803,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                             }
                             else
                             {
-                                atomicAddAtLane(a.fFixed + base, jForceLaneOffset, toFixed(pendV[jm]));
+                                atomicAddAtLane(a.fFixed + base, jForceLaneOffset, toFixed(pendV[jm], fixedBound));
                             }
                         }
                     }
@@ -945,7 +947,7 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                     }
                     else
                     {
-                        atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v));
+                        atomicAdd(a.fFixed + 3 * ai + tj, toFixed(v, fixedBound));
                     }
                 }
             }
