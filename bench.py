@@ -26,12 +26,13 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed FP32)
 # The same FP32 vector pipe without packed FP32 instructions, which this build
 # does not use: one wave64 FMA per SIMD every 4 cycles.
 FP32_NONPACKED_TFLOPS = 78.6
-# HBM bytes per batched F+E launch of this workload (no split, 2 waves per
-# block): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device
-# copy) from two rocprofv3 --pmc runs of the current kernel,
+# HBM bytes per batched F+E launch of this workload (at most 48 pruned
+# cluster pairs per work item below 1,500 atoms, 2 waves per block):
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy) from
+# two rocprofv3 --pmc runs of the current kernel,
 # profiles/nonbonded_vf_ensemble256_counters.md.  Counters cannot be read
 # inside the timed run, so the measured figure is carried here.
-TRAFFIC_BYTES_PER_LAUNCH = 1.907e9
+TRAFFIC_BYTES_PER_LAUNCH = 2.126e9
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_PAIR_VF = 107       # Ewald + LJ force+energy (src/gromacs/gmxlib/nrnb.cpp:90-103)
 RC, RLIST_OUTER, RLIST_INNER, EWALD_RTOL, SIGMA = 1.0, 1.1, 1.04, 1e-5, 0.002
