@@ -1286,6 +1286,7 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
     pa.rlistOuter_sq = ro * ro;
     pa.rlistInner_sq = ri * ri;
     pa.numParts = numParts;
+    pa.numGroups = nb->ncjPacked;
     const bool fresh = nb->haveFreshList;
     if (fresh)
     {

@@ -185,18 +185,20 @@ __host__ __device__ constexpr int waveStageSlots(bool iClusterInLds)
 }
 
 // Occupancy floor (waves per SIMD) of the instantiations that have a variant
-// with the i-cluster data in LDS (0: none).  Each pass of the list walk takes
-// its lane indices from its own lane-counter read, uniform constants sit in
-// scalar registers and the j-force atomics use the scalar-base address form,
-// so the exclusion-free pass of the LDS variants needs 61-75 VGPRs (F) and
-// 70-90 (F+E); the exclusion pass and the final reductions need more.  The
-// tuned F+E kernel with a combination rule allocates 87 VGPRs (5 waves) and
-// is held to 72 for a seventh wave: its spills are stored and reloaded in the
-// exclusion pass only, and the batched ensemble launch measured 1.218 ms
-// (5 waves) -> 1.155 (6) -> 1.139 ms (7).  The other floors do not constrain
-// the allocation.  The register variants need no floor (the tuned F+E kernel
-// held to 4 waves, 128 VGPRs, spilled 8 registers and measured 2 % slower on a
-// 96,000-atom box).
+// with the i-cluster data in LDS (0: none; 1: the natural allocation).  Each
+// pass of the list walk takes its lane indices from its own lane-counter read,
+// uniform constants sit in scalar registers and the j-force atomics use the
+// scalar-base address form, so the LDS variants need 61-85 VGPRs (F) and
+// 74-112 (F+E) against 93-116 and 106-143 with the i-cluster data in
+// registers: every compile-time class has an LDS variant, at its natural
+// allocation.  The tuned F+E kernel with a combination rule allocates 87 VGPRs
+// (5 waves: its exclusion-free pass needs 70, the exclusion pass and the final
+// reductions more) and is held to 72 for a seventh wave: its spills are stored
+// and reloaded in the exclusion pass only, and the batched ensemble launch
+// measured 1.218 ms (5 waves) -> 1.155 (6) -> 1.139 ms (7).  The register
+// variants need no floor (the tuned F+E kernel held to 4 waves, 128 VGPRs,
+// spilled 8 registers and measured 2 % slower on a 96,000-atom box in an
+// earlier build).
 template<bool computeEnergy, int vdwType, int kFlav>
 constexpr int iClusterLdsWaves()
 {
@@ -204,24 +206,11 @@ constexpr int iClusterLdsWaves()
     {
         return 0;
     }
-    if (!computeEnergy)
-    {
-        return 4;
-    }
-    if (kFlav == c_flavTuned && vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
+    if (computeEnergy && kFlav == c_flavTuned && vdwType != NBNXM_VDW_CUT_TYPE_TABLE)
     {
         return 7;
     }
-    if (kFlav == c_flavEwaldForceSwitch || kFlav == c_flavEwaldLjPme)
-    {
-        return 3;
-    }
-    // The tabulated-Ewald F+E kernel runs 4 waves per SIMD either way (128 and
-    // 98 VGPRs), so the extra LDS read per cluster pair is all the LDS
-    // variant adds: it measured 6-9 % slower on 384,000- and 3 M-atom boxes,
-    // and the kernel has none.
-    const bool fits4 = vdwType != NBNXM_VDW_CUT_TYPE_TABLE && (kFlav == c_flavTuned || kFlav == c_flavRfPotShift);
-    return fits4 ? 4 : 0;
+    return 1;
 }
 
 // Waves per SIMD the register allocation must allow.
@@ -890,12 +879,12 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
                 {
                     // The group's list words were read (scalar) before this
                     // store (its mask by the pass scan), and no other wave
-                    // reads them.
-                    if (lane == 0)
-                    {
-                        a.pruneCjPacked[jp].imei[0].imask = prunedMask;
-                        a.pruneImaskOuter[jp] = prunedMask;
-                    }
+                    // reads them.  Every lane stores the same word (one
+                    // coalesced write): under a lane-0 branch the compiler
+                    // would wait for these stores before the next group's
+                    // staged data can be used.
+                    a.pruneCjPacked[jp].imei[0].imask = prunedMask;
+                    a.pruneImaskOuter[jp] = prunedMask;
                     prunedPairCount += __popc(prunedMask);
                 }
                 if (jn < s.cjPackedEnd)

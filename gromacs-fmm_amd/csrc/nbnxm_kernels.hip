@@ -209,17 +209,23 @@ __global__ __launch_bounds__(256) void nbnxmNonbondedKernelFp64(const NbKernelAr
 }
 
 // Prune (src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_pruneonly.cuh:104-343), split
-// 1: one workgroup of c_pruneWavesPerEntry waves per entry of the current
-// rolling part, wave k of it checking packed groups k, k + 4, ... (an entry of
-// a large box has hundreds of groups; one wave per entry would leave most
-// SIMDs idle on lists of a few thousand entries and on rolling parts).  The
-// list mechanics of the nonbonded kernel: list words through the constant
-// address space (scalar loads), groups with nothing to check skipped by a
-// scalar test, and the 32 j-atoms of the wave's next group fetched into
-// registers while the current one is checked, then staged in per-wave LDS, so
-// no distance waits for a global load.  The distances and ballots are those
-// of the reference.
-constexpr int c_pruneWavesPerEntry = 4;
+// 1: one workgroup of 1, 2 or 4 waves per entry of the current rolling part
+// (launchPrune chooses from the list's groups per entry), wave k of it
+// checking packed groups k, k + waves, ... (an entry of a large box has
+// hundreds of groups; one wave per entry would leave most SIMDs idle on lists
+// of a few thousand entries and on rolling parts).  The
+// list words of a wave's groups read 64 groups at a time with vector loads,
+// groups with nothing to check skipped by a ballot, and the 32 j-atoms of the
+// next group fetched into registers while the current one is checked, then
+// staged in per-wave LDS, so no distance waits for a global load.  The
+// distances and ballots are those of the reference.
+constexpr int c_pruneWavesPerEntry = 4; // at most
+// Waves per entry from the list's mean packed groups per entry (measured on
+// fresh lists: 96,000-atom water, 26 groups per entry: 4 waves 32.9 us, 2
+// waves 46.5; the 256-box ensemble, 14 groups per entry: 4 waves 500 us, 2
+// waves 450, 1 wave 446).
+constexpr double c_pruneGroupsFor4 = 20.0;
+constexpr double c_pruneGroupsFor2 = 6.0;
 
 template<bool haveFreshList>
 __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(const PruneArgs a)
@@ -228,6 +234,7 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
     const int ti = lane & 7;
     const int tj = lane >> 3;
     const int waveInEntry = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wavesPerEntry = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
     const int w = blockIdx.x;
 
     // The part index per block, as the reference keeps it (:119-127).
@@ -255,117 +262,140 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
         yi[c] = v.y + sh.y;
         zi[c] = v.z + sh.z;
     }
-    // The list words of a group are read (scalar) before its wave writes
-    // them, and each group is read and written by one wave only.
-    const ConstPtr<nbnxm_cj_packed_t> cjList = asConst(static_cast<const nbnxm_cj_packed_t*>(a.cjPacked));
-    const ConstPtr<unsigned int> outerList = asConst(static_cast<const unsigned int*>(a.imaskOuter));
-    // Bits still to check: every bit of a fresh list, else those of the outer
-    // mask that the inner mask has dropped (:229-245).
-    auto checkBits = [&](int g) -> unsigned int {
-        const unsigned int inner = cjList[g].imei[0].imask;
-        return haveFreshList ? inner : (inner ^ outerList[g]);
-    };
-    auto nextToCheck = [&](int g) {
-        while (g < s.cjPackedEnd && checkBits(g) == 0u)
-        {
-            g += c_pruneWavesPerEntry;
-        }
-        return g;
-    };
+    // The wave's groups are g = begin + waveInEntry + wavesPerEntry k.  Their list words
+    // are read 64 groups at a time, lane l holding group k0 + l (two vector
+    // loads for the packed group's six words, one for its outer mask), and the
+    // groups with bits to check are taken from a ballot: per group there is no
+    // scalar load to wait for, only its j-atoms' xq, fetched into registers
+    // while the group before it is checked and then staged in per-wave LDS.
+    // The new masks collect in the lanes of their groups and are stored with
+    // one vector store per chunk.  Each group is read and written by one wave
+    // only.
+    const int firstGroup = s.cjPackedBegin + waveInEntry;
+    const int numGroups = (s.cjPackedEnd > firstGroup) ? (s.cjPackedEnd - firstGroup + wavesPerEntry - 1) / wavesPerEntry : 0;
     __shared__ float4 sXq[c_pruneWavesPerEntry][c_jg * c_cl];
     float4* stageXq = sXq[waveInEntry];
-    float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    auto prefetch = [&](int g) {
-        const int g0 = __builtin_amdgcn_readfirstlane(cjList[g].cj[0]);
-        const int g1 = __builtin_amdgcn_readfirstlane(cjList[g].cj[1]);
-        const int g2 = __builtin_amdgcn_readfirstlane(cjList[g].cj[2]);
-        const int g3 = __builtin_amdgcn_readfirstlane(cjList[g].cj[3]);
-        const int k = (lane >> 3) & 3;
-        const int cjl = (k == 1) ? g1 : ((k == 2) ? g2 : ((k == 3) ? g3 : g0));
-        if (lane < 32)
-        {
-            pf = a.xq[cjl * c_cl + (lane & 7)];
-        }
-    };
-    auto stage = [&]() {
+    auto stage = [&](const float4 v) {
         __builtin_amdgcn_wave_barrier();
         if (lane < 32)
         {
-            stageXq[lane] = pf;
+            stageXq[lane] = v;
         }
         __builtin_amdgcn_wave_barrier();
     };
-
     int prunedPairCount = 0;
-    int jp = nextToCheck(s.cjPackedBegin + waveInEntry);
-    if (jp < s.cjPackedEnd)
+    for (int k0 = 0; k0 < numGroups; k0 += 64)
     {
-        prefetch(jp);
-        stage();
-    }
-    while (jp < s.cjPackedEnd)
-    {
-        unsigned int imaskFull, imaskNew;
-        const unsigned int imaskCheck = checkBits(jp);
-        if (haveFreshList)
+        const int myGroup = firstGroup + wavesPerEntry * (k0 + lane);
+        const bool valid = k0 + lane < numGroups;
+        int4 words = make_int4(0, 0, 0, 0);
+        unsigned int inner = 0u, outer = 0u;
+        if (valid)
         {
-            imaskFull = imaskCheck;
-            imaskNew = 0u;
-        }
-        else
-        {
-            imaskFull = outerList[jp];
-            imaskNew = cjList[jp].imei[0].imask;
-        }
-        const int jn = nextToCheck(jp + c_pruneWavesPerEntry);
-        if (jn < s.cjPackedEnd)
-        {
-            prefetch(jn);
-        }
-#pragma unroll
-        for (int jm = 0; jm < c_jg; jm++)
-        {
-            if (!(imaskCheck & (0xFFu << (jm * c_ncps))))
+            words = *reinterpret_cast<const int4*>(a.cjPacked[myGroup].cj);
+            inner = a.cjPacked[myGroup].imei[0].imask;
+            if (!haveFreshList)
             {
-                continue;
+                outer = a.imaskOuter[myGroup];
             }
-            const float4 xj = stageXq[jm * c_cl + tj];
-#pragma unroll
-            for (int c = 0; c < c_ncps; c++)
+        }
+        // Bits still to check: every bit of a fresh list, else those of the
+        // outer mask that the inner mask has dropped (:229-245).
+        const unsigned int check = haveFreshList ? inner : (inner ^ outer);
+        const unsigned long long todo = __builtin_amdgcn_ballot_w64(check != 0u);
+        if (todo == 0ull)
+        {
+            continue;
+        }
+        // The j-atoms of the group in lane k: lanes 0-31 (and, repeating them,
+        // 32-63) load atom lane & 7 of j-cluster (lane >> 3) & 3.
+        auto prefetch = [&](int k) -> float4 {
+            const int g0 = __builtin_amdgcn_readlane(words.x, k);
+            const int g1 = __builtin_amdgcn_readlane(words.y, k);
+            const int g2 = __builtin_amdgcn_readlane(words.z, k);
+            const int g3 = __builtin_amdgcn_readlane(words.w, k);
+            const int jc = (lane >> 3) & 3;
+            const int cjl = (jc == 1) ? g1 : ((jc == 2) ? g2 : ((jc == 3) ? g3 : g0));
+            return a.xq[cjl * c_cl + (lane & 7)];
+        };
+        unsigned int newInner = inner;
+        unsigned int newOuter = outer;
+        int k = __builtin_ctzll(todo);
+        unsigned long long rest = todo & (todo - 1ull);
+        stage(prefetch(k));
+        while (true)
+        {
+            const int kn = rest ? static_cast<int>(__builtin_ctzll(rest)) : -1;
+            float4 pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (kn >= 0)
             {
-                const unsigned int mask_ji = 1u << (jm * c_ncps + c);
-                if (!(imaskCheck & mask_ji))
+                pf = prefetch(kn);
+            }
+            const unsigned int imaskCheck = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(check), k));
+            unsigned int imaskFull, imaskNew;
+            if (haveFreshList)
+            {
+                imaskFull = imaskCheck;
+                imaskNew = 0u;
+            }
+            else
+            {
+                imaskFull = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(outer), k));
+                imaskNew = static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(inner), k));
+            }
+#pragma unroll
+            for (int jm = 0; jm < c_jg; jm++)
+            {
+                if (!(imaskCheck & (0xFFu << (jm * c_ncps))))
                 {
                     continue;
                 }
-                const float r2 = norm2Fixed(xi[c] - xj.x, yi[c] - xj.y, zi[c] - xj.z);
-                if (haveFreshList && __builtin_amdgcn_ballot_w64(r2 < a.rlistOuter_sq) == 0ull)
+                const float4 xj = stageXq[jm * c_cl + tj];
+#pragma unroll
+                for (int c = 0; c < c_ncps; c++)
                 {
-                    imaskFull &= ~mask_ji;
-                }
-                if (__builtin_amdgcn_ballot_w64(r2 < a.rlistInner_sq) != 0ull)
-                {
-                    imaskNew |= mask_ji;
+                    const unsigned int mask_ji = 1u << (jm * c_ncps + c);
+                    if (!(imaskCheck & mask_ji))
+                    {
+                        continue;
+                    }
+                    const float r2 = norm2Fixed(xi[c] - xj.x, yi[c] - xj.y, zi[c] - xj.z);
+                    // rlistInner <= rlistOuter: a pair inside the inner radius
+                    // keeps the cluster pair in both masks, so the outer test
+                    // runs only for cluster pairs with none.
+                    if (__builtin_amdgcn_ballot_w64(r2 < a.rlistInner_sq) != 0ull)
+                    {
+                        imaskNew |= mask_ji;
+                    }
+                    else if (haveFreshList && __builtin_amdgcn_ballot_w64(r2 < a.rlistOuter_sq) == 0ull)
+                    {
+                        imaskFull &= ~mask_ji;
+                    }
                 }
             }
-        }
-        if (lane == 0)
-        {
+            const bool mine = lane == k;
+            newInner = mine ? imaskNew : newInner;
             if (haveFreshList)
             {
-                a.imaskOuter[jp] = imaskFull;
+                newOuter = mine ? imaskFull : newOuter;
+                prunedPairCount += __popc(imaskNew);
             }
-            a.cjPacked[jp].imei[0].imask = imaskNew;
+            if (kn < 0)
+            {
+                break;
+            }
+            stage(pf);
+            k = kn;
+            rest &= rest - 1ull;
         }
-        if (haveFreshList)
+        if (check != 0u)
         {
-            prunedPairCount += __popc(imaskNew);
+            a.cjPacked[myGroup].imei[0].imask = newInner;
+            if (haveFreshList)
+            {
+                a.imaskOuter[myGroup] = newOuter;
+            }
         }
-        if (jn < s.cjPackedEnd)
-        {
-            stage();
-        }
-        jp = jn;
     }
     if (haveFreshList)
     {
@@ -379,7 +409,7 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
         if (threadIdx.x == 0)
         {
             int total = 0;
-            for (int k = 0; k < c_pruneWavesPerEntry; k++)
+            for (int k = 0; k < wavesPerEntry; k++)
             {
                 total += sCount[k];
             }
@@ -779,7 +809,7 @@ __device__ __forceinline__ float uniformOpen(uint32_t u)
     return (static_cast<float>(u >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
-__global__ void perturbKernel(const float4* x0, float4* xq, int first, int count, float sigma, uint2 key, const unsigned long long* step)
+__global__ void perturbKernel(const float4* x0, float4* xq, int first, int count, float sigma, uint2 key, const unsigned long long* step, unsigned int stepOffset)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count)
@@ -790,7 +820,7 @@ __global__ void perturbKernel(const float4* x0, float4* xq, int first, int count
     float4 v = x0[i];
     if (v.x > -1.0e5f && sigma > 0.0f)
     {
-        const unsigned long long st = *step;
+        const unsigned long long st = *step + stepOffset;
         const uint4 r = philox4x32_10(make_uint4(static_cast<uint32_t>(i), 0u, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32)), key);
         const float u1 = uniformOpen(r.x), u2 = uniformOpen(r.y), u3 = uniformOpen(r.z), u4 = uniformOpen(r.w);
         const float m1 = sqrtf(-2.0f * logf(u1));
@@ -897,13 +927,19 @@ hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t strea
     {
         return hipSuccess;
     }
+    // Waves per entry from the mean groups per entry: enough groups per wave
+    // that its fixed costs (i-cluster loads, list words, the entry's count)
+    // stay small, enough waves that a list of a few thousand entries fills
+    // the GPU.
+    const double groupsPerEntry = static_cast<double>(a.numGroups) / a.nsci;
+    const int waves = groupsPerEntry >= c_pruneGroupsFor4 ? 4 : (groupsPerEntry >= c_pruneGroupsFor2 ? 2 : 1);
     if (haveFreshList)
     {
-        hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * c_pruneWavesPerEntry), 0, stream, a);
+        hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * waves), 0, stream, a);
     }
     else
     {
-        hipLaunchKernelGGL(nbnxmPruneKernel<false>, dim3(blocks), dim3(64 * c_pruneWavesPerEntry), 0, stream, a);
+        hipLaunchKernelGGL(nbnxmPruneKernel<false>, dim3(blocks), dim3(64 * waves), 0, stream, a);
     }
     return hipGetLastError();
 }
@@ -1081,14 +1117,14 @@ hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, c
     return hipGetLastError();
 }
 
-hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream)
+hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream, unsigned int stepOffset)
 {
     if (count <= 0)
     {
         return hipSuccess;
     }
     const uint2 key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
-    hipLaunchKernelGGL(perturbKernel, dim3((count + 255) / 256), dim3(256), 0, stream, x0, xq, first, count, sigma, key, step);
+    hipLaunchKernelGGL(perturbKernel, dim3((count + 255) / 256), dim3(256), 0, stream, x0, xq, first, count, sigma, key, step, stepOffset);
     return hipGetLastError();
 }
 

@@ -121,6 +121,40 @@ def test_graph_replays_equal_eager(systems, ens):
         ens.run(1, nb.MODE_BATCHED, 0.0, 1, False)
 
 
+@pytest.mark.parametrize("steps,use_graph,per_graph", [(2, False, 0), (3, False, 0), (5, True, 5), (4, True, 2)])
+def test_overlapped_steps_equal_single_steps(systems, ens, steps, use_graph, per_graph):
+    """A batched run of several fixed-point steps runs the perturbation of
+    the step after next and the finishing of the previous step on a second
+    stream with two buffer sets; it leaves the state of the same number of
+    one-step runs (one stream, one buffer set) bit for bit: forces, energies,
+    shift forces and positions of the last step, and the step counter (the
+    next run continues from it)."""
+    announce(f"overlapped {steps} steps (graph {use_graph}, {per_graph} per graph) vs one-step runs")
+    ens.set_step(11)
+    for _ in range(steps + 1):
+        ens.run(1, nb.MODE_BATCHED, 0.002, 3, True)
+    next_f = [ens.forces(k) for k in range(len(systems))]
+    ens.set_step(11)
+    for _ in range(steps):
+        ens.run(1, nb.MODE_BATCHED, 0.002, 3, True)
+    ref_f = [ens.forces(k) for k in range(len(systems))]
+    ref_x = [ens.positions(k) for k in range(len(systems))]
+    ref_e, ref_fs = ens.energies()
+    for rep in range(2):
+        ens.set_step(11)
+        ens.run(steps, nb.MODE_BATCHED, 0.002, 3, True, use_graph=use_graph, steps_per_graph=per_graph)
+        ens.sync()
+        for k in range(len(systems)):
+            assert np.array_equal(ens.forces(k), ref_f[k]), (rep, k)
+            assert np.array_equal(ens.positions(k), ref_x[k]), (rep, k)
+        e, fs = ens.energies()
+        assert np.array_equal(e, ref_e) and np.array_equal(fs, ref_fs), rep
+        # A one-step run afterwards continues from the step counter.
+        ens.run(1, nb.MODE_BATCHED, 0.002, 3, True)
+        for k in range(len(systems)):
+            assert np.array_equal(ens.forces(k), next_f[k]), (rep, k)
+
+
 def test_perturbed_positions_covered_by_list(systems, ens):
     announce("perturbed positions vs brute force")
     ens.set_step(123)
