@@ -373,7 +373,8 @@ struct SearchParams
     const int* exclElements; // global atom indices
     int maxPairsPerEntry;
     // pass 1
-    int* counts;             // per task: groups, exclusion entries, sci records
+    int* counts;             // groups, exclusion entries, sci records of task t at t, stride + t, 2 stride + t
+    int countStride;         // numTasks + 1 (element numTasks of each array stays 0: the scans' totals)
     // pass 2
     const int* groupOffset;
     const int* exclOffset;
@@ -551,9 +552,9 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
     {
         if (!write && lane == 0)
         {
-            P.counts[3 * task + 0] = 0;
-            P.counts[3 * task + 1] = 0;
-            P.counts[3 * task + 2] = 0;
+            P.counts[task] = 0;
+            P.counts[P.countStride + task] = 0;
+            P.counts[2 * P.countStride + task] = 0;
         }
         return;
     }
@@ -564,9 +565,9 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
             atomicOr(P.error, 1);
             if (!write)
             {
-                P.counts[3 * task + 0] = 0;
-                P.counts[3 * task + 1] = 0;
-                P.counts[3 * task + 2] = 0;
+                P.counts[task] = 0;
+                P.counts[P.countStride + task] = 0;
+                P.counts[2 * P.countStride + task] = 0;
             }
         }
         return;
@@ -681,9 +682,9 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
             atomicOr(P.error, 2);
             if (!write)
             {
-                P.counts[3 * task + 0] = 0;
-                P.counts[3 * task + 1] = 0;
-                P.counts[3 * task + 2] = 0;
+                P.counts[task] = 0;
+                P.counts[P.countStride + task] = 0;
+                P.counts[2 * P.countStride + task] = 0;
             }
         }
         return;
@@ -741,9 +742,9 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
     {
         if (lane == 0)
         {
-            P.counts[3 * task + 0] = ngroups;
-            P.counts[3 * task + 1] = numEvents;
-            P.counts[3 * task + 2] = sNumPieces;
+            P.counts[task] = ngroups;
+            P.counts[P.countStride + task] = numEvents;
+            P.counts[2 * P.countStride + task] = sNumPieces;
         }
         return;
     }
@@ -857,6 +858,37 @@ __global__ void fillAllOnesKernel(nbnxm_excl_t* excl)
 
 using namespace nbnxm_amd;
 
+// After the scans of the per-task counts: the list totals (groups, exclusion
+// masks less the all-ones one, entries), the error flag, and the first group,
+// mask and entry of every system (its first task's offsets), in one small
+// array for one copy to the host.
+__global__ void searchSummaryKernel(const int* groupOffset,
+                                    const int* exclOffset,
+                                    const int* sciOffset,
+                                    int numTasks,
+                                    const int* colCellStart,
+                                    const int* colSysStart,
+                                    int numSystems,
+                                    const int* error,
+                                    int* summary)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0)
+    {
+        summary[0] = groupOffset[numTasks];
+        summary[1] = exclOffset[numTasks];
+        summary[2] = sciOffset[numTasks];
+        summary[3] = *error;
+    }
+    if (k <= numSystems)
+    {
+        const int t = min(colCellStart[colSysStart[k]] * c_numHalfShifts, numTasks);
+        summary[4 + 3 * k + 0] = groupOffset[t];
+        summary[4 + 3 * k + 1] = exclOffset[t];
+        summary[4 + 3 * k + 2] = sciOffset[t];
+    }
+}
+
 struct nbnxm_gpu_search
 {
     hipStream_t stream = nullptr;
@@ -880,7 +912,7 @@ struct nbnxm_gpu_search
     DeviceBuffer<int> atomIndex, cellOfAtom, numClustersInCell, cellSys;
     DeviceBuffer<float> clusterBb, cellBb;
     DeviceBuffer<int> exclRanges, exclElements;
-    DeviceBuffer<int> counts, groupOffset, exclOffset, sciOffset, countsCol;
+    DeviceBuffer<int> counts, groupOffset, exclOffset, sciOffset, countsCol, summary;
     DeviceBuffer<nbnxm_sci_t> sci;
     DeviceBuffer<nbnxm_cj_packed_t> cjPacked;
     DeviceBuffer<nbnxm_excl_t> excl;
@@ -1077,73 +1109,51 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     P.exclRanges = anyExcl ? s->exclRanges.ptr : nullptr;
     P.exclElements = anyExcl ? s->exclElements.ptr : nullptr;
     P.maxPairsPerEntry = maxPairsPerEntry;
-    NBNXM_HIP_CHECK(s->counts.resize(3 * static_cast<size_t>(numTasks)));
+    const int stride = numTasks + 1;
+    NBNXM_HIP_CHECK(s->counts.resize(3 * static_cast<size_t>(stride)));
     NBNXM_HIP_CHECK(s->error.resize(1));
     NBNXM_HIP_CHECK(launchFillI32(s->error.ptr, 1, 0, st));
+    NBNXM_HIP_CHECK(launchFillI32(s->counts.ptr, 3 * static_cast<size_t>(stride), 0, st));
     P.counts = s->counts.ptr;
+    P.countStride = stride;
     P.error = s->error.ptr;
     hipLaunchKernelGGL(searchKernel<false>, dim3(numTasks), dim3(64), 0, st, P);
     NBNXM_HIP_CHECK(hipGetLastError());
 
-    // Scans of the three counts (strided by 3: copy into columns on the host side of a small kernel-free path).
-    std::vector<int> hCounts(3 * static_cast<size_t>(numTasks));
-    int hError = 0;
-    NBNXM_HIP_CHECK(hipMemcpyAsync(hCounts.data(), s->counts.ptr, sizeof(int) * hCounts.size(), hipMemcpyDeviceToHost, st));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(&hError, s->error.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    // Offsets of every task in the three outputs: scans on the device; only
+    // the totals, the error flag and the systems' first offsets come back.
+    NBNXM_HIP_CHECK(s->groupOffset.resize(stride));
+    NBNXM_HIP_CHECK(s->exclOffset.resize(stride));
+    NBNXM_HIP_CHECK(s->sciOffset.resize(stride));
+    NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr, s->groupOffset.ptr, stride));
+    NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr + stride, s->exclOffset.ptr, stride));
+    NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr + 2 * stride, s->sciOffset.ptr, stride));
+    const int summarySize = 4 + 3 * (numSystems + 1);
+    NBNXM_HIP_CHECK(s->summary.resize(summarySize));
+    hipLaunchKernelGGL(searchSummaryKernel, dim3((numSystems + 1 + 63) / 64), dim3(64), 0, st, s->groupOffset.ptr, s->exclOffset.ptr, s->sciOffset.ptr,
+                       numTasks, s->colCellStart.ptr, s->colSysStart.ptr, numSystems, s->error.ptr, s->summary.ptr);
+    NBNXM_HIP_CHECK(hipGetLastError());
+    std::vector<int> hSummary(summarySize);
+    NBNXM_HIP_CHECK(hipMemcpyAsync(hSummary.data(), s->summary.ptr, sizeof(int) * summarySize, hipMemcpyDeviceToHost, st));
     NBNXM_HIP_CHECK(hipStreamSynchronize(st));
+    const int hError = hSummary[3];
     if (hError)
     {
         return fail(NBNXM_ERR_UNSUPPORTED, (hError & 1) ? "gpu_search_build: an entry exceeds 1024 j-clusters" : "gpu_search_build: an entry needs more than 32 exclusion masks");
     }
-    std::vector<int> hOff(3 * static_cast<size_t>(numTasks) + 3, 0);
-    {
-        int g = 0, e = 0, c = 0;
-        for (int t = 0; t < numTasks; t++)
-        {
-            hOff[3 * t + 0] = g;
-            hOff[3 * t + 1] = e;
-            hOff[3 * t + 2] = c;
-            g += hCounts[3 * t + 0];
-            e += hCounts[3 * t + 1];
-            c += hCounts[3 * t + 2];
-        }
-        s->ngroups = g;
-        s->nexcl = e + 1;
-        s->nsci = c;
-    }
+    s->ngroups = hSummary[0];
+    s->nexcl = hSummary[1] + 1;
+    s->nsci = hSummary[2];
     // Per-system ranges of entries, groups and exclusion masks.
     s->sysSciStart.assign(numSystems + 1, 0);
     s->sysGroupStart.assign(numSystems + 1, 0);
     s->sysExclStart.assign(numSystems + 1, 0);
     for (int k = 0; k <= numSystems; k++)
     {
-        const int t = s->sysCellStart[k] * c_numHalfShifts;
-        if (t < numTasks)
-        {
-            s->sysGroupStart[k] = hOff[3 * t + 0];
-            s->sysExclStart[k] = hOff[3 * t + 1];
-            s->sysSciStart[k] = hOff[3 * t + 2];
-        }
-        else
-        {
-            s->sysGroupStart[k] = s->ngroups;
-            s->sysExclStart[k] = s->nexcl - 1;
-            s->sysSciStart[k] = s->nsci;
-        }
+        s->sysGroupStart[k] = hSummary[4 + 3 * k + 0];
+        s->sysExclStart[k] = hSummary[4 + 3 * k + 1];
+        s->sysSciStart[k] = hSummary[4 + 3 * k + 2];
     }
-    std::vector<int> hG(numTasks), hE(numTasks), hS(numTasks);
-    for (int t = 0; t < numTasks; t++)
-    {
-        hG[t] = hOff[3 * t];
-        hE[t] = hOff[3 * t + 1];
-        hS[t] = hOff[3 * t + 2];
-    }
-    NBNXM_HIP_CHECK(s->groupOffset.resize(numTasks));
-    NBNXM_HIP_CHECK(s->exclOffset.resize(numTasks));
-    NBNXM_HIP_CHECK(s->sciOffset.resize(numTasks));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->groupOffset.ptr, hG.data(), sizeof(int) * numTasks, hipMemcpyHostToDevice, st));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->exclOffset.ptr, hE.data(), sizeof(int) * numTasks, hipMemcpyHostToDevice, st));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(s->sciOffset.ptr, hS.data(), sizeof(int) * numTasks, hipMemcpyHostToDevice, st));
     NBNXM_HIP_CHECK(s->sci.resize(std::max(s->nsci, 1)));
     NBNXM_HIP_CHECK(s->cjPacked.resize(std::max(s->ngroups, 1)));
     NBNXM_HIP_CHECK(s->excl.resize(s->nexcl));
