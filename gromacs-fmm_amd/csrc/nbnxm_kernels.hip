@@ -292,7 +292,11 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
         unsigned int inner = 0u, outer = 0u;
         if (valid)
         {
-            words = *reinterpret_cast<const int4*>(a.cjPacked[myGroup].cj);
+            // A packed group is 24 bytes: its j-cluster words are 8-byte aligned.
+            const int2* const cj = reinterpret_cast<const int2*>(a.cjPacked[myGroup].cj);
+            const int2 w01 = cj[0];
+            const int2 w23 = cj[1];
+            words = make_int4(w01.x, w01.y, w23.x, w23.y);
             inner = a.cjPacked[myGroup].imei[0].imask;
             if (!haveFreshList)
             {

@@ -155,6 +155,25 @@ def test_overlapped_steps_equal_single_steps(systems, ens, steps, use_graph, per
             assert np.array_equal(ens.forces(k), next_f[k]), (rep, k)
 
 
+def test_overlapped_steps_equal_single_steps_fp64(systems):
+    """The FP64 accumulation mode takes the same two-stream sequence: three
+    steps in one run equal three one-step runs bit for bit."""
+    announce("overlapped 3 steps vs one-step runs, fp64 mode")
+    ens64 = nb.Ensemble(systems[:3], RC, RTOL, 1.1, 1.04, nb.VDW_CUT_TYPE_TABLE, nb.ACCUM_FP64)
+    ens64.set_step(5)
+    for _ in range(3):
+        ens64.run(1, nb.MODE_BATCHED, 0.002, 9, True)
+    ref_f = [ens64.forces(k) for k in range(3)]
+    ref_e, ref_fs = ens64.energies()
+    ens64.set_step(5)
+    ens64.run(3, nb.MODE_BATCHED, 0.002, 9, True)
+    ens64.sync()
+    for k in range(3):
+        assert np.array_equal(ens64.forces(k), ref_f[k]), k
+    e, fs = ens64.energies()
+    assert np.array_equal(e, ref_e) and np.array_equal(fs, ref_fs)
+
+
 def test_perturbed_positions_covered_by_list(systems, ens):
     announce("perturbed positions vs brute force")
     ens.set_step(123)
