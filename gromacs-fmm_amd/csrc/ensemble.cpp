@@ -102,18 +102,6 @@ struct nbnxm_ensemble
     DeviceBuffer<int> shiftRanges;
     DeviceBuffer<int> shiftEntryList;
     DeviceBuffer<unsigned long long> step;
-    // Second set of the per-step buffers (coordinates, fixed-point forces,
-    // per-work-item energy partials): in a batched sequence of steps, step s
-    // uses set (last - s) & 1, so the last step of every sequence uses the
-    // first set and the next step's perturbation and the previous step's
-    // force conversion can run beside the nonbonded kernel.
-    DeviceBuffer<float4> xqAlt;
-    DeviceBuffer<unsigned long long> fFixedAlt;
-    DeviceBuffer<double> partialsAlt;
-    // Stream of the perturbation and step-finishing kernels of an overlapped
-    // sequence (default priority), and the events that order the two streams.
-    hipStream_t aux = nullptr;
-    HipEvent evFork, evInit, evNonbonded, evBatch[2];
 
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
@@ -128,17 +116,9 @@ struct nbnxm_ensemble
         {
             (void)hipStreamSynchronize(stream);
         }
-        if (aux)
-        {
-            (void)hipStreamSynchronize(aux);
-        }
         if (graphExec)
         {
             (void)hipGraphExecDestroy(graphExec);
-        }
-        if (aux)
-        {
-            (void)hipStreamDestroy(aux);
         }
         if (stream)
         {
@@ -278,7 +258,6 @@ int applySplit(nbnxm_ensemble* e, const std::vector<int>& bucketEdges, const std
     NBNXM_HIP_CHECK(e->dWork.resize(std::max(nwork, 1)));
     NBNXM_HIP_CHECK(e->dPlainOrder.resize(std::max(nwork, 1)));
     NBNXM_HIP_CHECK(e->partials.resize(8 * static_cast<size_t>(std::max(nwork, 1))));
-    NBNXM_HIP_CHECK(e->partialsAlt.resize(8 * static_cast<size_t>(std::max(nwork, 1))));
     NBNXM_HIP_CHECK(e->entryRanges.resize(nsys + 1));
     NBNXM_HIP_CHECK(e->shiftRanges.resize(shiftRanges.size()));
     NBNXM_HIP_CHECK(e->entryList.resize(entryList.size()));
@@ -293,15 +272,14 @@ int applySplit(nbnxm_ensemble* e, const std::vector<int>& bucketEdges, const std
     NBNXM_HIP_CHECK(hipMemcpy(e->entryList.ptr, entryList.data(), sizeof(int) * entryList.size(), hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(hipMemcpy(e->shiftEntryList.ptr, shiftList.data(), sizeof(int) * shiftList.size(), hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(e->partials.ptr), 16 * static_cast<size_t>(std::max(nwork, 1)), 0, e->stream));
-    NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(e->partialsAlt.ptr), 16 * static_cast<size_t>(std::max(nwork, 1)), 0, e->stream));
     NBNXM_HIP_CHECK(hipStreamSynchronize(e->stream));
     return NBNXM_OK;
 }
 
-NbKernelArgs kernelArgs(nbnxm_ensemble* e, int set = 0)
+NbKernelArgs kernelArgs(nbnxm_ensemble* e)
 {
     NbKernelArgs a;
-    a.xq = set ? e->xqAlt.ptr : e->xq.ptr;
+    a.xq = e->xq.ptr;
     a.atomTypes = e->types.ptr;
     a.ljComb = e->ljComb.ptr;
     a.nbfp = e->nbfp.ptr;
@@ -316,8 +294,8 @@ NbKernelArgs kernelArgs(nbnxm_ensemble* e, int set = 0)
     a.sysParamsB = e->paramsB.ptr;
     a.sysParamsD = e->paramsD.ptr;
     a.fFloat = e->fFloat.ptr;
-    a.fFixed = set ? e->fFixedAlt.ptr : e->fFixed.ptr;
-    a.entryPartials = set ? e->partialsAlt.ptr : e->partials.ptr;
+    a.fFixed = e->fFixed.ptr;
+    a.entryPartials = e->partials.ptr;
     a.computeFshift = 1;
     return a;
 }
@@ -379,60 +357,6 @@ int enqueueStep(nbnxm_ensemble* e, int mode, float sigma, uint64_t seed, bool co
     return NBNXM_OK;
 }
 
-// Whether a batched sequence of `steps` evaluations runs overlapped.
-bool overlapped(const nbnxm_ensemble* e, int steps)
-{
-    return steps >= 2 && e->opt.accumulation != NBNXM_ACCUM_FLOAT;
-}
-
-// A batched sequence of evaluations (fixed-point or FP64 accumulation, at
-// least two steps) with the step-finishing work beside the nonbonded kernels:
-// the main stream runs the nonbonded kernel of every step; the auxiliary
-// stream perturbs the coordinates of the first two steps, then, after the
-// nonbonded kernel of step s, converts and clears its forces, reduces its
-// energies, advances the step counter (finishStepKernel) and perturbs the
-// coordinates of step s + 2.  Step s uses buffer set (steps - 1 - s) & 1, so
-// step s + 1 accumulates into the other set while step s is finished, and
-// step s + 2 is perturbed into the set step s read.  The nonbonded kernel of
-// step s >= 2 waits for the batch after step s - 2 (the perturbation of its
-// coordinates and the clearing of its forces).  Every kernel computes what it
-// computes in the one-stream order, so the results are bit-identical; the
-// last step uses the first set, which holds the state afterwards.
-int enqueueOverlapped(nbnxm_ensemble* e, int steps, float sigma, uint64_t seed, bool computeEnergy)
-{
-    const int nsys = static_cast<int>(e->sys.size());
-    auto set = [&](int s) { return (steps - 1 - s) & 1; };
-    auto xqOf = [&](int s) { return set(s) ? e->xqAlt.ptr : e->xq.ptr; };
-    NBNXM_HIP_CHECK(hipEventRecord(e->evFork.e, e->stream));
-    NBNXM_HIP_CHECK(hipStreamWaitEvent(e->aux, e->evFork.e, 0));
-    NBNXM_HIP_CHECK(launchPerturb(e->x0.ptr, xqOf(0), 0, e->totalSlots, sigma, seed, e->step.ptr, e->aux, 0));
-    NBNXM_HIP_CHECK(launchPerturb(e->x0.ptr, xqOf(1), 0, e->totalSlots, sigma, seed, e->step.ptr, e->aux, 1));
-    NBNXM_HIP_CHECK(hipEventRecord(e->evInit.e, e->aux));
-    NBNXM_HIP_CHECK(hipStreamWaitEvent(e->stream, e->evInit.e, 0));
-    for (int s = 0; s < steps; s++)
-    {
-        if (s >= 2)
-        {
-            NBNXM_HIP_CHECK(hipStreamWaitEvent(e->stream, e->evBatch[s & 1].e, 0));
-        }
-        const NbKernelArgs a = kernelArgs(e, set(s));
-        NBNXM_HIP_CHECK(launchNonbonded(a, e->opt.accumulation, e->vdwType, computeEnergy, e->launchConfig, e->stream));
-        NBNXM_HIP_CHECK(hipEventRecord(e->evNonbonded.e, e->stream));
-        NBNXM_HIP_CHECK(hipStreamWaitEvent(e->aux, e->evNonbonded.e, 0));
-        NBNXM_HIP_CHECK(launchFinishStep(a.fFixed, e->totalSlots, nullptr, e->fOutDouble.ptr, a.entryPartials, e->entryRanges.ptr,
-                                         e->entryList.ptr, e->shiftRanges.ptr, e->shiftEntryList.ptr, nsys, e->energies.ptr, e->fshift.ptr, 1,
-                                         e->step.ptr, e->aux));
-        if (s + 2 < steps)
-        {
-            // The step counter now counts step s as done: step s + 2 is one ahead.
-            NBNXM_HIP_CHECK(launchPerturb(e->x0.ptr, xqOf(s + 2), 0, e->totalSlots, sigma, seed, e->step.ptr, e->aux, 1));
-        }
-        NBNXM_HIP_CHECK(hipEventRecord(e->evBatch[s & 1].e, e->aux));
-    }
-    NBNXM_HIP_CHECK(hipStreamWaitEvent(e->stream, e->evBatch[(steps - 1) & 1].e, 0));
-    return NBNXM_OK;
-}
-
 int buildGraph(nbnxm_ensemble* e, int steps, float sigma, uint64_t seed, bool computeEnergy)
 {
     int rc = dropGraph(e);
@@ -442,11 +366,7 @@ int buildGraph(nbnxm_ensemble* e, int steps, float sigma, uint64_t seed, bool co
     }
     hipGraph_t graph = nullptr;
     NBNXM_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    if (overlapped(e, steps))
-    {
-        rc = enqueueOverlapped(e, steps, sigma, seed, computeEnergy);
-    }
-    for (int s = 0; s < steps && rc == NBNXM_OK && !overlapped(e, steps); s++)
+    for (int s = 0; s < steps && rc == NBNXM_OK; s++)
     {
         rc = enqueueStep(e, NBNXM_MODE_BATCHED, sigma, seed, computeEnergy);
     }
@@ -507,10 +427,6 @@ int runSteps(nbnxm_ensemble* e, int mode, int numSteps, float sigma, uint64_t se
             }
         }
         return NBNXM_OK;
-    }
-    if (mode == NBNXM_MODE_BATCHED && overlapped(e, numSteps))
-    {
-        return enqueueOverlapped(e, numSteps, sigma, seed, computeEnergy);
     }
     for (int s = 0; s < numSteps; s++)
     {
@@ -624,12 +540,6 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     }
     e->launchConfig.wavesPerBlock = (opt->wavesPerBlock >= 1 && opt->wavesPerBlock <= 4) ? opt->wavesPerBlock : 4;
     NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    NBNXM_HIP_CHECK(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking));
-    NBNXM_HIP_CHECK(hipEventCreateWithFlags(&e->evFork.e, hipEventDisableTiming));
-    NBNXM_HIP_CHECK(hipEventCreateWithFlags(&e->evInit.e, hipEventDisableTiming));
-    NBNXM_HIP_CHECK(hipEventCreateWithFlags(&e->evNonbonded.e, hipEventDisableTiming));
-    NBNXM_HIP_CHECK(hipEventCreateWithFlags(&e->evBatch[0].e, hipEventDisableTiming));
-    NBNXM_HIP_CHECK(hipEventCreateWithFlags(&e->evBatch[1].e, hipEventDisableTiming));
 
     // Grids and unsplit lists: on the device in one pass, or on host threads.
     e->sys.resize(numSystems);
@@ -803,7 +713,6 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     const size_t ns = e->totalSlots;
     NBNXM_HIP_CHECK(e->x0.resize(ns));
     NBNXM_HIP_CHECK(e->xq.resize(ns));
-    NBNXM_HIP_CHECK(e->xqAlt.resize(ns));
     NBNXM_HIP_CHECK(e->types.resize(ns));
     NBNXM_HIP_CHECK(e->ljComb.resize(ns));
     NBNXM_HIP_CHECK(e->nbfp.resize(hnbfp.size()));
@@ -816,7 +725,6 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     NBNXM_HIP_CHECK(e->dExcl.resize(e->excl.size()));
     NBNXM_HIP_CHECK(e->dImaskOuter.resize(std::max<size_t>(e->cjPacked.size(), 1)));
     NBNXM_HIP_CHECK(e->fFixed.resize(3 * ns));
-    NBNXM_HIP_CHECK(e->fFixedAlt.resize(3 * ns));
     NBNXM_HIP_CHECK(e->fFloat.resize(3 * ns));
     NBNXM_HIP_CHECK(e->fOutDouble.resize(3 * ns));
     NBNXM_HIP_CHECK(e->energies.resize(2 * static_cast<size_t>(numSystems)));
@@ -825,7 +733,6 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
 
     NBNXM_HIP_CHECK(hipMemcpy(e->x0.ptr, hx.data(), sizeof(float4) * ns, hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(hipMemcpy(e->xq.ptr, hx.data(), sizeof(float4) * ns, hipMemcpyHostToDevice));
-    NBNXM_HIP_CHECK(hipMemcpy(e->xqAlt.ptr, hx.data(), sizeof(float4) * ns, hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(hipMemcpy(e->types.ptr, htypes.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(hipMemcpy(e->ljComb.ptr, hcomb.data(), sizeof(float2) * ns, hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(hipMemcpy(e->nbfp.ptr, hnbfp.data(), sizeof(float2) * hnbfp.size(), hipMemcpyHostToDevice));
@@ -843,7 +750,6 @@ int nbnxm_ensemble_create(int numSystems, const nbnxm_system_desc_t* systems, co
     }
     NBNXM_HIP_CHECK(hipMemcpy(e->dExcl.ptr, e->excl.data(), sizeof(nbnxm_excl_t) * e->excl.size(), hipMemcpyHostToDevice));
     NBNXM_HIP_CHECK(launchFillU64(e->fFixed.ptr, 3 * ns, e->stream));
-    NBNXM_HIP_CHECK(launchFillU64(e->fFixedAlt.ptr, 3 * ns, e->stream));
     NBNXM_HIP_CHECK(launchFillU64(e->step.ptr, 1, e->stream));
 
     // Fresh-list prune of every group: rlistOuter mask and rlistInner mask.

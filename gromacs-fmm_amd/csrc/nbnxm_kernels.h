@@ -269,12 +269,10 @@ hipError_t launchWriteCoordinates(const float4* xw, int numAtoms, float* x, hipS
 hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, const int* cells, int numAtoms, const float* fToAdd, float* f, bool accumulate, hipStream_t stream);
 
 // xq = x0 + N(0, sigma) for real atoms (q and fillers kept) over slots
-// [first, first+count); the step number is *step + stepOffset, read on the
-// device so a captured graph can be replayed (an evaluation perturbed ahead of
-// the step counter passes its distance); Philox4x32-10 counter-based generator
-// keyed by (seed), counter (slot, step).
-hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream,
-                         unsigned int stepOffset = 0);
+// [first, first+count); the step number is read from *step on the device so a
+// captured graph can be replayed; Philox4x32-10 counter-based generator keyed
+// by (seed), counter (slot, step).
+hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream);
 // *step += 1 (one thread), ends an evaluation in a captured graph.
 hipError_t launchAdvanceStep(unsigned long long* step, hipStream_t stream);
 

@@ -458,6 +458,53 @@ __global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* f
     convertForce(i, fFixed, fOut, fOutDouble, clear);
 }
 
+// Lane `lane` of a wave adds fields first..first+N-1 of the entry partials of
+// list[k] for k = begin + lane, begin + lane + 64, ... in that order; four
+// entries' loads are issued before their adds, so a long list (the central
+// shift of one large system) is not one dependent load chain per entry.
+template<int N>
+__device__ __forceinline__ void sumEntryPartials(const int* list, int begin, int end, int lane, const double* partials, int first, double (&acc)[N])
+{
+    int k = begin + lane;
+    for (; k + 3 * 64 < end; k += 4 * 64)
+    {
+        int e[4];
+        double v[4][N];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            e[u] = list[k + u * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+#pragma unroll
+            for (int c = 0; c < N; c++)
+            {
+                v[u][c] = partials[8 * static_cast<size_t>(e[u]) + first + c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+#pragma unroll
+            for (int c = 0; c < N; c++)
+            {
+                acc[c] += v[u][c];
+            }
+        }
+    }
+    for (; k < end; k += 64)
+    {
+        const int e = list[k];
+#pragma unroll
+        for (int c = 0; c < N; c++)
+        {
+            acc[c] += partials[8 * static_cast<size_t>(e) + first + c];
+        }
+    }
+}
+
 // One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
 __device__ __forceinline__ void reduceEnergySlot(int sys,
                                                  int slot,
@@ -473,15 +520,10 @@ __device__ __forceinline__ void reduceEnergySlot(int sys,
 {
     if (slot == 0)
     {
-        double e0 = 0.0, e1 = 0.0;
-        for (int k = entryRanges[sys] + lane; k < entryRanges[sys + 1]; k += 64)
-        {
-            const int e = entryList[k];
-            e0 += partials[8 * static_cast<size_t>(e)];
-            e1 += partials[8 * static_cast<size_t>(e) + 1];
-        }
-        e0 = reduceWaveD(e0);
-        e1 = reduceWaveD(e1);
+        double acc[2] = { 0.0, 0.0 };
+        sumEntryPartials(entryList, entryRanges[sys], entryRanges[sys + 1], lane, partials, 0, acc);
+        const double e0 = reduceWaveD(acc[0]);
+        const double e1 = reduceWaveD(acc[1]);
         if (lane == 0)
         {
             energiesOut[2 * sys] = e0;
@@ -494,17 +536,11 @@ __device__ __forceinline__ void reduceEnergySlot(int sys,
         return;
     }
     const int sidx = sys * NBNXM_NUM_SHIFTS + slot - 1;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int k = shiftRanges[sidx] + lane; k < shiftRanges[sidx + 1]; k += 64)
-    {
-        const int e = shiftEntryList[k];
-        f0 += partials[8 * static_cast<size_t>(e) + 2];
-        f1 += partials[8 * static_cast<size_t>(e) + 3];
-        f2 += partials[8 * static_cast<size_t>(e) + 4];
-    }
-    f0 = reduceWaveD(f0);
-    f1 = reduceWaveD(f1);
-    f2 = reduceWaveD(f2);
+    double acc[3] = { 0.0, 0.0, 0.0 };
+    sumEntryPartials(shiftEntryList, shiftRanges[sidx], shiftRanges[sidx + 1], lane, partials, 2, acc);
+    const double f0 = reduceWaveD(acc[0]);
+    const double f1 = reduceWaveD(acc[1]);
+    const double f2 = reduceWaveD(acc[2]);
     if (lane == 0)
     {
         fshiftOut[3 * sidx + 0] = f0;
@@ -554,8 +590,26 @@ __global__ __launch_bounds__(256) void finishStepKernel(unsigned long long* fFix
     }
     if (static_cast<int>(blockIdx.x) < convertBlocks)
     {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i < n3)
+        // Two components per thread: 16-byte loads and stores (the buffers
+        // are allocation-aligned).
+        const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+        if (i + 1 < n3)
+        {
+            ulonglong2* const p = reinterpret_cast<ulonglong2*>(fFixed + i);
+            const ulonglong2 v = *p;
+            const double d0 = static_cast<double>(static_cast<long long>(v.x)) * (1.0 / c_fixedScaleD);
+            const double d1 = static_cast<double>(static_cast<long long>(v.y)) * (1.0 / c_fixedScaleD);
+            if (fOut)
+            {
+                *reinterpret_cast<float2*>(fOut + i) = make_float2(static_cast<float>(d0), static_cast<float>(d1));
+            }
+            if (fOutDouble)
+            {
+                *reinterpret_cast<double2*>(fOutDouble + i) = make_double2(d0, d1);
+            }
+            *p = make_ulonglong2(0ull, 0ull);
+        }
+        else if (i < n3)
         {
             convertForce(i, fFixed, fOut, fOutDouble, 1);
         }
@@ -813,7 +867,7 @@ __device__ __forceinline__ float uniformOpen(uint32_t u)
     return (static_cast<float>(u >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
-__global__ void perturbKernel(const float4* x0, float4* xq, int first, int count, float sigma, uint2 key, const unsigned long long* step, unsigned int stepOffset)
+__global__ void perturbKernel(const float4* x0, float4* xq, int first, int count, float sigma, uint2 key, const unsigned long long* step)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count)
@@ -824,7 +878,7 @@ __global__ void perturbKernel(const float4* x0, float4* xq, int first, int count
     float4 v = x0[i];
     if (v.x > -1.0e5f && sigma > 0.0f)
     {
-        const unsigned long long st = *step + stepOffset;
+        const unsigned long long st = *step;
         const uint4 r = philox4x32_10(make_uint4(static_cast<uint32_t>(i), 0u, static_cast<uint32_t>(st), static_cast<uint32_t>(st >> 32)), key);
         const float u1 = uniformOpen(r.x), u2 = uniformOpen(r.y), u3 = uniformOpen(r.z), u4 = uniformOpen(r.w);
         const float m1 = sqrtf(-2.0f * logf(u1));
@@ -1005,7 +1059,7 @@ hipError_t launchFinishStep(unsigned long long* fFixed,
                             hipStream_t stream)
 {
     const int n3 = fFixed ? 3 * numAtoms : 0;
-    const int convertBlocks = (n3 + 255) / 256;
+    const int convertBlocks = (n3 + 511) / 512; // two components per thread
     const int reduceBlocks = (numSystems * (1 + NBNXM_NUM_SHIFTS) + 3) / 4;
     const int blocks = convertBlocks + reduceBlocks;
     if (blocks == 0)
@@ -1121,14 +1175,14 @@ hipError_t launchReduceForces(unsigned long long* fFixed, const float* fFloat, c
     return hipGetLastError();
 }
 
-hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream, unsigned int stepOffset)
+hipError_t launchPerturb(const float4* x0, float4* xq, int first, int count, float sigma, uint64_t seed, const unsigned long long* step, hipStream_t stream)
 {
     if (count <= 0)
     {
         return hipSuccess;
     }
     const uint2 key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
-    hipLaunchKernelGGL(perturbKernel, dim3((count + 255) / 256), dim3(256), 0, stream, x0, xq, first, count, sigma, key, step, stepOffset);
+    hipLaunchKernelGGL(perturbKernel, dim3((count + 255) / 256), dim3(256), 0, stream, x0, xq, first, count, sigma, key, step);
     return hipGetLastError();
 }
 

@@ -122,14 +122,12 @@ def test_graph_replays_equal_eager(systems, ens):
 
 
 @pytest.mark.parametrize("steps,use_graph,per_graph", [(2, False, 0), (3, False, 0), (5, True, 5), (4, True, 2)])
-def test_overlapped_steps_equal_single_steps(systems, ens, steps, use_graph, per_graph):
-    """A batched run of several fixed-point steps runs the perturbation of
-    the step after next and the finishing of the previous step on a second
-    stream with two buffer sets; it leaves the state of the same number of
-    one-step runs (one stream, one buffer set) bit for bit: forces, energies,
-    shift forces and positions of the last step, and the step counter (the
-    next run continues from it)."""
-    announce(f"overlapped {steps} steps (graph {use_graph}, {per_graph} per graph) vs one-step runs")
+def test_multi_step_runs_equal_single_steps(systems, ens, steps, use_graph, per_graph):
+    """A batched run of several fixed-point steps, eager or in graphs of
+    `per_graph` steps, leaves the state of the same number of one-step runs
+    bit for bit: forces, energies, shift forces and positions of the last
+    step, and the step counter (the next run continues from it)."""
+    announce(f"{steps} steps in one run (graph {use_graph}, {per_graph} per graph) vs one-step runs")
     ens.set_step(11)
     for _ in range(steps + 1):
         ens.run(1, nb.MODE_BATCHED, 0.002, 3, True)
@@ -155,10 +153,10 @@ def test_overlapped_steps_equal_single_steps(systems, ens, steps, use_graph, per
             assert np.array_equal(ens.forces(k), next_f[k]), (rep, k)
 
 
-def test_overlapped_steps_equal_single_steps_fp64(systems):
-    """The FP64 accumulation mode takes the same two-stream sequence: three
-    steps in one run equal three one-step runs bit for bit."""
-    announce("overlapped 3 steps vs one-step runs, fp64 mode")
+def test_multi_step_runs_equal_single_steps_fp64(systems):
+    """FP64 accumulation mode: three steps in one run equal three one-step
+    runs bit for bit."""
+    announce("3 steps in one run vs one-step runs, fp64 mode")
     ens64 = nb.Ensemble(systems[:3], RC, RTOL, 1.1, 1.04, nb.VDW_CUT_TYPE_TABLE, nb.ACCUM_FP64)
     ens64.set_step(5)
     for _ in range(3):
