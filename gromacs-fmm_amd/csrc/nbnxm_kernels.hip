@@ -226,6 +226,7 @@ constexpr int c_pruneWavesPerEntry = 4; // at most
 // waves 450, 1 wave 446).
 constexpr double c_pruneGroupsFor4 = 20.0;
 constexpr double c_pruneGroupsFor2 = 6.0;
+constexpr int c_pruneMinWaves = 8192; // 8 per SIMD
 
 template<bool haveFreshList>
 __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(const PruneArgs a)
@@ -988,9 +989,13 @@ hipError_t launchPrune(const PruneArgs& a, bool haveFreshList, hipStream_t strea
     // Waves per entry from the mean groups per entry: enough groups per wave
     // that its fixed costs (i-cluster loads, list words, the entry's count)
     // stay small, enough waves that a list of a few thousand entries fills
-    // the GPU.
+    // the GPU; and at least enough for c_pruneMinWaves waves in the launch (a
+    // rolling part holds a fraction of the entries: 1,226 of a 6,127-entry
+    // list in 5 parts took 8.6 us at 2 waves per entry, 6.8 us at 4).
     const double groupsPerEntry = static_cast<double>(a.numGroups) / a.nsci;
-    const int waves = groupsPerEntry >= c_pruneGroupsFor4 ? 4 : (groupsPerEntry >= c_pruneGroupsFor2 ? 2 : 1);
+    const int byGroups = groupsPerEntry >= c_pruneGroupsFor4 ? 4 : (groupsPerEntry >= c_pruneGroupsFor2 ? 2 : 1);
+    const int toFill = (c_pruneMinWaves + blocks - 1) / blocks;
+    const int waves = std::min(c_pruneWavesPerEntry, std::max(byGroups, toFill));
     if (haveFreshList)
     {
         hipLaunchKernelGGL(nbnxmPruneKernel<true>, dim3(blocks), dim3(64 * waves), 0, stream, a);
