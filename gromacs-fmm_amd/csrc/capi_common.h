@@ -44,7 +44,7 @@ struct DeviceListView
     const int* atomIndex = nullptr;              // slot -> atom, -1 for fillers
     const float4* xw = nullptr;                  // wrapped x, y, z per atom
     const nbnxm_sci_t* sci = nullptr;
-    const nbnxm_sci_t* hostSci = nullptr;        // host copy of sci
+    const nbnxm_sci_t* hostSci = nullptr;        // host copy of sci, or null (lists derived on the device)
     int nsci = 0;
     const nbnxm_cj_packed_t* cjPacked = nullptr;
     int ncjPacked = 0;

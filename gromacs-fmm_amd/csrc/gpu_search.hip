@@ -1009,7 +1009,7 @@ int downloadViewsOf(nbnxm_gpu_search* s)
 // The device part of a build, once the systems' host grids (hostGrids,
 // sysAtomStart, sysColStart), the FP64 coordinates (x) and, with anyExcl,
 // the exclusions are in place: wrap and sort, cells and clusters, the list
-// (count, scan on the host, write); downloadViews also copies everything the
+// (count, scan on the device, write); downloadViews also copies everything the
 // per-system views read to the host.
 int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, int maxPairsPerEntry, bool downloadViews)
 {
@@ -1168,20 +1168,15 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     hipLaunchKernelGGL(searchKernel<true>, dim3(numTasks), dim3(64), 0, st, P);
     NBNXM_HIP_CHECK(hipGetLastError());
 
-    // Host copies: the entries always (callers derive per-shift lists from
-    // them), the rest for the per-system views unless deferred.
-    s->hSci.resize(s->nsci);
-    if (s->nsci)
-    {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(s->hSci.data(), s->sci.ptr, sizeof(nbnxm_sci_t) * s->nsci, hipMemcpyDeviceToHost, st));
-    }
+    // Host copies for the per-system views, unless deferred: a list that
+    // stays on the device is not waited for here (its users order their
+    // work after this stream's).
     s->rlistF = static_cast<float>(rlist);
     s->viewsCurrent = false;
     if (downloadViews)
     {
         return downloadViewsOf(s);
     }
-    NBNXM_HIP_CHECK(hipStreamSynchronize(st));
     return NBNXM_OK;
 }
 
@@ -1484,7 +1479,7 @@ int gpuSearchBuildDevice(nbnxm_gpu_search_t* s,
     view->atomIndex = s->atomIndex.ptr;
     view->xw = s->xw.ptr;
     view->sci = s->sci.ptr;
-    view->hostSci = s->hSci.data();
+    view->hostSci = nullptr; // the task derives its entry lists on the device
     view->nsci = s->nsci;
     view->cjPacked = s->cjPacked.ptr;
     view->ncjPacked = s->ngroups;

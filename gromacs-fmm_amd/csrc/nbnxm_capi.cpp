@@ -227,7 +227,11 @@ struct nbnxm_gpu
     DeviceBuffer<int> entryList;
     DeviceBuffer<int> shiftRanges;
     DeviceBuffer<int> shiftEntryList;
+    DeviceBuffer<int> shiftKeys;          // a device list's entry shifts, and sorted
+    DeviceBuffer<int> shiftKeysSorted;
     DeviceBuffer<int> slotRange;          // {0, numAtoms}
+    bool shiftVecOfBox = false;           // shiftVec holds the shifts of shiftVecBox
+    double shiftVecBox[3] = { 0.0, 0.0, 0.0 };
     DeviceBuffer<int> atomIndex;          // grid slot -> atom or -1 (buffer ops)
     DeviceBuffer<int> cellIndex;          // atom -> grid slot (buffer ops)
     int numRealAtoms = -1;                // atoms of the buffer-op map, -1 = none
@@ -482,7 +486,8 @@ namespace
 {
 
 // The list of a task from host or device arrays (kind), with the per-shift
-// entry lists derived from the host copy of the entries.
+// entry lists derived from the host copy of the entries, or on the device
+// when there is none (hostSci null: a list that stays on the device).
 int setPairlist(nbnxm_gpu* nb,
                 const nbnxm_sci_t* hostSci,
                 const nbnxm_sci_t* sciSrc,
@@ -521,39 +526,72 @@ int setPairlist(nbnxm_gpu* nb,
     }
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->excl.ptr, exclSrc, sizeof(nbnxm_excl_t) * nexcl, kind, nb->stream));
 
-    // Identity work order and the per-shift entry lists for the FP64 reductions.
-    std::vector<int> iota(nsci);
-    std::iota(iota.begin(), iota.end(), 0);
-    std::vector<int> shiftRanges(NBNXM_NUM_SHIFTS + 1, 0);
-    for (int i = 0; i < nsci; i++)
-    {
-        shiftRanges[hostSci[i].shift + 1]++;
-    }
-    for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
-    {
-        shiftRanges[k + 1] += shiftRanges[k];
-    }
-    std::vector<int> shiftList(nsci);
-    {
-        std::vector<int> fillPos(shiftRanges.begin(), shiftRanges.end() - 1);
-        for (int i = 0; i < nsci; i++)
-        {
-            shiftList[fillPos[hostSci[i].shift]++] = i;
-        }
-    }
-    const int entryRanges[2] = { 0, nsci };
+    // Identity work order and the per-shift entry lists for the FP64
+    // reductions: entry i in shift list s(i), in entry order within a list.
     NBNXM_HIP_CHECK(nb->entryRanges.resize(2));
     NBNXM_HIP_CHECK(nb->entryList.resize(std::max(nsci, 1)));
     NBNXM_HIP_CHECK(nb->shiftRanges.resize(NBNXM_NUM_SHIFTS + 1));
     NBNXM_HIP_CHECK(nb->shiftEntryList.resize(std::max(nsci, 1)));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryRanges.ptr, entryRanges, sizeof(entryRanges), hipMemcpyHostToDevice, nb->stream));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftRanges.ptr, shiftRanges.data(), sizeof(int) * shiftRanges.size(), hipMemcpyHostToDevice, nb->stream));
-    if (nsci > 0)
+    size_t shiftSortBytes = 0;
+    if (hostSci)
     {
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryList.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sciOrder.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->iota.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
-        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        std::vector<int> iota(nsci);
+        std::iota(iota.begin(), iota.end(), 0);
+        std::vector<int> shiftRanges(NBNXM_NUM_SHIFTS + 1, 0);
+        for (int i = 0; i < nsci; i++)
+        {
+            shiftRanges[hostSci[i].shift + 1]++;
+        }
+        for (int k = 0; k < NBNXM_NUM_SHIFTS; k++)
+        {
+            shiftRanges[k + 1] += shiftRanges[k];
+        }
+        std::vector<int> shiftList(nsci);
+        {
+            std::vector<int> fillPos(shiftRanges.begin(), shiftRanges.end() - 1);
+            for (int i = 0; i < nsci; i++)
+            {
+                shiftList[fillPos[hostSci[i].shift]++] = i;
+            }
+        }
+        const int entryRanges[2] = { 0, nsci };
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryRanges.ptr, entryRanges, sizeof(entryRanges), hipMemcpyHostToDevice, nb->stream));
+        NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftRanges.ptr, shiftRanges.data(), sizeof(int) * shiftRanges.size(), hipMemcpyHostToDevice, nb->stream));
+        if (nsci > 0)
+        {
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->entryList.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->sciOrder.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->iota.ptr, iota.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+            NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftEntryList.ptr, shiftList.data(), sizeof(int) * nsci, hipMemcpyHostToDevice, nb->stream));
+        }
+    }
+    else
+    {
+        // The same arrays from the device copy of the entries: identity
+        // orders and shift keys, a stable sort of the entries by shift (the
+        // entry order within a shift kept), the ranges from the sorted keys.
+        NBNXM_HIP_CHECK(nb->shiftKeys.resize(std::max(nsci, 1)));
+        NBNXM_HIP_CHECK(nb->shiftKeysSorted.resize(std::max(nsci, 1)));
+        NBNXM_HIP_CHECK(launchEntryArrays(nb->sci.ptr, nsci, nb->entryList.ptr, nb->sciOrder.ptr, nb->iota.ptr, nb->shiftKeys.ptr, nb->entryRanges.ptr,
+                                          nb->stream));
+        int shiftBits = 1;
+        while ((1 << shiftBits) < NBNXM_NUM_SHIFTS)
+        {
+            shiftBits++;
+        }
+        NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, shiftSortBytes, nb->shiftKeys.ptr, nb->shiftKeysSorted.ptr, nb->iota.ptr,
+                                                           nb->shiftEntryList.ptr, std::max(nsci, 1), 0, shiftBits, nb->stream));
+        // Sized once for this sort and the sci sort, before either runs.
+        size_t sciSortBytes = 0;
+        NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sciSortBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr,
+                                                           std::max(nsci, 1), 0, 14, nb->stream));
+        NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(std::max(shiftSortBytes, sciSortBytes), 1)));
+        if (nsci > 0)
+        {
+            NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nb->sortTemp.ptr, shiftSortBytes, nb->shiftKeys.ptr, nb->shiftKeysSorted.ptr, nb->iota.ptr,
+                                                               nb->shiftEntryList.ptr, nsci, 0, shiftBits, nb->stream));
+        }
+        NBNXM_HIP_CHECK(launchShiftRanges(nb->shiftKeysSorted.ptr, nsci, nb->shiftRanges.ptr, nb->stream));
     }
     NBNXM_HIP_CHECK(launchFillI32(nb->rollingPart.ptr, rollingSize, 0, nb->stream));
     // Outer masks start empty: the prune kernels write those of the groups
@@ -562,11 +600,16 @@ int setPairlist(nbnxm_gpu* nb,
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->imaskOuter.ptr), ncj, 0, nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->partials.ptr), 2 * 8 * static_cast<size_t>(std::max(nsci, 1)), 0, nb->stream));
 
-    // Radix-sort scratch for the sci ordering.
+    // Radix-sort scratch for the sci ordering (the shift sort above, if any,
+    // ran before it on the same stream).
     size_t tempBytes = 0;
     NBNXM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, nb->sciCount.ptr, nb->sciCountSorted.ptr, nb->iota.ptr, nb->sciOrder.ptr, std::max(nsci, 1), 0, 14, nb->stream));
-    NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(tempBytes, 1)));
-    NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    NBNXM_HIP_CHECK(nb->sortTemp.resize(std::max<size_t>(std::max(tempBytes, shiftSortBytes), 1)));
+    if (hostSci)
+    {
+        // The copies above read host arrays of this call.
+        NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    }
     nb->haveFreshList = true;
     nb->haveSortedOrder = false;
     nb->haveList = true;
@@ -1181,6 +1224,7 @@ int nbnxm_gpu_upload_shiftvec(nbnxm_gpu_t* nb, const float* shiftVec)
     NBNXM_HIP_CHECK(nb->shiftVec.resize(NBNXM_NUM_SHIFTS));
     NBNXM_HIP_CHECK(hipMemcpyAsync(nb->shiftVec.ptr, sv, sizeof(sv), hipMemcpyHostToDevice, nb->stream));
     NBNXM_HIP_CHECK(hipStreamSynchronize(nb->stream));
+    nb->shiftVecOfBox = false;
     return NBNXM_OK;
 }
 
@@ -1915,7 +1959,7 @@ int nbnxm_amd::gpuTaskAdoptDeviceList(nbnxm_gpu_t* nb,
     const bool needTypes = nb && (nb->vdwType == NBNXM_VDW_CUT_TYPE_TABLE || nb->ic.ljPme);
     const bool needComb = nb && nb->vdwType != NBNXM_VDW_CUT_TYPE_TABLE;
     if (!nb || !box || v.numAtoms < 1 || v.numSlots < v.numAtoms || !v.atomIndex || !v.xw || !charges || !types
-        || (needComb && !typeComb) || v.nsci < 0 || v.ncjPacked < 0 || v.nexcl < 1 || (v.nsci > 0 && (!v.sci || !v.hostSci))
+        || (needComb && !typeComb) || v.nsci < 0 || v.ncjPacked < 0 || v.nexcl < 1 || (v.nsci > 0 && !v.sci)
         || (v.ncjPacked > 0 && !v.cjPacked) || !v.excl)
     {
         return fail(NBNXM_ERR_INVALID, "gpu adopt device list: null pointer or bad size");
@@ -1954,23 +1998,27 @@ int nbnxm_amd::gpuTaskAdoptDeviceList(nbnxm_gpu_t* nb,
     NBNXM_HIP_CHECK(launchFillU64(nb->fFixed.ptr, 3 * static_cast<size_t>(n), nb->stream));
     NBNXM_HIP_CHECK(launchFillI32(reinterpret_cast<int*>(nb->fFloat.ptr), 3 * static_cast<size_t>(n), 0, nb->stream));
     nb->virialBlocks = std::min(512, std::max(1, (n + 1023) / 1024));
-    const int range[2] = { 0, n };
     NBNXM_HIP_CHECK(nb->slotRange.resize(2));
     NBNXM_HIP_CHECK(nb->virialPartials.resize(9 * static_cast<size_t>(nb->virialBlocks)));
     NBNXM_HIP_CHECK(nb->virial.resize(9));
-    NBNXM_HIP_CHECK(hipMemcpyAsync(nb->slotRange.ptr, range, sizeof(range), hipMemcpyHostToDevice, nb->stream));
+    NBNXM_HIP_CHECK(launchStoreInt2(nb->slotRange.ptr, 0, n, nb->stream));
 
-    // Shift vectors of the box, then the list (its host steps synchronise
-    // the task stream, which also retires the copies above).
-    float shiftVec[3 * NBNXM_NUM_SHIFTS];
-    int rc = nbnxm_calc_shifts(box, shiftVec);
-    if (rc == NBNXM_OK)
+    // Shift vectors of the box (uploaded when the box is not the one they
+    // were last computed for), then the list.
+    if (!nb->shiftVecOfBox || box[0] != nb->shiftVecBox[0] || box[1] != nb->shiftVecBox[1] || box[2] != nb->shiftVecBox[2])
     {
-        rc = nbnxm_gpu_upload_shiftvec(nb, shiftVec);
-    }
-    if (rc != NBNXM_OK)
-    {
-        return rc;
+        float shiftVec[3 * NBNXM_NUM_SHIFTS];
+        int rc = nbnxm_calc_shifts(box, shiftVec);
+        if (rc == NBNXM_OK)
+        {
+            rc = nbnxm_gpu_upload_shiftvec(nb, shiftVec);
+        }
+        if (rc != NBNXM_OK)
+        {
+            return rc;
+        }
+        nb->shiftVecOfBox = true;
+        std::copy(box, box + 3, nb->shiftVecBox);
     }
     return setPairlist(nb, v.hostSci, v.sci, v.nsci, v.cjPacked, v.ncjPacked, v.excl, v.nexcl, hipMemcpyDeviceToDevice);
 }

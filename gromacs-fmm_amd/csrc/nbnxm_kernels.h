@@ -279,6 +279,15 @@ hipError_t launchAdvanceStep(unsigned long long* step, hipStream_t stream);
 // Fill kernels (graph-safe clears).
 hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream);
 hipError_t launchFillI32(int* p, size_t n, int value, hipStream_t stream);
+// p[0] = a, p[1] = b (one thread).
+hipError_t launchStoreInt2(int* p, int a, int b, hipStream_t stream);
+// Work-order arrays of a list whose n entries are in device memory:
+// entryList, sciOrder and iota = 0..n-1, shiftKeys[i] = sci[i].shift and
+// entryRanges = {0, n}.
+hipError_t launchEntryArrays(const nbnxm_sci_t* sci, int n, int* entryList, int* sciOrder, int* iota, int* shiftKeys, int* entryRanges, hipStream_t stream);
+// shiftRanges[k] (k = 0..NBNXM_NUM_SHIFTS) = number of the n sorted shift
+// keys below k: the ranges of the per-shift entry lists.
+hipError_t launchShiftRanges(const int* sortedKeys, int n, int* shiftRanges, hipStream_t stream);
 // Convert and clear the fixed-point forces (fFixed may be null: no forces to
 // convert), reduce energies and shift forces per system and advance the step
 // counter (step may be null), in one launch.

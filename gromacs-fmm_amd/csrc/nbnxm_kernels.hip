@@ -919,6 +919,55 @@ __global__ void fillI32Kernel(int* p, size_t n, int value)
     }
 }
 
+// Work-order arrays of a list whose entries are on the device: 0..n-1 into
+// three arrays, each entry's shift index as a sort key, and {0, n}.
+__global__ void entryArraysKernel(const nbnxm_sci_t* sci, int n, int* entryList, int* sciOrder, int* iota, int* shiftKeys, int* entryRanges)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        entryList[i] = i;
+        sciOrder[i] = i;
+        iota[i] = i;
+        shiftKeys[i] = sci[i].shift;
+    }
+    if (i == 0)
+    {
+        entryRanges[0] = 0;
+        entryRanges[1] = n;
+    }
+}
+
+// shiftRanges[k] = number of sorted keys below k, k = 0..numShifts.
+__global__ void shiftRangesKernel(const int* sortedKeys, int n, int numShifts, int* shiftRanges)
+{
+    const int k = threadIdx.x;
+    if (k > numShifts)
+    {
+        return;
+    }
+    int lo = 0, hi = n;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (sortedKeys[mid] < k)
+        {
+            lo = mid + 1;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    shiftRanges[k] = lo;
+}
+
+__global__ void storeInt2Kernel(int* p, int a, int b)
+{
+    p[0] = a;
+    p[1] = b;
+}
+
 template<bool E, int V>
 hipError_t launchFp64(const NbKernelArgs& a, const NbLaunchConfig& cfg, hipStream_t stream)
 {
@@ -1204,6 +1253,26 @@ hipError_t launchFillU64(unsigned long long* p, size_t n, hipStream_t stream)
         return hipSuccess;
     }
     hipLaunchKernelGGL(fillU64Kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launchEntryArrays(const nbnxm_sci_t* sci, int n, int* entryList, int* sciOrder, int* iota, int* shiftKeys, int* entryRanges, hipStream_t stream)
+{
+    const int blocks = std::max(1, (n + 255) / 256);
+    hipLaunchKernelGGL(entryArraysKernel, dim3(blocks), dim3(256), 0, stream, sci, n, entryList, sciOrder, iota, shiftKeys, entryRanges);
+    return hipGetLastError();
+}
+
+hipError_t launchShiftRanges(const int* sortedKeys, int n, int* shiftRanges, hipStream_t stream)
+{
+    static_assert(NBNXM_NUM_SHIFTS + 1 <= 64, "one wave covers the ranges");
+    hipLaunchKernelGGL(shiftRangesKernel, dim3(1), dim3(64), 0, stream, sortedKeys, n, NBNXM_NUM_SHIFTS, shiftRanges);
+    return hipGetLastError();
+}
+
+hipError_t launchStoreInt2(int* p, int a, int b, hipStream_t stream)
+{
+    hipLaunchKernelGGL(storeInt2Kernel, dim3(1), dim3(1), 0, stream, p, a, b);
     return hipGetLastError();
 }
 

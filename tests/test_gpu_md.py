@@ -219,3 +219,36 @@ def test_device_search_with_exclusions():
         d_x.upload(x)
     assert checked == 3
     assert ver.stats()["searches"] == 4
+
+
+def test_device_search_equals_host_search_bitwise():
+    """Fixed coordinates, a search every 4 steps: the device search builds the
+    host search's lists (same split target, since every list has the same
+    cluster-pair count), and a task that adopts a device list derives its
+    work order and per-shift entry lists on the device.  Forces, energies and
+    virial of every step (rolling prunes included) equal those of the host
+    search bit for bit, so the FP64 reductions read the same entries in the
+    same order."""
+    announce("md water host search vs device search bitwise")
+    w = nb.systems.water_box(1000, None, seed=6)
+    ic = nb.interaction_const_ex(1.0, ewald_rtol=1e-5)
+    params = np.asarray(w.c6c12, dtype=np.float64) * np.array([6.0, 12.0])
+    runs = {}
+    for gpu_search in (False, True):
+        ver = nb.NonbondedVerlet(w.types, params, w.q, ic, nstlist=4, rlist_outer=1.12, rlist_inner=1.04,
+                                 nstlist_prune=2, excl_ranges=w.excl_ranges, excl_elements=w.excl_elements,
+                                 gpu_search=gpu_search)
+        d_x = nb.DeviceArray(w.x.astype(np.float32))
+        d_f = nb.DeviceArray.zeros(w.x.shape)
+        out = []
+        for step in range(11):
+            ver.step(step, w.box, d_x.ptr, d_f.ptr, 0, compute_energy=True, compute_virial=True)
+            out.append((d_f.download(), ver.energies(), ver.virial()))
+        st = ver.stats()
+        assert st["searches"] == 3 and st["prunes"] > 0
+        runs[gpu_search] = out
+    for step, (host, dev) in enumerate(zip(runs[False], runs[True])):
+        assert np.array_equal(host[0], dev[0]), step
+        assert host[1] == dev[1], step
+        assert np.array_equal(np.asarray(host[2]), np.asarray(dev[2])), step
+
