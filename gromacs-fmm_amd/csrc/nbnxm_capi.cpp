@@ -209,7 +209,6 @@ struct nbnxm_gpu
     DeviceBuffer<int> rollingPart;
     DeviceBuffer<int> sciCount;
     DeviceBuffer<int> sciCountSorted;
-    DeviceBuffer<int> sciHistogram;
     DeviceBuffer<int> iota;
     DeviceBuffer<char> sortTemp;
     bool haveFreshList = false;
@@ -510,7 +509,6 @@ int setPairlist(nbnxm_gpu* nb,
     NBNXM_HIP_CHECK(nb->sciCount.resize(nsci));
     NBNXM_HIP_CHECK(nb->sciCountSorted.resize(nsci));
     NBNXM_HIP_CHECK(nb->iota.resize(nsci));
-    NBNXM_HIP_CHECK(nb->sciHistogram.resize(NBNXM_SCI_HISTOGRAM_SIZE));
     NBNXM_HIP_CHECK(nb->partials.resize(8 * static_cast<size_t>(std::max(nsci, 1))));
     const int wavesPerBlockPrune = 4;
     const int rollingSize = ((nsci + wavesPerBlockPrune - 1) / wavesPerBlockPrune + 1) * wavesPerBlockPrune;
@@ -1295,7 +1293,11 @@ int nbnxm_gpu_clear_outputs(nbnxm_gpu_t* nb, int computeVirial)
 // After the prune of a fresh list (by the prune kernel or the combined one):
 // entries ordered by descending pruned pair count (the role of
 // nbnxnKernelBucketSciSort, src/gromacs/nbnxm/cuda/nbnxm_cuda_kernel_sci_sort.cuh:66-87);
-// a stable radix sort on the histogram bin keeps the order deterministic.
+// a stable radix sort on each entry's bin (NBNXM_SCI_HISTOGRAM_SIZE - 1 -
+// count, clamped at 0) keeps the order deterministic.  The reference's
+// counting sort needs the bins' histogram; the radix sort does not, so no
+// atomic counts are kept (on a list of 15,000 nearly equal entries those
+// atomics on a few bins doubled the prune kernel's time).
 static int sortEntriesOfFreshList(nbnxm_gpu_t* nb)
 {
     if (nb->nsci > 0)
@@ -1322,7 +1324,6 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
     pa.imaskOuter = nb->imaskOuter.ptr;
     pa.rollingPart = nb->rollingPart.ptr;
     pa.sciCount = nb->sciCount.ptr;
-    pa.sciHistogram = nb->sciHistogram.ptr;
     // Without dynamic pruning the list is pruned to rlistOuter only, as the
     // reference's combined kernel does.
     const float ro = static_cast<float>(nb->listParams.rlistOuter);
@@ -1338,7 +1339,6 @@ static int launchPruneImpl(nbnxm_gpu_t* nb, int numParts)
         {
             return fail(NBNXM_ERR_INVALID, "gpu_launch_kernel_pruneonly: a fresh list is pruned in one part");
         }
-        NBNXM_HIP_CHECK(launchFillI32(nb->sciHistogram.ptr, NBNXM_SCI_HISTOGRAM_SIZE, 0, nb->stream));
     }
     int trc = timingBegin(nb, fresh ? c_timePrune : c_timeRollingPrune);
     if (trc != NBNXM_OK)
@@ -1431,9 +1431,7 @@ int nbnxm_gpu_launch_kernel(nbnxm_gpu_t* nb, int computeEnergy, int computeViria
         a.pruneCjPacked = nb->cjPacked.ptr;
         a.pruneImaskOuter = nb->imaskOuter.ptr;
         a.pruneSciCount = nb->sciCount.ptr;
-        a.pruneSciHistogram = nb->sciHistogram.ptr;
         a.pruneRlist_sq = ro * ro;
-        NBNXM_HIP_CHECK(launchFillI32(nb->sciHistogram.ptr, NBNXM_SCI_HISTOGRAM_SIZE, 0, nb->stream));
     }
     const int trc = timingBegin(nb, c_timeNonbonded + 2 * (freshList ? 1 : 0) + (computeEnergy ? 1 : 0));
     if (trc != NBNXM_OK)
@@ -1864,8 +1862,13 @@ int nbnxm_gpu_min_ci_balanced(nbnxm_gpu_t* nb)
     int device = 0, cus = 0;
     NBNXM_HIP_CHECK(hipGetDevice(&device));
     NBNXM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    // Enough entries for several waves on every SIMD (4 per CU).
-    return 4 * 4 * cus;
+    // 50 entries per compute unit, the factor of the reference's OpenCL and
+    // SYCL back ends (nbnxm_ocl_data_mgmt.cpp:86, nbnxm_sycl_data_mgmt.cpp:68-80;
+    // CUDA uses 44-61 per SM): on a 96,000-atom water list 12,000-22,000
+    // entries ran the force launch in about half the time of the 2,500
+    // unsplit or 4,200 entries, and a list that already has more entries
+    // than this (3 M atoms: 56,000) is not split.
+    return 50 * cus;
 }
 
 int nbnxm_gpu_is_kernel_ewald_analytical(const nbnxm_gpu_t* nb)

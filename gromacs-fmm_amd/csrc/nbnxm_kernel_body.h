@@ -230,7 +230,7 @@ constexpr int minWavesPerSimd()
 // dynamic pruning (the _prune kernels, nbnxm_cuda_kernel.cuh:145-157,494-501,
 // 680-685,722-741): each cluster pair of the list with no atom pair inside
 // rlist leaves the group's mask, which is written back (and to the outer mask),
-// and the entry's pruned cluster-pair count goes to the histogram bins of the
+// and the entry's pruned cluster-pair count gives its bin, the key of the
 // entry sort.
 template<bool computeEnergy, int vdwType, int accumulation, int kFlav, bool kILds = false, bool kPrune = false>
 __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav, kILds>())) void nbnxmNonbondedKernel(const NbKernelArgs a, const NbFlavourParams flav, int wavesPerBlock, int stageOffset)
@@ -907,12 +907,10 @@ __global__ __launch_bounds__(256, (minWavesPerSimd<computeEnergy, vdwType, kFlav
         const int tj = lane >> 3;
         if constexpr (kPrune)
         {
-            // The entry's histogram bin for the entry sort (:722-741); one wave per entry.
+            // The entry's bin, the key of the entry sort (:722-741); one wave per entry.
             if (lane == 0)
             {
-                const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
-                atomicAdd(a.pruneSciHistogram + index, 1);
-                a.pruneSciCount[work] = index;
+                a.pruneSciCount[work] = max(NBNXM_SCI_HISTOGRAM_SIZE - prunedPairCount - 1, 0);
             }
         }
 
@@ -1009,7 +1007,7 @@ hipError_t launchTunedFlavour(const NbKernelArgs& a, const NbFlavourParams& flav
     // A fresh list without dynamic pruning: the combined prune kernel (its
     // outputs set, entries in list order).
     const bool prune = a.pruneCjPacked != nullptr;
-    if (prune && (!a.pruneImaskOuter || !a.pruneSciCount || !a.pruneSciHistogram || a.sciOrder))
+    if (prune && (!a.pruneImaskOuter || !a.pruneSciCount || a.sciOrder))
     {
         return hipErrorInvalidValue;
     }

@@ -41,11 +41,10 @@ struct NbKernelArgs
 
     // Combined prune of a fresh list (set only for that launch; entries in
     // list order): masks written back in place and to the outer masks, the
-    // histogram bin of each entry for the entry sort.
+    // bin of each entry (its sort key) for the entry sort.
     nbnxm_cj_packed_t* pruneCjPacked = nullptr;
     unsigned int* pruneImaskOuter = nullptr;
     int* pruneSciCount = nullptr;
-    int* pruneSciHistogram = nullptr;
     float pruneRlist_sq = 0.0f;
 };
 
@@ -200,8 +199,7 @@ struct PruneArgs
     nbnxm_cj_packed_t* cjPacked = nullptr; // imei[0].imask updated (inner mask)
     unsigned int* imaskOuter = nullptr;    // per packed group
     int* rollingPart = nullptr;            // per block part index (device), as the reference
-    int* sciCount = nullptr;               // histogram bin per entry (fresh list)
-    int* sciHistogram = nullptr;           // NBNXM_SCI_HISTOGRAM_SIZE bins (fresh list)
+    int* sciCount = nullptr;               // bin per entry, the sort key (fresh list)
     float rlistOuter_sq = 0;
     float rlistInner_sq = 0;
     int numParts = 1;

@@ -419,10 +419,6 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
                 total += sCount[k];
             }
             const int index = max(NBNXM_SCI_HISTOGRAM_SIZE - total - 1, 0);
-            if (a.sciHistogram)
-            {
-                atomicAdd(a.sciHistogram + index, 1);
-            }
             if (a.sciCount)
             {
                 a.sciCount[work] = index;
