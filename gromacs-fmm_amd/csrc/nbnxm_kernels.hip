@@ -455,22 +455,28 @@ __global__ void convertForcesKernel(unsigned long long* fFixed, int n3, float* f
     convertForce(i, fFixed, fOut, fOutDouble, clear);
 }
 
-// Lane `lane` of a wave adds fields first..first+N-1 of the entry partials of
-// list[k] for k = begin + lane, begin + lane + 64, ... in that order; four
-// entries' loads are issued before their adds, so a long list (the central
-// shift of one large system) is not one dependent load chain per entry.
+// One workgroup of c_reduceThreads per (system, slot): slot 0 = energies,
+// slots 1..45 = shift vector s-1.  Thread t adds fields first..first+N-1 of
+// the entry partials of list[k] for k = begin + t, begin + t + c_reduceThreads,
+// ... in that order, four entries' loads issued before their adds; the
+// workgroup then sums its waves' totals in wave order.  The order is fixed,
+// so the sums are bitwise reproducible, and every launch path uses this
+// function.
+constexpr int c_reduceThreads = 256;
+
 template<int N>
-__device__ __forceinline__ void sumEntryPartials(const int* list, int begin, int end, int lane, const double* partials, int first, double (&acc)[N])
+__device__ __forceinline__ void sumEntryPartials(const int* list, int begin, int end, const double* partials, int first, double (&acc)[N])
 {
-    int k = begin + lane;
-    for (; k + 3 * 64 < end; k += 4 * 64)
+    constexpr int stride = c_reduceThreads;
+    int k = begin + static_cast<int>(threadIdx.x);
+    for (; k + 3 * stride < end; k += 4 * stride)
     {
         int e[4];
         double v[4][N];
 #pragma unroll
         for (int u = 0; u < 4; u++)
         {
-            e[u] = list[k + u * 64];
+            e[u] = list[k + u * stride];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -491,7 +497,7 @@ __device__ __forceinline__ void sumEntryPartials(const int* list, int begin, int
             }
         }
     }
-    for (; k < end; k += 64)
+    for (; k < end; k += stride)
     {
         const int e = list[k];
 #pragma unroll
@@ -500,12 +506,39 @@ __device__ __forceinline__ void sumEntryPartials(const int* list, int begin, int
             acc[c] += partials[8 * static_cast<size_t>(e) + first + c];
         }
     }
+    // Wave totals, then the workgroup's in wave order (thread 0 holds them).
+    __shared__ double sWave[c_reduceThreads / 64][N];
+#pragma unroll
+    for (int c = 0; c < N; c++)
+    {
+        acc[c] = reduceWaveD(acc[c]);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+#pragma unroll
+        for (int c = 0; c < N; c++)
+        {
+            sWave[threadIdx.x >> 6][c] = acc[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int c = 0; c < N; c++)
+        {
+            double t = sWave[0][c];
+            for (int w = 1; w < c_reduceThreads / 64; w++)
+            {
+                t += sWave[w][c];
+            }
+            acc[c] = t;
+        }
+    }
 }
 
-// One wave per (system, slot): slot 0 = energies, slots 1..45 = shift vector s-1.
 __device__ __forceinline__ void reduceEnergySlot(int sys,
                                                  int slot,
-                                                 int lane,
                                                  const double* partials,
                                                  const int* entryRanges,
                                                  const int* entryList,
@@ -518,13 +551,11 @@ __device__ __forceinline__ void reduceEnergySlot(int sys,
     if (slot == 0)
     {
         double acc[2] = { 0.0, 0.0 };
-        sumEntryPartials(entryList, entryRanges[sys], entryRanges[sys + 1], lane, partials, 0, acc);
-        const double e0 = reduceWaveD(acc[0]);
-        const double e1 = reduceWaveD(acc[1]);
-        if (lane == 0)
+        sumEntryPartials(entryList, entryRanges[sys], entryRanges[sys + 1], partials, 0, acc);
+        if (threadIdx.x == 0)
         {
-            energiesOut[2 * sys] = e0;
-            energiesOut[2 * sys + 1] = e1;
+            energiesOut[2 * sys] = acc[0];
+            energiesOut[2 * sys + 1] = acc[1];
         }
         return;
     }
@@ -534,29 +565,26 @@ __device__ __forceinline__ void reduceEnergySlot(int sys,
     }
     const int sidx = sys * NBNXM_NUM_SHIFTS + slot - 1;
     double acc[3] = { 0.0, 0.0, 0.0 };
-    sumEntryPartials(shiftEntryList, shiftRanges[sidx], shiftRanges[sidx + 1], lane, partials, 2, acc);
-    const double f0 = reduceWaveD(acc[0]);
-    const double f1 = reduceWaveD(acc[1]);
-    const double f2 = reduceWaveD(acc[2]);
-    if (lane == 0)
+    sumEntryPartials(shiftEntryList, shiftRanges[sidx], shiftRanges[sidx + 1], partials, 2, acc);
+    if (threadIdx.x == 0)
     {
-        fshiftOut[3 * sidx + 0] = f0;
-        fshiftOut[3 * sidx + 1] = f1;
-        fshiftOut[3 * sidx + 2] = f2;
+        fshiftOut[3 * sidx + 0] = acc[0];
+        fshiftOut[3 * sidx + 1] = acc[1];
+        fshiftOut[3 * sidx + 2] = acc[2];
     }
 }
 
-__global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partials,
-                                                           const int* entryRanges,
-                                                           const int* entryList,
-                                                           const int* shiftRanges,
-                                                           const int* shiftEntryList,
-                                                           int computeFshift,
-                                                           double* energiesOut,
-                                                           double* fshiftOut)
+__global__ __launch_bounds__(c_reduceThreads) void reduceEnergiesKernel(const double* partials,
+                                                                        const int* entryRanges,
+                                                                        const int* entryList,
+                                                                        const int* shiftRanges,
+                                                                        const int* shiftEntryList,
+                                                                        int computeFshift,
+                                                                        double* energiesOut,
+                                                                        double* fshiftOut)
 {
-    reduceEnergySlot(blockIdx.x / (1 + NBNXM_NUM_SHIFTS), blockIdx.x % (1 + NBNXM_NUM_SHIFTS), threadIdx.x, partials, entryRanges,
-                     entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    reduceEnergySlot(blockIdx.x / (1 + NBNXM_NUM_SHIFTS), blockIdx.x % (1 + NBNXM_NUM_SHIFTS), partials, entryRanges, entryList, shiftRanges,
+                     shiftEntryList, computeFshift, energiesOut, fshiftOut);
 }
 
 // The end of a batched step in one launch: blocks [0, convertBlocks) convert
@@ -565,7 +593,7 @@ __global__ __launch_bounds__(64) void reduceEnergiesKernel(const double* partial
 // forces (as reduceEnergiesKernel), and thread 0 of block 0 advances the step
 // counter (as advanceStepKernel; the step's perturbation read it before the
 // nonbonded launch).  One launch instead of three dependent ones.
-__global__ __launch_bounds__(256) void finishStepKernel(unsigned long long* fFixed,
+__global__ __launch_bounds__(c_reduceThreads) void finishStepKernel(unsigned long long* fFixed,
                                                         int n3,
                                                         float* fOut,
                                                         double* fOutDouble,
@@ -612,13 +640,10 @@ __global__ __launch_bounds__(256) void finishStepKernel(unsigned long long* fFix
         }
         return;
     }
-    const int pair = (static_cast<int>(blockIdx.x) - convertBlocks) * (blockDim.x / 64) + static_cast<int>(threadIdx.x / 64);
-    if (pair >= numSystems * (1 + NBNXM_NUM_SHIFTS))
-    {
-        return;
-    }
-    reduceEnergySlot(pair / (1 + NBNXM_NUM_SHIFTS), pair % (1 + NBNXM_NUM_SHIFTS), threadIdx.x & 63, partials, entryRanges, entryList,
-                     shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    // One workgroup per (system, slot) after the conversion blocks.
+    const int pair = static_cast<int>(blockIdx.x) - convertBlocks;
+    reduceEnergySlot(pair / (1 + NBNXM_NUM_SHIFTS), pair % (1 + NBNXM_NUM_SHIFTS), partials, entryRanges, entryList, shiftRanges, shiftEntryList,
+                     computeFshift, energiesOut, fshiftOut);
 }
 
 // Single-sum virial, atom part: per block FP64 partial sums of x (x) f over the
@@ -1088,7 +1113,8 @@ hipError_t launchReduceEnergies(const double* entryPartials,
     {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(reduceEnergiesKernel, dim3(numSystems * (1 + NBNXM_NUM_SHIFTS)), dim3(64), 0, stream, entryPartials, entryRanges, entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
+    hipLaunchKernelGGL(reduceEnergiesKernel, dim3(numSystems * (1 + NBNXM_NUM_SHIFTS)), dim3(c_reduceThreads), 0, stream, entryPartials, entryRanges,
+                       entryList, shiftRanges, shiftEntryList, computeFshift, energiesOut, fshiftOut);
     return hipGetLastError();
 }
 
@@ -1109,14 +1135,14 @@ hipError_t launchFinishStep(unsigned long long* fFixed,
                             hipStream_t stream)
 {
     const int n3 = fFixed ? 3 * numAtoms : 0;
-    const int convertBlocks = (n3 + 511) / 512; // two components per thread
-    const int reduceBlocks = (numSystems * (1 + NBNXM_NUM_SHIFTS) + 3) / 4;
+    const int convertBlocks = (n3 + 2 * c_reduceThreads - 1) / (2 * c_reduceThreads); // two components per thread
+    const int reduceBlocks = numSystems * (1 + NBNXM_NUM_SHIFTS);
     const int blocks = convertBlocks + reduceBlocks;
     if (blocks == 0)
     {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(finishStepKernel, dim3(blocks), dim3(256), 0, stream, fFixed, n3, fOut, fOutDouble, convertBlocks, entryPartials,
+    hipLaunchKernelGGL(finishStepKernel, dim3(blocks), dim3(c_reduceThreads), 0, stream, fFixed, n3, fOut, fOutDouble, convertBlocks, entryPartials,
                        entryRanges, entryList, shiftRanges, shiftEntryList, numSystems, computeFshift, energiesOut, fshiftOut, step);
     return hipGetLastError();
 }

@@ -82,38 +82,71 @@ __global__ void writeParamBlocksKernel(ParamBlocks* dst, ParamBlocks src)
 // The sums of reduceEnergiesKernel (nbnxm_kernels.hip) for one system whose
 // entry lists the host never saw: slot 0 adds the energies of all entries in
 // entry order, slot 1 + s the shift forces of the entries with shift s in
-// entry order.  Lane l adds positions l, l + 64, ... of each list, as there,
-// so the sums are the same bits: the positions of shift s are found with a
-// ballot per chunk of 64 entries and handed to their lanes through LDS.
-__global__ __launch_bounds__(64) void reduceEntriesInOrderKernel(const double* partials,
-                                                                 const nbnxm_sci_t* sci,
-                                                                 int nsci,
-                                                                 int computeEnergy,
-                                                                 int computeFshift,
-                                                                 double* energies,
-                                                                 double* fshift)
+// entry order.  As there, one workgroup of 256 threads per slot: thread t
+// adds positions t, t + 256, ... of the slot's list in that order, each wave
+// sums its threads with reduceWaveD, and thread 0 adds the four wave totals
+// in wave order, so the sums are the same bits.  The positions of shift s
+// are found chunk by chunk (256 entries): a ballot per wave, the waves'
+// match counts through LDS, and each match handed to the thread of its
+// position.
+constexpr int c_slotThreads = 256;
+
+template<int N>
+__device__ __forceinline__ void finishSlotSum(double (&acc)[N], double* out)
+{
+    __shared__ double sWave[c_slotThreads / 64][N];
+#pragma unroll
+    for (int c = 0; c < N; c++)
+    {
+        acc[c] = reduceWaveD(acc[c]);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+#pragma unroll
+        for (int c = 0; c < N; c++)
+        {
+            sWave[threadIdx.x >> 6][c] = acc[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int c = 0; c < N; c++)
+        {
+            double t = sWave[0][c];
+            for (int w = 1; w < c_slotThreads / 64; w++)
+            {
+                t += sWave[w][c];
+            }
+            out[c] = t;
+        }
+    }
+}
+
+__global__ __launch_bounds__(c_slotThreads) void reduceEntriesInOrderKernel(const double* partials,
+                                                                            const nbnxm_sci_t* sci,
+                                                                            int nsci,
+                                                                            int computeEnergy,
+                                                                            int computeFshift,
+                                                                            double* energies,
+                                                                            double* fshift)
 {
     const int slot = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     if (slot == 0)
     {
         if (!computeEnergy)
         {
             return;
         }
-        double e0 = 0.0, e1 = 0.0;
-        for (int k = lane; k < nsci; k += 64)
+        double acc[2] = { 0.0, 0.0 };
+        for (int k = tid; k < nsci; k += c_slotThreads)
         {
-            e0 += partials[8 * static_cast<size_t>(k)];
-            e1 += partials[8 * static_cast<size_t>(k) + 1];
+            acc[0] += partials[8 * static_cast<size_t>(k)];
+            acc[1] += partials[8 * static_cast<size_t>(k) + 1];
         }
-        e0 = reduceWaveD(e0);
-        e1 = reduceWaveD(e1);
-        if (lane == 0)
-        {
-            energies[0] = e0;
-            energies[1] = e1;
-        }
+        finishSlotSum(acc, energies);
         return;
     }
     if (!computeFshift)
@@ -121,47 +154,53 @@ __global__ __launch_bounds__(64) void reduceEntriesInOrderKernel(const double* p
         return;
     }
     const int s = slot - 1;
-    __shared__ double sv[3][64];
-    __shared__ int sFilled[64];
-    sFilled[lane] = 0;
+    const int wave = tid >> 6;
+    __shared__ double sv[3][c_slotThreads];
+    __shared__ int sFilled[c_slotThreads];
+    __shared__ int sWaveMatches[c_slotThreads / 64];
+    sFilled[tid] = 0;
     __syncthreads();
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    int count = 0; // positions of shift s before this chunk (wave-uniform)
-    for (int base = 0; base < nsci; base += 64)
+    double acc[3] = { 0.0, 0.0, 0.0 };
+    int count = 0; // positions of shift s before this chunk (uniform)
+    for (int base = 0; base < nsci; base += c_slotThreads)
     {
-        const int e = base + lane;
+        const int e = base + tid;
         const bool match = e < nsci && sci[e].shift == s;
         const unsigned long long m = __builtin_amdgcn_ballot_w64(match);
+        if ((tid & 63) == 0)
+        {
+            sWaveMatches[wave] = __popcll(m);
+        }
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < c_slotThreads / 64; w++)
+        {
+            before += (w < wave) ? sWaveMatches[w] : 0;
+            total += sWaveMatches[w];
+        }
         if (match)
         {
             const int rank = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(m >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(m), 0u)));
-            const int t = (count + rank) & 63; // at most 64 matches per chunk: distinct slots
+            // At most c_slotThreads matches per chunk: distinct threads.
+            const int t = (count + before + rank) & (c_slotThreads - 1);
             sv[0][t] = partials[8 * static_cast<size_t>(e) + 2];
             sv[1][t] = partials[8 * static_cast<size_t>(e) + 3];
             sv[2][t] = partials[8 * static_cast<size_t>(e) + 4];
             sFilled[t] = 1;
         }
         __syncthreads();
-        if (sFilled[lane])
+        if (sFilled[tid])
         {
-            f0 += sv[0][lane];
-            f1 += sv[1][lane];
-            f2 += sv[2][lane];
-            sFilled[lane] = 0;
+            acc[0] += sv[0][tid];
+            acc[1] += sv[1][tid];
+            acc[2] += sv[2][tid];
+            sFilled[tid] = 0;
         }
+        count += total;
         __syncthreads();
-        count += __popcll(m);
     }
-    f0 = reduceWaveD(f0);
-    f1 = reduceWaveD(f1);
-    f2 = reduceWaveD(f2);
-    if (lane == 0)
-    {
-        fshift[3 * s + 0] = f0;
-        fshift[3 * s + 1] = f1;
-        fshift[3 * s + 2] = f2;
-    }
+    finishSlotSum(acc, fshift + 3 * s);
 }
 
 } // namespace
@@ -286,7 +325,7 @@ extern "C" int nbnxm_gpu_launch_kernel_noalloc(const nbnxm_interaction_const_t* 
     NBNXM_HIP_CHECK(launchNonbonded(a, accumulation, in->vdwType, computeEnergy != 0, cfg, st));
     if (computeEnergy || computeFshift)
     {
-        hipLaunchKernelGGL(reduceEntriesInOrderKernel, dim3(1 + NBNXM_NUM_SHIFTS), dim3(64), 0, st, a.entryPartials, in->sci, in->nsci,
+        hipLaunchKernelGGL(reduceEntriesInOrderKernel, dim3(1 + NBNXM_NUM_SHIFTS), dim3(c_slotThreads), 0, st, a.entryPartials, in->sci, in->nsci,
                            computeEnergy ? 1 : 0, computeFshift ? 1 : 0, d_energies, d_fshift);
         NBNXM_HIP_CHECK(hipGetLastError());
     }
