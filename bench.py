@@ -119,7 +119,8 @@ def fresh_list_table(launches):
     rlistOuter inside the interaction kernel, then the entries are sorted) or
     the prune kernel followed by the interaction kernel on the pruned, sorted
     list (dynamic pruning with rlistInner = rlistOuter).  Wall clock per
-    fresh launch (upload of the list excluded) over `launches` uploads."""
+    fresh launch up to its completion (upload of the list and the force copy
+    excluded) over `launches` uploads."""
     import nbnxm_amd as nb
 
     s, search, plist, atoms = water96k_setup(nb, nb.VDW_CUT_COMB_GEOM)
@@ -132,7 +133,7 @@ def fresh_list_table(launches):
                 task.init_pairlist(plist)
                 t0 = time.perf_counter()
                 task.launch(energy, False)
-                task.fetch(want_double=False)
+                task.wait()
                 if k >= 2:
                     times.append(time.perf_counter() - t0)
             print(json.dumps({"fresh_list": "combined prune kernel" if combined else "prune kernel + interaction kernel",
@@ -168,11 +169,12 @@ def placement_table(launches):
                     task.init_pairlist(plist)
                     for _ in range(3):
                         task.launch(energy, False)
-                    task.fetch()
+                    task.wait()
+                    # Launches only: no force copy in the timed loop.
                     t0 = time.perf_counter()
                     for _ in range(launches):
                         task.launch(energy, False)
-                    task.fetch()
+                    task.wait()
                     ms[mode].append((time.perf_counter() - t0) / launches * 1e3)
                     del task
                 print(json.dumps({"placement": name, "atoms": s.n, "entries": int(len(plist.sci)), "energies": energy,
@@ -181,34 +183,47 @@ def placement_table(launches):
 
 
 def flavour_table(launches):
-    """One JSON line per interaction flavour and F / F+E: one evaluation of the
-    96,000-atom water box (rc 1.0 nm, list 1.1 nm, fixed point) per launch,
-    wall clock over `launches` launches after warm-up."""
+    """One JSON line per interaction flavour, list and F / F+E: one evaluation
+    of the 96,000-atom water box (rc 1.0 nm, list 1.1 nm, fixed point) per
+    launch, wall clock over `launches` launches after warm-up.  Two lists: the
+    unsplit one (2,539 entries) and the one split toward min_ci_balanced
+    entries, as a search handed the library's `gpu_min_ci_balanced` builds it
+    (target max(36, cluster pairs / min_ci_balanced), the simplified
+    get_nsubpair_target of the MD step and the calculator)."""
     import nbnxm_amd as nb
 
     geom = water96k_setup(nb, nb.VDW_CUT_COMB_GEOM)
     table = water96k_setup(nb, nb.VDW_CUT_TYPE_TABLE)
     useful = geom[1].count_pairs(RC)
+    probe = nb.GpuTask(nb.interaction_const(RC, EWALD_RTOL), geom[3], RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
+    min_ci = int(probe._lib.nbnxm_gpu_min_ci_balanced(probe._h))
+    del probe
+    lists = {}
+    for vdw, setup in ((nb.VDW_CUT_COMB_GEOM, geom), (nb.VDW_CUT_TYPE_TABLE, table)):
+        target = max(36, -(-setup[2].num_cluster_pairs() // min_ci))
+        lists[vdw] = {"unsplit": setup[2:], "split": nb.build(setup[0], RLIST_OUTER, vdw, target)[1:]}
     for name, kw in FLAVOUR_OPTIONS.items():
-        s, search, plist, atoms = table if kw.get("lj_pme") else geom
-        ic = nb.interaction_const(RC, EWALD_RTOL) if not kw else nb.interaction_const_ex(RC, ewald_rtol=EWALD_RTOL, **kw)
-        for energy in (False, True):
-            task = nb.GpuTask(ic, atoms, RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
-            task.init_pairlist(plist)
-            for _ in range(3):
-                task.launch(energy, False)
-            task.fetch()
-            t0 = time.perf_counter()
-            for _ in range(launches):
-                task.launch(energy, False)
-            task.fetch()
-            ms = (time.perf_counter() - t0) / launches * 1e3
-            flops = FLAVOUR_WEIGHTS[name][1 if energy else 0]
-            print(json.dumps({"flavour": name, "energies": energy, "ms_per_launch": ms,
-                              "gpairs_per_s": useful / ms / 1e6, "useful_pairs": useful,
-                              "algorithmic_tflops": useful * flops / (ms * 1e-3) / 1e12, "flops_per_pair": flops,
-                              "lj": "type table" if kw.get("lj_pme") else "geometric"}), flush=True)
-            del task
+        for split in ("unsplit", "split"):
+            plist, atoms = lists[nb.VDW_CUT_TYPE_TABLE if kw.get("lj_pme") else nb.VDW_CUT_COMB_GEOM][split]
+            ic = nb.interaction_const(RC, EWALD_RTOL) if not kw else nb.interaction_const_ex(RC, ewald_rtol=EWALD_RTOL, **kw)
+            for energy in (False, True):
+                task = nb.GpuTask(ic, atoms, RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
+                task.init_pairlist(plist)
+                for _ in range(3):
+                    task.launch(energy, False)
+                task.wait()
+                # Launches only: no force copy in the timed loop.
+                t0 = time.perf_counter()
+                for _ in range(launches):
+                    task.launch(energy, False)
+                task.wait()
+                ms = (time.perf_counter() - t0) / launches * 1e3
+                flops = FLAVOUR_WEIGHTS[name][1 if energy else 0]
+                print(json.dumps({"flavour": name, "list": split, "entries": int(len(plist.sci)), "energies": energy, "ms_per_launch": ms,
+                                  "gpairs_per_s": useful / ms / 1e6, "useful_pairs": useful,
+                                  "algorithmic_tflops": useful * flops / (ms * 1e-3) / 1e12, "flops_per_pair": flops,
+                                  "lj": "type table" if kw.get("lj_pme") else "geometric"}), flush=True)
+                del task
 
 
 def md_step_table(steps):
