@@ -196,7 +196,7 @@ def flavour_table(launches):
     table = water96k_setup(nb, nb.VDW_CUT_TYPE_TABLE)
     useful = geom[1].count_pairs(RC)
     probe = nb.GpuTask(nb.interaction_const(RC, EWALD_RTOL), geom[3], RLIST_OUTER, RLIST_INNER, nb.ACCUM_FIXED64)
-    min_ci = int(probe._lib.nbnxm_gpu_min_ci_balanced(probe._h))
+    min_ci = probe.min_ci_balanced()
     del probe
     lists = {}
     for vdw, setup in ((nb.VDW_CUT_COMB_GEOM, geom), (nb.VDW_CUT_TYPE_TABLE, table)):

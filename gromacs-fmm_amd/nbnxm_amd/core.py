@@ -330,6 +330,13 @@ class GpuTask:
         check(self._lib.nbnxm_gpu_wait_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift)), "gpu_wait_finish_task")
         return f, fd, elj.value, eel.value, fshift
 
+    def min_ci_balanced(self):
+        """gpu_min_ci_balanced: the entry count a list producer should split toward."""
+        n = self._lib.nbnxm_gpu_min_ci_balanced(self._h)
+        if n < 0:
+            check(n, "gpu_min_ci_balanced")
+        return int(n)
+
     def wait(self):
         """wait_finish_task without a force copy: (e_lj, e_el) of the last launch."""
         elj = ctypes.c_double(0.0)
