@@ -337,14 +337,6 @@ class GpuTask:
             check(n, "gpu_min_ci_balanced")
         return int(n)
 
-    def wait(self):
-        """wait_finish_task without a force copy: (e_lj, e_el) of the last launch."""
-        elj = ctypes.c_double(0.0)
-        eel = ctypes.c_double(0.0)
-        fshift = np.zeros(3 * NUM_SHIFTS, dtype=np.float64)
-        check(self._lib.nbnxm_gpu_wait_finish_task(self._h, ctypes.byref(elj), ctypes.byref(eel), ptr(fshift)), "gpu_wait_finish_task")
-        return elj.value, eel.value
-
     # GPU buffer ops: device pointers (ints) to float32 arrays in atom order.
     def init_x_to_nbat_x(self, num_atoms, atom_indices):
         ai = np.ascontiguousarray(atom_indices, dtype=np.int32)
