@@ -204,7 +204,6 @@ __global__ __launch_bounds__(64) void buildCellsKernel(const int* sortedAtoms,
                                                        int numCols,
                                                        const int* colSysStart,
                                                        int numSystems,
-                                                       int numCells,
                                                        int* atomIndex,
                                                        int* cellOfAtom,
                                                        int* numClustersInCell,
@@ -212,8 +211,10 @@ __global__ __launch_bounds__(64) void buildCellsKernel(const int* sortedAtoms,
                                                        float* clusterBb,
                                                        float* cellBb)
 {
+    // Launched for an upper bound of the cell count; the count itself is
+    // colCellStart[numCols].
     const int cell = blockIdx.x;
-    if (cell >= numCells)
+    if (cell >= colCellStart[numCols])
     {
         return;
     }
@@ -358,7 +359,7 @@ struct SearchParams
 {
     const SysGrid* grids;
     ShiftTable shifts;
-    int numCells;
+    const int* numCells;     // the cell count (device; tasks are launched for an upper bound)
     float rlist;
     float rlist2;
     const int* colCellStart;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(64) void searchKernel(const SearchParams P)
     const int task = blockIdx.x;
     const int icell = task / c_numHalfShifts;
     const int sIdx = task % c_numHalfShifts;
-    if (icell >= P.numCells)
+    if (icell >= *P.numCells)
     {
         return;
     }
@@ -860,8 +861,9 @@ using namespace nbnxm_amd;
 
 // After the scans of the per-task counts: the list totals (groups, exclusion
 // masks less the all-ones one, entries), the error flag, and the first group,
-// mask and entry of every system (its first task's offsets), in one small
-// array for one copy to the host.
+// mask, entry and cell of every system (its first task's offsets; the entry
+// after the last system holds the totals, the cell count among them), in one
+// small array for one copy to the host.
 __global__ void searchSummaryKernel(const int* groupOffset,
                                     const int* exclOffset,
                                     const int* sciOffset,
@@ -882,10 +884,12 @@ __global__ void searchSummaryKernel(const int* groupOffset,
     }
     if (k <= numSystems)
     {
-        const int t = min(colCellStart[colSysStart[k]] * c_numHalfShifts, numTasks);
-        summary[4 + 3 * k + 0] = groupOffset[t];
-        summary[4 + 3 * k + 1] = exclOffset[t];
-        summary[4 + 3 * k + 2] = sciOffset[t];
+        const int cellStart = colCellStart[colSysStart[k]];
+        const int t = min(cellStart * c_numHalfShifts, numTasks);
+        summary[4 + 4 * k + 0] = groupOffset[t];
+        summary[4 + 4 * k + 1] = exclOffset[t];
+        summary[4 + 4 * k + 2] = sciOffset[t];
+        summary[4 + 4 * k + 3] = cellStart;
     }
 }
 
@@ -1048,31 +1052,26 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     NBNXM_HIP_CHECK(hipGetLastError());
     NBNXM_HIP_CHECK(exclusiveScan(s, s->colCount.ptr, s->colAtomStart.ptr, ncol + 1));
     NBNXM_HIP_CHECK(exclusiveScan(s, s->colCells.ptr, s->colCellStart.ptr, ncol + 1));
-    std::vector<int> hColCellStart(ncol + 1);
-    NBNXM_HIP_CHECK(hipMemcpyAsync(hColCellStart.data(), s->colCellStart.ptr, sizeof(int) * (ncol + 1), hipMemcpyDeviceToHost, st));
-    NBNXM_HIP_CHECK(hipStreamSynchronize(st));
-    const int numCells = hColCellStart[ncol];
-    s->numCells = numCells;
-    s->sysCellStart.assign(numSystems + 1, 0);
-    for (int k = 0; k <= numSystems; k++)
-    {
-        s->sysCellStart[k] = hColCellStart[s->sysColStart[k]];
-    }
+    // The cell count stays on the device until the list sizes come back:
+    // cells, tasks and their buffers are sized for an upper bound (a column
+    // of n atoms has ceil(n / 64) cells), and the kernels skip the cells and
+    // tasks beyond colCellStart[ncol].
+    const int maxCells = ncol + (N + c_cellAtoms - 1) / c_cellAtoms;
 
     // 3. cells and clusters.
-    NBNXM_HIP_CHECK(s->atomIndex.resize(static_cast<size_t>(numCells) * c_cellAtoms));
+    NBNXM_HIP_CHECK(s->atomIndex.resize(static_cast<size_t>(maxCells) * c_cellAtoms));
     NBNXM_HIP_CHECK(s->cellOfAtom.resize(N));
-    NBNXM_HIP_CHECK(s->numClustersInCell.resize(numCells));
-    NBNXM_HIP_CHECK(s->cellSys.resize(numCells));
-    NBNXM_HIP_CHECK(s->clusterBb.resize(6 * static_cast<size_t>(numCells) * c_ncps));
-    NBNXM_HIP_CHECK(s->cellBb.resize(6 * static_cast<size_t>(numCells)));
-    hipLaunchKernelGGL(buildCellsKernel, dim3(numCells), dim3(64), 0, st, s->valsSorted.ptr, s->xw.ptr, s->colAtomStart.ptr, s->colCellStart.ptr, ncol,
-                       s->colSysStart.ptr, numSystems, numCells, s->atomIndex.ptr, s->cellOfAtom.ptr, s->numClustersInCell.ptr, s->cellSys.ptr,
+    NBNXM_HIP_CHECK(s->numClustersInCell.resize(maxCells));
+    NBNXM_HIP_CHECK(s->cellSys.resize(maxCells));
+    NBNXM_HIP_CHECK(s->clusterBb.resize(6 * static_cast<size_t>(maxCells) * c_ncps));
+    NBNXM_HIP_CHECK(s->cellBb.resize(6 * static_cast<size_t>(maxCells)));
+    hipLaunchKernelGGL(buildCellsKernel, dim3(maxCells), dim3(64), 0, st, s->valsSorted.ptr, s->xw.ptr, s->colAtomStart.ptr, s->colCellStart.ptr, ncol,
+                       s->colSysStart.ptr, numSystems, s->atomIndex.ptr, s->cellOfAtom.ptr, s->numClustersInCell.ptr, s->cellSys.ptr,
                        s->clusterBb.ptr, s->cellBb.ptr);
     NBNXM_HIP_CHECK(hipGetLastError());
 
     // 4. search: count, scan, write.
-    const int numTasks = numCells * c_numHalfShifts;
+    const int numTasks = maxCells * c_numHalfShifts;
     SearchParams P{};
     P.grids = s->grids.ptr;
     {
@@ -1095,7 +1094,7 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
             }
         }
     }
-    P.numCells = numCells;
+    P.numCells = s->colCellStart.ptr + ncol;
     P.rlist = static_cast<float>(rlist);
     P.rlist2 = P.rlist * P.rlist;
     P.colCellStart = s->colCellStart.ptr;
@@ -1128,7 +1127,7 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr, s->groupOffset.ptr, stride));
     NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr + stride, s->exclOffset.ptr, stride));
     NBNXM_HIP_CHECK(exclusiveScan(s, s->counts.ptr + 2 * stride, s->sciOffset.ptr, stride));
-    const int summarySize = 4 + 3 * (numSystems + 1);
+    const int summarySize = 4 + 4 * (numSystems + 1);
     NBNXM_HIP_CHECK(s->summary.resize(summarySize));
     hipLaunchKernelGGL(searchSummaryKernel, dim3((numSystems + 1 + 63) / 64), dim3(64), 0, st, s->groupOffset.ptr, s->exclOffset.ptr, s->sciOffset.ptr,
                        numTasks, s->colCellStart.ptr, s->colSysStart.ptr, numSystems, s->error.ptr, s->summary.ptr);
@@ -1144,16 +1143,19 @@ int runSearch(nbnxm_gpu_search* s, int numSystems, bool anyExcl, double rlist, i
     s->ngroups = hSummary[0];
     s->nexcl = hSummary[1] + 1;
     s->nsci = hSummary[2];
-    // Per-system ranges of entries, groups and exclusion masks.
+    // Per-system ranges of entries, groups, exclusion masks and cells.
     s->sysSciStart.assign(numSystems + 1, 0);
     s->sysGroupStart.assign(numSystems + 1, 0);
     s->sysExclStart.assign(numSystems + 1, 0);
+    s->sysCellStart.assign(numSystems + 1, 0);
     for (int k = 0; k <= numSystems; k++)
     {
-        s->sysGroupStart[k] = hSummary[4 + 3 * k + 0];
-        s->sysExclStart[k] = hSummary[4 + 3 * k + 1];
-        s->sysSciStart[k] = hSummary[4 + 3 * k + 2];
+        s->sysGroupStart[k] = hSummary[4 + 4 * k + 0];
+        s->sysExclStart[k] = hSummary[4 + 4 * k + 1];
+        s->sysSciStart[k] = hSummary[4 + 4 * k + 2];
+        s->sysCellStart[k] = hSummary[4 + 4 * k + 3];
     }
+    s->numCells = s->sysCellStart[numSystems];
     NBNXM_HIP_CHECK(s->sci.resize(std::max(s->nsci, 1)));
     NBNXM_HIP_CHECK(s->cjPacked.resize(std::max(s->ngroups, 1)));
     NBNXM_HIP_CHECK(s->excl.resize(s->nexcl));
