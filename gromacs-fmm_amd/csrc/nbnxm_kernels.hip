@@ -238,12 +238,19 @@ __global__ __launch_bounds__(64 * c_pruneWavesPerEntry) void nbnxmPruneKernel(co
     const int wavesPerEntry = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
     const int w = blockIdx.x;
 
-    // The part index per block, as the reference keeps it (:119-127).
-    const int part = __builtin_amdgcn_readfirstlane(a.rollingPart[w]);
-    __syncthreads();
-    if (threadIdx.x == 0)
+    // The part index per block, as the reference keeps it (:119-127).  A
+    // fresh list is pruned in one part: its index stays 0 (the counters are
+    // reset after the prune), so the fresh kernel neither reads nor writes
+    // it and its first dependent load is the entry's.
+    int part = 0;
+    if (!haveFreshList)
     {
-        a.rollingPart[w] = (part + 1) % a.numParts;
+        part = __builtin_amdgcn_readfirstlane(a.rollingPart[w]);
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            a.rollingPart[w] = (part + 1) % a.numParts;
+        }
     }
     const int numInPart = (a.nsci - part + a.numParts - 1) / a.numParts;
     if (w >= numInPart)
