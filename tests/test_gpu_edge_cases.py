@@ -143,3 +143,33 @@ def test_type_table_beyond_lds():
     assert elj == pytest.approx(elj64, rel=2e-5)
     assert eel == pytest.approx(eel64, rel=2e-5)
     assert np.abs(fshift - fs64).max() <= 1e-4 * max(1.0, np.abs(fs64).max())
+
+
+def test_dilute_gas_md_device_search(dilute_gas):
+    """The nonbonded part of MD steps on the dilute gas with the device search
+    (searches every 2 steps; every search after the first stays on the device,
+    whose entry lists the task derives on the device): zero forces and LJ
+    energy, the self Coulomb energy, and the same bits as the host search."""
+    announce("edge: dilute gas md steps, device vs host search")
+    ic = nb.interaction_const_ex(RC, ewald_rtol=RTOL)
+    params = np.asarray(dilute_gas.c6c12, dtype=np.float64) * np.array([6.0, 12.0])
+    runs = {}
+    for gpu_search in (False, True):
+        ver = nb.NonbondedVerlet(dilute_gas.types, params, dilute_gas.q, ic, nstlist=2, rlist_outer=RLIST,
+                                 excl_ranges=dilute_gas.excl_ranges, excl_elements=dilute_gas.excl_elements,
+                                 gpu_search=gpu_search)
+        d_x = nb.DeviceArray(dilute_gas.x.astype(np.float32))
+        d_f = nb.DeviceArray.zeros(dilute_gas.x.shape)
+        out = []
+        for step in range(6):
+            ver.step(step, dilute_gas.box, d_x.ptr, d_f.ptr, 0, compute_energy=True, compute_virial=True)
+            out.append((d_f.download(), ver.energies(), ver.virial()))
+        assert ver.stats()["searches"] == 3
+        runs[gpu_search] = out
+    expect = self_energy(ic, dilute_gas.q)
+    for step, (host, dev) in enumerate(zip(runs[False], runs[True])):
+        assert np.all(dev[0] == 0.0), step
+        assert dev[1][0] == 0.0, step
+        assert dev[1][1] == pytest.approx(expect, rel=1e-6), step
+        assert np.array_equal(host[0], dev[0]) and host[1] == dev[1], step
+        assert np.array_equal(np.asarray(host[2]), np.asarray(dev[2])), step
